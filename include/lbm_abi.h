@@ -1,0 +1,230 @@
+/*
+ * C-ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann hot path.
+ *
+ * The reference (Chiuzzmm/taichi-lbm) has no FFI: its operator interface is
+ * duck-typed Python objects whose @ti.kernel methods mutate Taichi fields.
+ * Each entry point below replaces one such kernel (cited file:line relative to
+ * the reference tree) or a fused sequence of them, and is bound from Python
+ * with ctypes (taichi_lbm/_hip.py).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative LBM_E* status; the
+ *    thread-local text of the last failure is returned by lbm_last_error().
+ *    Nothing is thrown or aborted across the ABI.
+ *  - All buffers are device pointers owned by the caller; the library never
+ *    allocates or frees them.  `stream` is a hipStream_t passed as void*
+ *    (NULL = the null stream).  Calls are asynchronous unless stated.
+ *  - Field layout ("plane" layout): a scalar field over the NX x NY lattice is
+ *    one plane of NX*NY values, node (i, j) at offset i*NY + j (i = x slowest,
+ *    the order of the reference's (NX, NY) Taichi fields).  Multi-valued fields
+ *    are consecutive planes `plane_stride` elements apart:
+ *      populations  plane (c*9 + k)            c = component, k = D2Q9 index
+ *      macro        planes rho_0..rho_{nc-1}, total_rho, ux, uy
+ *      vectors      plane (c*2 + axis)
+ *  - D2Q9 numbering is the reference's (lbm.py:43-45):
+ *      c = (0,0) (1,0) (0,1) (-1,0) (0,-1) (1,1) (-1,1) (-1,-1) (1,-1).
+ */
+#ifndef TAICHI_LBM_AMD_LBM_ABI_H
+#define TAICHI_LBM_AMD_LBM_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* status codes */
+#define LBM_OK 0
+#define LBM_E_INVALID (-1)  /* bad argument / unsupported combination */
+#define LBM_E_HIP (-2)      /* a HIP runtime call failed */
+
+/* collision models (CollisionEngine.py:36-451) */
+enum { LBM_BGK = 0, LBM_TRT = 1, LBM_MRT = 2 };
+
+/* population storage formats */
+enum {
+  LBM_STORE_F32 = 0,       /* f itself, fp32 */
+  LBM_STORE_F16_SHIFT = 1, /* f - w_k, IEEE half; arithmetic in fp32 */
+  LBM_STORE_BF16_SHIFT = 2 /* f - w_k, bfloat16; arithmetic in fp32 */
+};
+
+/* Shan-Chen variants (ExternalForce.py:55-145) */
+enum { LBM_SC_NONE = 0, LBM_SC_C1 = 1, LBM_SC_C2 = 2 };
+
+/* node code: one uint16 per node, compiled on the host from the ordered
+ * boundary-condition list (BoundaryCondition.py:459-483).               */
+#define LBM_CODE_BOUNCE_MASK 0x00FFu /* bit k-1 set: population k takes f2(x, opp k)   */
+#define LBM_CODE_ACTIVE 0x0100u      /* node belongs to the fluid (collision) group     */
+#define LBM_CODE_KEEP_MACRO 0x0200u  /* rho/u of this node are read by a NEEM rule      */
+#define LBM_CODE_NEEM_SHIFT 10       /* bits 10..13: NEEM rule index + 1 (0 = none)     */
+#define LBM_CODE_NEEM_MASK 0x3C00u
+
+enum { LBM_NEEM_VELOCITY = 1, LBM_NEEM_PRESSURE = 2 };
+
+/* Where the relaxation time comes from.  The per-node state (lbm_fields.tau,
+ * lbm_gfields.tau) holds BGK tau, TRT tau_sym or MRT diag[7] (= 1/tau); TRT
+ * tau_antisym is always magic/(tau_sym - 0.5) + 0.5 (CollisionEngine.py:155,
+ * 164) and the other MRT rates are diag = (0, s_bulk, 1, 0, 1, 0, 1, s, s)
+ * (:301-318), so neither needs storage.                                    */
+enum {
+  LBM_TAU_CONST = 0,     /* uniform value from initTau (lbm_lattice.tau / s_shear) */
+  LBM_TAU_FIELD = 1,     /* per-node state, read only                              */
+  LBM_TAU_POWER_LAW = 2  /* shear_rate + relaxation_time each step, state updated  */
+};
+#define LBM_MAX_NEEM 15
+#define LBM_MAX_NC 2
+
+/* One non-equilibrium-extrapolation rule.  VelocityNEEM
+ * (BoundaryCondition.py:149-166) or PressureNEEM (:209-226).             */
+typedef struct lbm_neem_rule {
+  int32_t kind;      /* LBM_NEEM_VELOCITY | LBM_NEEM_PRESSURE               */
+  int32_t direction; /* outward population index d in 1..4; x_nb = x - c_d */
+  float ux, uy;      /* velocity_value (velocity rule)                      */
+  float rho;         /* rho_value (pressure rule; written to total_rho only)*/
+  int32_t pad;
+} lbm_neem_rule;
+
+/* Lattice description and per-run physical parameters.  Parameters are read
+ * at every call (the reference freezes Python scalars at first compile).   */
+typedef struct lbm_lattice {
+  int32_t nx, ny, nc;
+  int32_t collision;  /* LBM_BGK | LBM_TRT | LBM_MRT                          */
+  int32_t storage;    /* LBM_STORE_*                                          */
+  int32_t tau_mode;   /* LBM_TAU_CONST | LBM_TAU_FIELD | LBM_TAU_POWER_LAW     */
+  int32_t shan_chen;  /* LBM_SC_*                                             */
+  int32_t n_neem;     /* number of rules in the lbm_neem_rule table           */
+  int64_t plane_stride;
+  /* relaxation (fp32 values exactly as the reference's initTau computes them)*/
+  float tau[LBM_MAX_NC];       /* BGK tau / TRT tau_sym (constant-tau runs)   */
+  float tau_anti[LBM_MAX_NC];  /* TRT tau_antisym (constant-tau runs)         */
+  float magic[LBM_MAX_NC];     /* TRT magic parameter                         */
+  float s_shear[LBM_MAX_NC];   /* MRT diag[7] = diag[8] = 1/tau               */
+  float s_bulk[LBM_MAX_NC];    /* MRT diag[1] = 1/tau_bulk                    */
+  float visc_m, visc_nm1;      /* nu_app = m * shear_rate^(n-1): m and (n - 1)
+                                  computed in double, then rounded; Newtonian: (nu, 0) */
+  float tau_min, tau_max;      /* Collision.tau_max_min bounds                */
+  float gx, gy;                /* gravity_force (force density, lbm.py:220)   */
+  float g_coh[LBM_MAX_NC * LBM_MAX_NC]; /* C1: g_coh[0]; C2: matrix row-major */
+  float psi_rho0[LBM_MAX_NC];  /* SC_psi rho0 per component (EOS.py:40-47)    */
+  int32_t pad0, pad1;
+} lbm_lattice;
+
+/* Device buffers of one lattice (see the plane layout above). */
+typedef struct lbm_fields {
+  void* f[2];           /* population ping-pong, nc*9 planes each           */
+  float* macro[2];      /* macro ping-pong, nc+3 planes each                */
+  float* tau;           /* nc planes, per-node relaxation state             */
+  float* psi[2];        /* nc planes each, Shan-Chen psi ping-pong          */
+  float* body_force;    /* nc*2 planes                                      */
+  float* sc_force;      /* nc*2 planes                                      */
+  float* shear_rate;    /* nc planes                                        */
+  const uint16_t* code; /* one node code per node                           */
+} lbm_fields;
+
+/* ---- version / errors ---------------------------------------------------*/
+int lbm_abi_version(void);
+const char* lbm_last_error(void);
+
+/* ---- fused hot path -----------------------------------------------------*/
+/* Runs `nsteps` calls of the cases' lbm_solve() (cases/Circular cylinder.py:
+ * 182-193, cases/Power Law Fluid.py:160-173, cases/Bubble or two-phase
+ * separation.py:197-212): density -> [update_psi -> psi halo -> cau_SCforce]
+ * -> force_density -> velocity -> [shear_rate -> relaxation_time] -> collide,
+ * with the boundary list of step n-1 (streaming, bounce-back, NEEM) fused in
+ * front of step n as a pull.  Launch t reads f[p], macro[p] (p = (parity +
+ * t) % 2) and writes f[1-p], macro[1-p]; the newest post-collision
+ * populations end in f[(parity + nsteps) % 2].
+ *   first_local = 1: f[parity] holds post-stream populations f (the state
+ *   after init_LBM or after lbm_finish) and the first launch reads them in
+ *   place; 0: f[parity] holds post-collision populations f2 and the first
+ *   launch streams them.
+ * The last launch also writes every node's rho/total_rho/u (and body force,
+ * SC force, shear rate where enabled), as the reference's fields hold them
+ * before the boundary pass.                                                */
+int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
+            int32_t nsteps, int32_t parity, int32_t first_local, void* stream);
+
+/* The reference's boundary pass alone (BoundaryEngine.apply_boundary_conditions,
+ * BoundaryCondition.py:481-483) applied to f2 = f[parity]: writes the
+ * post-stream f into f[1-parity] and applies the NEEM overrides of vel / rho /
+ * total_rho (BoundaryCondition.py:161-163, 222-223) to macro[parity].        */
+int lbm_finish(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
+               int32_t parity, void* stream);
+
+/* ---- reductions (FP64) --------------------------------------------------*/
+/* out[c] = sum over active nodes of rho_c; out[nc], out[nc+1] = sum of
+ * total_rho*u (momentum); out[nc+2] = number of non-finite values among
+ * rho/u.  Reads macro planes; synchronous on `stream`.                     */
+int lbm_reduce_macro(const lbm_lattice* lat, const float* macro, const uint16_t* code,
+                     double* out, void* stream);
+/* Residual of cases/Power Law Fluid.py:183-192: returns sum |u - u_old|^2
+ * over all nodes and then copies u into u_old (2 planes each).            */
+int lbm_residual(int64_t n_nodes, int64_t plane_stride, const float* vel, float* vel_old,
+                 double* out, void* stream);
+
+/* ---- granular kernels: one per reference @ti.kernel ----------------------
+ * Fields use the plane layout; `group` is an int32 (count, 2) list of (i, j)
+ * in device memory.  These reproduce the unfused reference sequence.       */
+typedef struct lbm_gfields {
+  float* f;            /* post-stream populations (lb_field.f)   */
+  float* f2;           /* post-collision populations (lb_field.f2) */
+  float* rho;          /* nc planes */
+  float* total_rho;    /* 1 plane   */
+  float* vel;          /* 2 planes  */
+  float* body_force;   /* nc*2 planes */
+  float* sc_force;     /* nc*2 planes */
+  float* shear_rate;   /* nc planes */
+  float* pressure;     /* nc planes */
+  float* total_pressure; /* 1 plane */
+  float* temperature;  /* 1 plane (lb_field.T) */
+  float* psi;          /* nc planes of (NX+2) x (NY+2), psi(-1,-1) at offset 0 */
+  const int32_t* neighbor; /* (NX, NY, 9, 2) int32, lbm.py:50 layout */
+  float* tau;          /* nc planes: BGK tau / TRT tau_sym / MRT diag[7] */
+} lbm_gfields;
+
+/* LBField.init_LBM, lbm.py:119-125 */
+int lbm_g_init_lbm(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* MacroscopicEngine.density, lbm.py:153-164 */
+int lbm_g_density(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* MacroscopicEngine.pressure0/1/2, lbm.py:166-212 (variant from lat->shan_chen and nc) */
+int lbm_g_pressure(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* MacroscopicEngine.force_density, lbm.py:215-220 */
+int lbm_g_force_density(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* MacroscopicEngine.velocity, lbm.py:223-237 */
+int lbm_g_velocity(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* {BGK,TRT,MRT}Collision.shear_rate, CollisionEngine.py:80-92, 166-178, 332-377 */
+int lbm_g_shear_rate(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* {BGK,TRT,MRT}Collision.relaxation_time, CollisionEngine.py:72-78, 157-164, 322-329 */
+int lbm_g_relaxation_time(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* {BGK,TRT,MRT}Collision.apply, CollisionEngine.py:96-110, 223-245, 424-451 */
+int lbm_g_collide(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+
+/* boundary kinds for lbm_g_boundary */
+enum {
+  LBM_BC_PERIODIC = 1,       /* PeriodicAllBoundary.apply, BoundaryCondition.py:424-432 */
+  LBM_BC_INSIDE = 2,         /* InsideBoundary.apply, :440-449 (-1 entries are skipped) */
+  LBM_BC_BOUNCEBACK = 3,     /* BounceBackWall.apply, :244-259 */
+  LBM_BC_VELOCITY_NEEM = 4,  /* VelocityNEEM.apply, :154-166 */
+  LBM_BC_PRESSURE_NEEM = 5   /* PressureNEEM.apply, :215-226 */
+};
+int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
+                   const int32_t* group, int32_t count, void* stream);
+
+/* ShanChenForceC1/C2.update_psi, ExternalForce.py:70-75, 114-120 */
+int lbm_g_sc_update_psi(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+/* PeriodicPsi_LR (which & 1) then PeriodicPsi_BT (which & 2), SCBoundaryCondition.py:126-144 */
+int lbm_g_sc_periodic_psi(const lbm_lattice* lat, const lbm_gfields* g, int32_t which, void* stream);
+/* ShanChenForceC1/C2.cau_SCforce, ExternalForce.py:78-91, 124-145 */
+int lbm_g_sc_force(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+
+/* ---- memory-bandwidth probe -----------------------------------------------*/
+/* Plain fp32 copy of n elements (the achievable-bandwidth reference used to
+ * scale HBM counters and to quote a measured copy ceiling).               */
+int lbm_copy_f32(const float* src, float* dst, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TAICHI_LBM_AMD_LBM_ABI_H */

@@ -1,0 +1,297 @@
+"""Node grouping and boundary conditions -- drop-in for
+taichi_lbm/BoundaryCondition.py of the reference.
+
+Geometry predicates are evaluated on the host.  A predicate is called once
+with broadcastable integer index arrays ``I`` (NX, 1) and ``J`` (1, NY) and
+must return a boolean array that broadcasts to (NX, NY); predicates written
+for scalars (Python ``if``) are evaluated node by node instead.  Field reads inside predicates are
+bounds-checked (the reference's read out of range silently).
+"""
+from __future__ import annotations
+
+import ctypes
+import numpy as np
+
+from . import _hip as H
+from ._codes import BCSpec, UNKNOWN
+from ._device import _Scalar, device, stream, torch
+from .lbm import gfields, group_device, lattice_struct
+
+
+class _GroupNodes:
+    """The ``group.group`` vector field of MaskAndGroup (BoundaryCondition.py:11)."""
+
+    def __init__(self, owner):
+        self._o = owner
+
+    @property
+    def shape(self):
+        return (self._o.capacity,)
+
+    def to_numpy(self):
+        out = np.zeros((self._o.capacity, 2), np.int32)
+        out[:len(self._o.nodes)] = self._o.nodes
+        return out
+
+    def __getitem__(self, m):
+        if not 0 <= m < self._o.capacity:
+            raise IndexError(m)
+        return tuple(self.to_numpy()[m])
+
+
+class MaskAndGroup:
+    """BoundaryCondition.py:8-12: a node list of capacity NX*NY plus a count.
+
+    The reference compacts with atomic_add, so its order is arbitrary; here the
+    nodes are in (i, j) lexicographic order.
+    """
+
+    def __init__(self, NX, NY, nodes=None):
+        self.NX, self.NY = int(NX), int(NY)
+        self.capacity = self.NX * self.NY
+        self.nodes = np.zeros((0, 2), np.int32) if nodes is None else np.asarray(nodes, np.int32).reshape(-1, 2)
+        self.count = _Scalar(len(self.nodes))
+        self.group = _GroupNodes(self)
+        self._dev = None
+
+    def device_nodes(self):
+        if len(self.nodes) == 0:
+            return None
+        if self._dev is None:
+            self._dev = torch().as_tensor(np.ascontiguousarray(self.nodes), device=device())
+        return self._dev
+
+    def flat(self):
+        return self.nodes[:, 0].astype(np.int64) * self.NY + self.nodes[:, 1]
+
+
+class BoundarySpec:
+    """BoundaryCondition.py:14-20."""
+
+    def __init__(self, geometry_fn, direction=None, value_fn=None):
+        self.geometry_fn = geometry_fn
+        self.direction = direction
+        self.value_fn = value_fn
+
+
+def evaluate_predicate(fn, NX: int, NY: int) -> np.ndarray:
+    """Boolean (NX, NY) map of a geometry predicate (vectorised, else per node)."""
+    I = np.arange(NX)[:, None]
+    J = np.arange(NY)[None, :]
+    try:
+        res = np.broadcast_to(np.asarray(fn(I, J)), (NX, NY))
+        return res.astype(bool)
+    except (ValueError, TypeError):
+        out = np.zeros((NX, NY), bool)
+        for i in range(NX):
+            for j in range(NY):
+                out[i, j] = bool(fn(i, j))
+        return out
+
+
+class BoundaryClassifier:
+    """BoundaryCondition.py:22-39.  ``shape`` is any object with ``.shape`` (the
+    reference passes a Taichi field) or an (NX, NY) tuple."""
+
+    def __init__(self, shape):
+        self.shape = shape
+        sh = shape.shape if hasattr(shape, "shape") else tuple(shape)
+        self.NX, self.NY = int(sh[0]), int(sh[1])
+
+    def get_group(self, spec: BoundarySpec) -> MaskAndGroup:
+        hit = evaluate_predicate(spec.geometry_fn, self.NX, self.NY)
+        return MaskAndGroup(self.NX, self.NY, np.argwhere(hit).astype(np.int32))
+
+
+class BoundaryCondition:
+    """BoundaryCondition.py:41-54."""
+
+    kind = "fluid"
+
+    def __init__(self, spec: BoundarySpec):
+        self.spec = spec
+        self.group = None
+
+    def precompute(self, classifier: BoundaryClassifier):
+        self.group = classifier.get_group(self.spec)
+
+    def apply(self, lb_field):
+        pass
+
+    def fused_spec(self, name="") -> BCSpec:
+        return BCSpec(self.kind, self.group.nodes, name=name)
+
+    def _granular(self, lb_field, kind, rule=None):
+        lb_field.materialize()
+        g = group_device(self.group)
+        need_nb = kind in (H.BC_INSIDE, H.BC_BOUNCEBACK)
+        H.check(H.lib().lbm_g_boundary(ctypes.byref(lattice_struct(lb_field)),
+                                       ctypes.byref(gfields(lb_field, neighbor=need_nb)),
+                                       kind, ctypes.byref(rule) if rule is not None else None, g[0], g[1],
+                                       stream()), type(self).__name__)
+
+
+class PlaneBoundary(BoundaryCondition):
+    """BoundaryCondition.py:57-72."""
+
+    def __init__(self, spec: BoundarySpec, direction):
+        super().__init__(spec)
+        self.direction = direction
+        self.unknow = list(UNKNOWN[direction]) if direction in UNKNOWN else None
+
+
+class VelocityBoundary(PlaneBoundary):
+    def __init__(self, spec, velocity_value, direction):
+        super().__init__(spec, direction)
+        self.velocity_value = tuple(float(v) for v in velocity_value)
+
+
+class PressureBoundary(PlaneBoundary):
+    def __init__(self, spec, rho_value, direction):
+        super().__init__(spec, direction)
+        self.rho_value = float(rho_value)
+
+
+class VelocityNEEM(VelocityBoundary):
+    """BoundaryCondition.py:149-166."""
+
+    kind = "velocity_neem"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = H.NEEM_VELOCITY, int(self.direction)
+        r.ux, r.uy = (float(np.float32(v)) for v in self.velocity_value)
+        return r
+
+    def fused_spec(self, name=""):
+        ux, uy = (float(np.float32(v)) for v in self.velocity_value)
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), ux, uy, 0.0, name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_VELOCITY_NEEM, self.rule())
+
+
+class PressureNEEM(PressureBoundary):
+    """BoundaryCondition.py:209-226."""
+
+    kind = "pressure_neem"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = H.NEEM_PRESSURE, int(self.direction)
+        r.rho = float(np.float32(self.rho_value))
+        return r
+
+    def fused_spec(self, name=""):
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), 0.0, 0.0,
+                      float(np.float32(self.rho_value)), name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_PRESSURE_NEEM, self.rule())
+
+
+class BounceBackWall(BoundaryCondition):
+    """BoundaryCondition.py:231-259 (halfway bounce-back through the neighbour table)."""
+
+    kind = "bounceback"
+
+    def __init__(self, spec: BoundarySpec):
+        super().__init__(spec)
+        self.bounce_map = np.array([0, 3, 4, 1, 2, 7, 8, 5, 6])
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_BOUNCEBACK)
+
+
+class PeriodicAllBoundary(BoundaryCondition):
+    """BoundaryCondition.py:419-432 (periodic pull streaming)."""
+
+    kind = "periodic"
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_PERIODIC)
+
+
+class InsideBoundary(BoundaryCondition):
+    """BoundaryCondition.py:435-449 (pull streaming through the neighbour table)."""
+
+    kind = "inside"
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_INSIDE)
+
+
+class FluidBoundary(BoundaryCondition):
+    """BoundaryCondition.py:452-456 (no-op entry holding the fluid group)."""
+
+    kind = "fluid"
+
+
+def _out_of_scope(name, where):
+    def init(self, *a, **k):
+        raise NotImplementedError(
+            f"{name} ({where}) is not part of the MI355X hot path; supported boundaries: PeriodicAllBoundary, "
+            "InsideBoundary, BounceBackWall, VelocityNEEM, PressureNEEM, FluidBoundary")
+    return type(name, (BoundaryCondition,), {"__init__": init})
+
+
+VelocityBB = _out_of_scope("VelocityBB", "BoundaryCondition.py:80-102")
+VelocityEquilibrium = _out_of_scope("VelocityEquilibrium", "BoundaryCondition.py:132-147")
+PressureABC = _out_of_scope("PressureABC", "BoundaryCondition.py:177-205")
+OpenNeumann = _out_of_scope("OpenNeumann", "BoundaryCondition.py:315-334")
+OpenConvective1order = _out_of_scope("OpenConvective1order", "BoundaryCondition.py:337-359")
+OpenConvective2order = _out_of_scope("OpenConvective2order", "BoundaryCondition.py:362-390")
+OpenExtrapolation = _out_of_scope("OpenExtrapolation", "BoundaryCondition.py:394-415")
+
+
+class BoundaryEngine:
+    """BoundaryCondition.py:459-483: an ordered registry applied in insertion order."""
+
+    def __init__(self):
+        self.boundary_conditions = {}
+
+    def Mask_rectangle_identify(self, lb_field, xmin, xmax, ymin, ymax):
+        """BoundaryCondition.py:464-469 (strict inequalities, f32 bounds)."""
+        f32 = np.float32
+        I, J = np.meshgrid(np.arange(lb_field.NX), np.arange(lb_field.NY), indexing="ij")
+        hit = (I < f32(xmax)) & (I > f32(xmin)) & (J < f32(ymax)) & (J > f32(ymin))
+        m = lb_field.mask.arr
+        m[hit] = -1
+        lb_field._geometry_changed()
+
+    def Mask_cricle_identify(self, lb_field, x, y, r):
+        """BoundaryCondition.py:471-475: (i-x)^2 + (j-y)^2 < r^2 in f32."""
+        f32 = np.float32
+        I, J = np.meshgrid(np.arange(lb_field.NX), np.arange(lb_field.NY), indexing="ij")
+        dx = I.astype(f32) - f32(x)
+        dy = J.astype(f32) - f32(y)
+        hit = dx * dx + dy * dy < f32(r) * f32(r)
+        lb_field.mask.arr[hit] = -1
+        lb_field._geometry_changed()
+
+    Mask_circle_identify = Mask_cricle_identify   # README.md:105 spelling
+
+    def add_boundary_condition(self, name, bc):
+        """BoundaryCondition.py:477-479 (a repeated name replaces the entry in place)."""
+        self.boundary_conditions[name] = bc
+        print(f"  add boundary {name}, size: {bc.group.count[None]}")
+
+    def apply_boundary_conditions(self, lb_field):
+        """BoundaryCondition.py:481-483 (granular: one kernel per entry)."""
+        for bc in self.boundary_conditions.values():
+            bc.apply(lb_field)
+
+    def fused_list(self):
+        return [bc.fused_spec(name) for name, bc in self.boundary_conditions.items()]
+
+    def writing_boundary(self, lb_field, path="boundary.png"):
+        """BoundaryCondition.py:486-510: paints each group in insertion order into a PNG."""
+        from PIL import Image
+        colors = [(0.0, 0.0, 0.0), (52 / 255, 152 / 255, 219 / 255), (1.0, 0.0, 0.0),
+                  (143 / 255, 24 / 255, 172 / 255), (255 / 255, 190 / 255, 53 / 255), (1.0, 0.0, 0.0)]
+        img = np.zeros((lb_field.NX, lb_field.NY, 3), np.float32)
+        for num, bc in enumerate(self.boundary_conditions.values()):
+            nodes = bc.group.nodes
+            img[nodes[:, 0], nodes[:, 1]] = colors[num]   # the reference indexes colors[num] too
+        Image.fromarray((img * 255).astype(np.uint8)).save(path)
+        print("writing boundary finish")

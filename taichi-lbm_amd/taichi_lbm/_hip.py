@@ -1,0 +1,126 @@
+"""ctypes binding of liblbm_hip.so (declarations: include/lbm_abi.h).
+
+Loading fails loudly: there is no CPU fallback for any kernel.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+from ._build import LIB_PATH
+
+LBM_BGK, LBM_TRT, LBM_MRT = 0, 1, 2
+STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT = 0, 1, 2
+SC_NONE, SC_C1, SC_C2 = 0, 1, 2
+TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
+NEEM_VELOCITY, NEEM_PRESSURE = 1, 2
+BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM = 1, 2, 3, 4, 5
+
+CODE_BOUNCE_MASK = 0x00FF
+CODE_ACTIVE = 0x0100
+CODE_KEEP_MACRO = 0x0200
+CODE_NEEM_SHIFT = 10
+CODE_NEEM_MASK = 0x3C00
+MAX_NEEM = 15
+MAX_NC = 2
+ABI_VERSION = 1
+
+
+class NeemRule(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("direction", C.c_int32), ("ux", C.c_float), ("uy", C.c_float),
+                ("rho", C.c_float), ("pad", C.c_int32)]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nc", C.c_int32),
+        ("collision", C.c_int32), ("storage", C.c_int32), ("tau_mode", C.c_int32),
+        ("shan_chen", C.c_int32), ("n_neem", C.c_int32),
+        ("plane_stride", C.c_int64),
+        ("tau", C.c_float * MAX_NC), ("tau_anti", C.c_float * MAX_NC), ("magic", C.c_float * MAX_NC),
+        ("s_shear", C.c_float * MAX_NC), ("s_bulk", C.c_float * MAX_NC),
+        ("visc_m", C.c_float), ("visc_nm1", C.c_float),
+        ("tau_min", C.c_float), ("tau_max", C.c_float),
+        ("gx", C.c_float), ("gy", C.c_float),
+        ("g_coh", C.c_float * (MAX_NC * MAX_NC)),
+        ("psi_rho0", C.c_float * MAX_NC),
+        ("pad0", C.c_int32), ("pad1", C.c_int32),
+    ]
+
+
+class Fields(C.Structure):
+    _fields_ = [
+        ("f", C.c_void_p * 2), ("macro", C.c_void_p * 2), ("tau", C.c_void_p),
+        ("psi", C.c_void_p * 2), ("body_force", C.c_void_p), ("sc_force", C.c_void_p),
+        ("shear_rate", C.c_void_p), ("code", C.c_void_p),
+    ]
+
+
+class GFields(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in (
+        "f", "f2", "rho", "total_rho", "vel", "body_force", "sc_force", "shear_rate", "pressure",
+        "total_pressure", "temperature", "psi", "neighbor", "tau")]
+
+
+class LBMError(RuntimeError):
+    """A failure reported by the HIP library (message from lbm_last_error)."""
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("LBM_HIP_LIB", str(LIB_PATH)))
+    if not path.exists():
+        raise RuntimeError(
+            f"HIP library {path} is missing: build it with taichi_lbm._build.build() "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(str(path))
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
+    LP, FP, NP, GP = C.POINTER(Lattice), C.POINTER(Fields), C.POINTER(NeemRule), C.POINTER(GFields)
+    sig = {
+        "lbm_abi_version": ([], C.c_int),
+        "lbm_last_error": ([], C.c_char_p),
+        "lbm_run": ([LP, FP, NP, I32, I32, I32, P], C.c_int),
+        "lbm_finish": ([LP, FP, NP, I32, P], C.c_int),
+        "lbm_reduce_macro": ([LP, P, P, D, P], C.c_int),
+        "lbm_residual": ([I64, I64, P, P, D, P], C.c_int),
+        "lbm_copy_f32": ([P, P, I64, P], C.c_int),
+        "lbm_g_boundary": ([LP, GP, I32, NP, P, I32, P], C.c_int),
+        "lbm_g_sc_periodic_psi": ([LP, GP, I32, P], C.c_int),
+    }
+    for name in ("lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density", "lbm_g_velocity",
+                 "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_sc_update_psi",
+                 "lbm_g_sc_force"):
+        sig[name] = ([LP, GP, P, I32, P], C.c_int)
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    if L.lbm_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{path}: ABI version {L.lbm_abi_version()} != {ABI_VERSION}")
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of every entry point the header declares (checked by the tests)."""
+    return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_finish", "lbm_reduce_macro", "lbm_residual",
+            "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
+            "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
+            "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force"]
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().lbm_last_error().decode(errors="replace")
+        raise LBMError(f"{what}: {msg} (status {rc})" if what else f"{msg} (status {rc})")
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (None -> NULL)."""
+    return 0 if t is None else int(t.data_ptr())

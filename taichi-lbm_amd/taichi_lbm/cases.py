@@ -1,0 +1,249 @@
+"""Workload builders: the benchmark and test configurations set up through the
+drop-in API, the way the reference's case scripts do (cases/*.py: mask ->
+groups -> boundary list -> collision -> init_hydro -> init_LBM).
+
+Every builder returns a dict with the LBField, engines, collision and the
+Stepper arguments, plus a plain description (``spec``) of what was built so a
+checker can rebuild the same case independently.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .BoundaryCondition import (BounceBackWall, BoundaryClassifier, BoundaryEngine, BoundarySpec, FluidBoundary,
+                                PeriodicAllBoundary, PressureNEEM, VelocityNEEM)
+from .CollisionEngine import BGKCollision, MRTCollision, TRTCollision
+from .FluidModels import NewtonianFluid, PowerLawFluid
+from .lbm import LBField, MacroscopicEngine
+
+C = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
+
+
+def solid_neighbour_map(mask: np.ndarray, y_walls: bool = False) -> np.ndarray:
+    """Fluid nodes with a solid neighbour x - c_k (and, with y_walls, a
+    neighbour beyond the bottom/top rows): the bounce-back wall group."""
+    from ._codes import shifted
+    NX, NY = mask.shape
+    solid = mask == -1
+    out = np.zeros((NX, NY), bool)
+    for k in range(1, 9):
+        out |= shifted(solid, int(C[k, 0]), int(C[k, 1]), False)
+        if y_walls and C[k, 1] != 0:
+            edge = np.zeros((NX, NY), bool)
+            edge[:, 0 if C[k, 1] > 0 else NY - 1] = True
+            out |= edge
+    return out & (mask == 1)
+
+
+def make_collision(kind, group, NX, NY, nc, nu, fluid_model=None, magic=0.25, nu_bulk=None):
+    params = {'group': group, 'fluid_model': fluid_model or NewtonianFluid(nu[0]), 'NX': NX, 'NY': NY,
+              'num_components': nc, 'shearviscosity': list(nu), 'magic': [magic] * nc,
+              'bulkviscosity': list(nu_bulk if nu_bulk is not None else nu)}
+    return {"bgk": BGKCollision, "trt": TRTCollision, "mrt": MRTCollision}[kind](params)
+
+
+def _finish(lbf, bce, fluid_bc, coll, rho0=1.0, u0=(0.0, 0.0)):
+    """init_hydro (rho, u on the fluid group) + init_LBM."""
+    macro = MacroscopicEngine(fluid_bc.group)
+    nodes = fluid_bc.group.nodes
+    rho = np.zeros((lbf.NX, lbf.NY, lbf.nc), np.float32)
+    vel = np.zeros((lbf.NX, lbf.NY, 2), np.float32)
+    rho[nodes[:, 0], nodes[:, 1], :] = rho0 / lbf.nc
+    vel[nodes[:, 0], nodes[:, 1], :] = u0
+    return macro, rho, vel
+
+
+def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0, collision="bgk", magic=0.25,
+             tau_bulk=None, name="cylinder"):
+    """Cylinder in a channel: bounce-back top/bottom rows and cylinder,
+    VelocityNEEM inlet (d=3) and PressureNEEM outlet (d=1), f = w at start."""
+    lbf = LBField(name, NX, NY, 1)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    bce.Mask_cricle_identify(lbf, cx, cy, r)
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    periodic.precompute(clf)
+    bce.add_boundary_condition("periodic", periodic)
+    wmap = solid_neighbour_map(m, y_walls=True)
+    wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+    wall.precompute(clf)
+    bce.add_boundary_condition("wall", wall)
+    inlet = VelocityNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1)),
+                         (u_in, 0.0), 3)
+    inlet.precompute(clf)
+    bce.add_boundary_condition("inlet", inlet)
+    outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
+                          rho_out, 1)
+    outlet.precompute(clf)
+    bce.add_boundary_condition("outlet", outlet)
+    lbf.neighbor_classify()
+    nu = (tau - 0.5) / 3.0
+    nu_b = None if tau_bulk is None else [(tau_bulk - 0.5) / 3.0]
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [nu], magic=magic, nu_bulk=nu_b)
+    macro, rho, vel = _finish(lbf, bce, fluid, coll)
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
+                spec=dict(kind="cylinder", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision, magic=magic))
+
+
+def place_disks(NX, NY, seed, rmin=8.0, rmax=32.0, solid_fraction=0.3, margin=2.0, max_tries=10_000_000):
+    """Random sequential placement of non-overlapping disks (seeded) until the
+    disk area reaches solid_fraction of the lattice; disks keep `margin` nodes
+    from the edges.  Returns an (m, 3) array of (x, y, r)."""
+    rng = np.random.default_rng(seed)
+    target = solid_fraction * NX * NY
+    cell = 2 * rmax + 2
+    gx, gy = int(math.ceil(NX / cell)), int(math.ceil(NY / cell))
+    grid = [[] for _ in range(gx * gy)]
+    disks = []
+    area = 0.0
+    tries = 0
+    while area < target and tries < max_tries:
+        tries += 1
+        r = rng.uniform(rmin, rmax)
+        x = rng.uniform(r + margin, NX - 1 - r - margin)
+        y = rng.uniform(r + margin, NY - 1 - r - margin)
+        cx_, cy_ = int(x // cell), int(y // cell)
+        ok = True
+        for ax in range(max(0, cx_ - 1), min(gx, cx_ + 2)):
+            for ay in range(max(0, cy_ - 1), min(gy, cy_ + 2)):
+                for (x2, y2, r2) in grid[ax * gy + ay]:
+                    if (x - x2) ** 2 + (y - y2) ** 2 < (r + r2 + 1.0) ** 2:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid[cx_ * gy + cy_].append((x, y, r))
+            disks.append((x, y, r))
+            area += math.pi * r * r
+    return np.array(disks, np.float64).reshape(-1, 3)
+
+
+def rasterize_disks(NX, NY, disks) -> np.ndarray:
+    """mask (1 fluid, -1 solid) with Mask_cricle_identify's f32 test per disk."""
+    mask = np.ones((NX, NY), np.int32)
+    f32 = np.float32
+    for x, y, r in disks:
+        i0, i1 = max(0, int(x - r) - 1), min(NX, int(x + r) + 2)
+        j0, j1 = max(0, int(y - r) - 1), min(NY, int(y + r) + 2)
+        I, J = np.meshgrid(np.arange(i0, i1), np.arange(j0, j1), indexing="ij")
+        dx = I.astype(f32) - f32(x)
+        dy = J.astype(f32) - f32(y)
+        hit = dx * dx + dy * dy < f32(r) * f32(r)
+        mask[i0:i1, j0:j1][hit] = -1
+    return mask
+
+
+def porous(NX=8192, NY=8192, seed=1, collision="mrt", power_n=0.7, power_m=0.01, tau_min=0.51, tau_max=2.0,
+           g=(1e-6, 0.0), nu0=1.0 / 6.0, nu_bulk=1.0 / 6.0, magic=0.25, solid_fraction=0.3, rmin=8.0, rmax=32.0,
+           name="porous"):
+    """Periodic lattice with seeded random non-overlapping solid disks, body
+    force g, power-law rheology with tau clamped to [tau_min, tau_max]."""
+    lbf = LBField(name, NX, NY, 1)
+    disks = place_disks(NX, NY, seed, rmin, rmax, solid_fraction)
+    lbf.mask.arr[...] = rasterize_disks(NX, NY, disks)
+    lbf._geometry_changed()
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    periodic.precompute(clf)
+    bce.add_boundary_condition("periodic", periodic)
+    wmap = solid_neighbour_map(m)
+    wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+    wall.precompute(clf)
+    bce.add_boundary_condition("wall", wall)
+    lbf.neighbor_classify()
+    lbf.set_gravity(g)
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [nu0], PowerLawFluid(power_m, power_n), magic=magic,
+                          nu_bulk=[nu_bulk])
+    coll.tau_min, coll.tau_max = tau_min, tau_max
+    macro, rho, vel = _finish(lbf, bce, fluid, coll)
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
+                disks=disks,
+                spec=dict(kind="porous", NX=NX, NY=NY, seed=seed, collision=collision, power_n=power_n,
+                          power_m=power_m, tau_min=tau_min, tau_max=tau_max, g=g, nu0=nu0, nu_bulk=nu_bulk,
+                          magic=magic, n_disks=len(disks)))
+
+
+def initialise(case):
+    """init_hydro + init_LBM on the device (needs a GPU)."""
+    lbf = case["lbf"]
+    lbf.rho.from_numpy(case["rho"])
+    lbf.vel.from_numpy(case["vel"])
+    lbf.init_LBM(case["coll"], case["fluid"].group)
+    if case.get("psi") is not None:
+        lbf.sc_field.psi_field.from_numpy(case["psi"])
+        lbf.T.fill(1.0)
+    return case
+
+
+def stepper(case):
+    """The fused Stepper for a built case."""
+    from .stepper import Stepper
+    return Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=case["power_law"],
+                   boundary_engine_sc=case.get("bsc"))
+
+
+def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt", nu=1.0 / 6.0, rho_mean=None,
+              amp=None, name="shan_chen"):
+    """Periodic Shan-Chen lattice.  nc=1: ShanChenForceC1 with cohesion G, rho =
+    rho_mean + amp*U(0,1) (cases/Bubble or two-phase separation.py:174);
+    nc=2: ShanChenForceC2 with G = [[0, gcross], [gcross, 0]] and each rho_c =
+    1 + amp*U(-1, 1), seeded per component.  psi = SC_psi(rho0=1)."""
+    from .EOS import SC_psi
+    from .ExternalForce import ShanChenForceC1, ShanChenForceC2
+    from .SCBoundaryCondition import BoundaryEngineSC, PeriodicPsi_BT, PeriodicPsi_LR
+    lbf = LBField(name, NX, NY, nc)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    stream_bc = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    stream_bc.precompute(clf)
+    bce.add_boundary_condition("stream", stream_bc)
+    lbf.neighbor_classify()
+    coll = make_collision(collision, fluid.group, NX, NY, nc, [nu] * nc, nu_bulk=[nu] * nc)
+    rng = np.random.default_rng(seed)
+    rho = np.zeros((NX, NY, nc), np.float32)
+    if nc == 1:
+        rho_mean = math.log(2.0) if rho_mean is None else rho_mean
+        amp = 0.1 if amp is None else amp
+        rho[..., 0] = rho_mean + amp * rng.random((NX, NY))
+        sc = ShanChenForceC1({'lb_field': lbf, 'g_coh': G, 'group': fluid.group, 'psi': SC_psi(1.0)})
+        g_spec = G
+    else:
+        amp = 0.01 if amp is None else amp
+        for c in range(nc):
+            rho[..., c] = 1.0 + amp * rng.uniform(-1.0, 1.0, (NX, NY))
+        Gm = np.array([[0.0, gcross], [gcross, 0.0]])
+        sc = ShanChenForceC2({'lb_field': lbf, 'g_coh': Gm, 'gadh': [0.0, 0.0], 'group': fluid.group,
+                              'psi_list': [SC_psi(1.0), SC_psi(1.0)], 'fluid_strategy': None})
+        g_spec = Gm
+    lbf.sc_field = sc
+    bsc = BoundaryEngineSC()
+    bsc.add_boundary_condition("periodicLR", PeriodicPsi_LR())
+    bsc.add_boundary_condition("periodicBT", PeriodicPsi_BT())
+    psi = np.zeros((NX + 2, NY + 2, nc), np.float32)
+    psi[1:-1, 1:-1, :] = SC_psi(1.0).get_psi(rho)
+    psi = np.pad(psi[1:-1, 1:-1, :], ((1, 1), (1, 1), (0, 0)), mode="wrap")
+    macro = MacroscopicEngine(fluid.group)
+    vel = np.zeros((NX, NY, 2), np.float32)
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
+                psi=psi, bsc=bsc, sc_spec=dict(kind="c1" if nc == 1 else "c2", g_coh=g_spec, rho0=(1.0,) * nc),
+                spec=dict(kind="shan_chen", NX=NX, NY=NY, nc=nc, G=G, gcross=gcross, seed=seed,
+                          collision=collision, nu=nu))
+

@@ -1,0 +1,150 @@
+"""Stepper: the fused form of a case script's ``lbm_solve()``.
+
+The reference advances a lattice with 9-14 Taichi kernels per step
+(cases/Circular cylinder.py:182-193, cases/Power Law Fluid.py:160-173,
+cases/Bubble or two-phase separation.py:197-212).  ``Stepper.run(n)`` performs
+n such steps with one HIP launch per step (two with Shan-Chen), through
+``lbm_run`` of the C-ABI, and leaves the LBField exactly as n calls of the
+granular sequence would (the final boundary pass runs lazily, on the first
+read of ``f`` / ``rho`` / ``vel``).
+
+Equivalent granular sequence (what each constructor flag selects):
+    density -> pressure -> [update_psi -> psi halo -> cau_SCforce]
+    -> force_density -> velocity -> [shear_rate -> relaxation_time]
+    -> collide -> boundary list -> [time_updata]
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _hip as H
+from ._codes import compile_codes
+from ._device import device, stream, torch
+from .lbm import gfields, group_device, lattice_struct
+
+_STORAGE = {"f32": H.STORE_F32, "f16": H.STORE_F16_SHIFT, "bf16": H.STORE_BF16_SHIFT}
+
+
+class Stepper:
+    def __init__(self, lb_field, macroscopic_engine, collision_engine, boundary_engine, power_law=False,
+                 boundary_engine_sc=None, advance_time=False, storage="f32"):
+        self.lbf = lb_field
+        self.macro = macroscopic_engine
+        self.coll = collision_engine
+        self.bce = boundary_engine
+        self.power_law = bool(power_law)
+        self.advance_time = bool(advance_time)
+        if storage != "f32":
+            raise NotImplementedError("shifted population storage is selected through BatchRunner")
+        self.storage = _STORAGE[storage]
+        self.sc = lb_field.sc_field
+        if self.sc is not None:
+            if boundary_engine_sc is None or not boundary_engine_sc.is_periodic():
+                raise NotImplementedError("the fused Shan-Chen path needs the psi boundaries "
+                                          "PeriodicPsi_LR then PeriodicPsi_BT")
+        a = {tuple(x) for x in macroscopic_engine.group.nodes.tolist()}
+        b = {tuple(x) for x in collision_engine.group.nodes.tolist()}
+        if a != b:
+            raise NotImplementedError("the macroscopic and collision groups differ")
+        self._compiled_version = None
+        self._compile()
+
+    def _compile(self):
+        lbf = self.lbf
+        code, rules = compile_codes(lbf.NX, lbf.NY, self.macro.group.nodes, self.bce.fused_list(),
+                                    lbf.neighbor_valid())
+        self.code_host = code
+        self.neem_host = rules
+        t = torch()
+        buf = np.zeros(lbf.ps, np.uint16)
+        buf[:lbf.N] = code
+        self.code = t.as_tensor(buf.view(np.int16), device=device())
+        arr = (H.NeemRule * max(1, len(rules)))()
+        for e, (kind, d, ux, uy, rho) in enumerate(rules):
+            arr[e].kind, arr[e].direction, arr[e].ux, arr[e].uy, arr[e].rho = kind, d, ux, uy, rho
+        self.neem = arr
+        self._compiled_version = lbf.geometry_version
+
+    def _tau_mode(self):
+        if self.power_law:
+            return H.TAU_POWER_LAW
+        return H.TAU_CONST if self.coll._uniform else H.TAU_FIELD
+
+    def _structs(self, tau_mode):
+        lbf = self.lbf
+        st = lbf.state()
+        L = lattice_struct(lbf, collision=self.coll, sc=self.sc, storage=self.storage, tau_mode=tau_mode,
+                           neem=self.neem_host)
+        F = H.Fields()
+        F.f[0], F.f[1] = H.ptr(st["pop"][0]), H.ptr(st["pop"][1])
+        F.macro[0], F.macro[1] = H.ptr(st["mac"][0]), H.ptr(st["mac"][1])
+        F.tau = H.ptr(self.coll.state_tensor())
+        F.body_force = H.ptr(lbf.aux("body_force"))
+        F.shear_rate = H.ptr(lbf.aux("shear_rate")) if tau_mode == H.TAU_POWER_LAW else None
+        if self.sc is not None:
+            planes = self.sc.planes_for_run()
+            F.psi[0], F.psi[1] = H.ptr(planes[0]), H.ptr(planes[1])
+            F.sc_force = H.ptr(lbf.aux("SCforce"))
+        F.code = H.ptr(self.code)
+        return L, F
+
+    def run(self, nsteps: int):
+        """Advance ``nsteps`` steps (asynchronous on the current stream)."""
+        nsteps = int(nsteps)
+        if nsteps <= 0:
+            return
+        lbf = self.lbf
+        if self._compiled_version != lbf.geometry_version:
+            self._compile()
+        st = lbf.state()
+        tau_mode = self._tau_mode()
+        L, F = self._structs(tau_mode)
+        if st["f_valid"]:
+            parity, first_local = 1 - st["f2_idx"], 1
+        else:
+            parity, first_local = st["f2_idx"], 0
+        H.check(H.lib().lbm_run(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
+                                stream()), "lbm_run")
+        st["f2_idx"] = (parity + nsteps) % 2
+        st["f_valid"] = False
+        if tau_mode == H.TAU_POWER_LAW:
+            self.coll._uniform = False
+        if self.sc is not None:
+            self.sc.after_run(st["f2_idx"])
+        if self.advance_time:
+            lbf.time[None] += nsteps
+        keep = (L, F, self.neem)
+        last = st["f2_idx"]
+
+        def finish():
+            H.check(H.lib().lbm_finish(ctypes.byref(keep[0]), ctypes.byref(keep[1]), keep[2], last, stream()),
+                    "lbm_finish")
+        lbf._finisher = finish
+
+        def refresh_pressure():
+            # MacroscopicEngine.pressure of the last step: it ran after density and
+            # before update_psi and the boundary pass, so it sees rho of that step
+            # without the NEEM overrides and psi of the step before (plane set 1 - last)
+            if self.sc is not None:
+                self.sc._halo_from_planes(1 - last)
+            g = group_device(self.macro.group)
+            H.check(H.lib().lbm_g_pressure(ctypes.byref(lattice_struct(lbf, sc=self.sc)),
+                                           ctypes.byref(gfields(lbf, sc=self.sc)), g[0], g[1], stream()),
+                    "pressure")
+            if self.sc is not None:
+                self.sc._halo_valid = False
+        lbf._pressure_refresh = refresh_pressure
+
+    # ---- FP64 diagnostics ---------------------------------------------------
+    def reduce(self):
+        """FP64 sums over the fluid group: {'mass': [per component], 'momentum': (x, y), 'nonfinite': n}."""
+        lbf = self.lbf
+        lbf.materialize()
+        L = lattice_struct(lbf)
+        out = (ctypes.c_double * (lbf.nc + 3))()
+        H.check(H.lib().lbm_reduce_macro(ctypes.byref(L), H.ptr(lbf._mac()), H.ptr(self.code), out, stream()),
+                "lbm_reduce_macro")
+        v = list(out)
+        return {"mass": v[:lbf.nc], "momentum": (v[lbf.nc], v[lbf.nc + 1]), "nonfinite": int(v[lbf.nc + 2])}

@@ -1,0 +1,53 @@
+"""Rebuilds a case made through the drop-in API as an oracle Solver with the
+same groups, parameters and initial state (test-only)."""
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import lbm_oracle as O
+
+
+def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None):
+    lbf = case["lbf"]
+    coll = case["coll"]
+    lat = O.Lattice(lbf.NX, lbf.NY, lbf.nc, dtype=dtype, gravity=lbf.gravity_force)
+    lat.mask[...] = lbf.mask.arr
+    lat.neighbor[...] = lbf.neighbor.arr
+    group = case["fluid"].group.nodes.astype(np.int64)
+    fm = coll.fluid_model
+    if type(fm).__name__ == "PowerLawFluid":
+        model = O.FluidModel("power", m=fm.m, n=fm.n)
+    else:
+        model = O.FluidModel("newtonian", nu=fm.nu)
+    kind = {0: "bgk", 1: "trt", 2: "mrt"}[coll.KIND]
+    oc = O.Collision(kind, group, model, list(coll.nu), magic=list(getattr(coll, "magic", [])) or None,
+                     nu_bulk=list(getattr(coll, "nu_bulk", [])) or None, tau_min=coll.tau_min, tau_max=coll.tau_max)
+    oc.init_tau(lat)
+    bcs = []
+    for name, bc in case["bce"].boundary_conditions.items():
+        k = bc.kind
+        val = None
+        if k == "velocity_neem":
+            val = bc.velocity_value
+        elif k == "pressure_neem":
+            val = bc.rho_value
+        bcs.append(O.BC(k, bc.group.nodes.astype(np.int64), getattr(bc, "direction", None), val))
+    dt = np.dtype(dtype)
+    lat.rho[...] = (case["rho"] if rho is None else rho).astype(dt)
+    lat.vel[...] = (case["vel"] if vel is None else vel).astype(dt)
+    sc = None
+    if case.get("sc_spec") is not None:
+        s = case["sc_spec"]
+        lat.psi = np.zeros((lbf.NX + 2, lbf.NY + 2, lbf.nc), dt)
+        psi = case.get("psi") if psi is None else psi
+        if psi is not None:
+            lat.psi[...] = psi.astype(dt)
+        lat.T[...] = 1.0
+        sc = O.ShanChen(s["kind"], group, s["g_coh"], s["rho0"])
+    if f is None:
+        O.init_lbm(lat, group)
+    else:
+        lat.f[...] = f.astype(dt)
+        lat.f2[...] = f.astype(dt)
+    solver = O.Solver(lat, group, oc, bcs, power_law=case.get("power_law", False), sc=sc)
+    return solver
