@@ -15,6 +15,7 @@ on the host for a bounded number of steps of the same lattice.
 """
 import argparse
 import contextlib
+import io
 import json
 import os
 import sys
@@ -103,6 +104,80 @@ def cpu_baseline(case, steps):
             "seconds": round(dt, 2)}
 
 
+def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100):
+    """512 independent cylinder channels (NX ~ U{128..1024}, NY ~ U{64..256},
+    r ~ U[4, NY/8] at (NX/5, NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
+    BGK or TRT (magic 1/4) by a seeded coin), FP32: one persistent launch per
+    step replayed from a graph, against the plain path (one lbm_run launch per
+    lattice per step)."""
+    import ctypes
+    import numpy as np
+    import torch
+    from taichi_lbm import _hip as H
+    from taichi_lbm import cases
+    from taichi_lbm.batch import BatchRunner, autotune_grid, spec_from_case
+    rng = np.random.default_rng(seed)
+    specs = []
+    t0 = time.time()
+    with contextlib.redirect_stdout(io.StringIO()):
+        for q in range(n_lattices):
+            NX, NY = int(rng.integers(128, 1025)), int(rng.integers(64, 257))
+            r, u = float(rng.uniform(4, NY / 8)), float(rng.uniform(0.01, 0.08))
+            tau, coll = float(rng.uniform(0.52, 0.8)), ("bgk" if rng.random() < 0.5 else "trt")
+            c = cases.cylinder(NX=NX, NY=NY, cx=NX / 5, cy=NY / 2, r=r, tau=tau, u_in=u, collision=coll,
+                               name=f"b{q}")
+            specs.append(spec_from_case(c))
+    nodes = sum(s.N for s in specs)
+    fluid = sum(int(((s.code & H.CODE_ACTIVE) != 0).sum()) for s in specs)
+    log(f"batched setup {time.time() - t0:.1f}s: {n_lattices} lattices, {nodes} nodes")
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        runner = BatchRunner(specs, storage="f32")
+        runner.run(2)
+        grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
+        log(f"autotuned grid {grid}: {timings}")
+        g = runner.capture(graph_steps)
+        g.replay()
+        torch.cuda.synchronize()
+        reps = max(1, steps // graph_steps)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            g.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_b = e0.elapsed_time(e1) / (reps * graph_steps)
+        # plain path: one launch per lattice per step (lbm_run on each lattice's own buffers)
+        lib = H.lib()
+        s = stream.cuda_stream
+        par = runner.parity
+        slots = sorted(runner.slots)
+        for l in slots:        # warm-up
+            H.check(lib.lbm_run(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
+                                ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
+                                            ctypes.POINTER(H.NeemRule)), 2, par, 0, s))
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for l in slots:
+            H.check(lib.lbm_run(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
+                                ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
+                                            ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, s))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_p = e0.elapsed_time(e1) / plain_steps
+        g.close()
+        runner.close()
+    sps_b = n_lattices / (ms_b * 1e-3 * solve_steps)
+    sps_p = n_lattices / (ms_p * 1e-3 * solve_steps)
+    return {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid, "steps_per_solve": solve_steps,
+            "ms_per_step_batched": round(ms_b, 4), "ms_per_step_plain": round(ms_p, 4),
+            "mlups_batched": round(fluid / (ms_b * 1e-3) / 1e6, 1),
+            "hbm_frac_batched": round(fluid * 72 / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "solves_per_s_batched": round(sps_b, 3), "solves_per_s_plain": round(sps_p, 3),
+            "speedup": round(sps_b / sps_p, 2), "grid_blocks": grid, "graph_steps": graph_steps,
+            "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +188,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trt", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-lattices", type=int, default=512)
+    ap.add_argument("--only-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -124,6 +202,9 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
 
+    if args.only_batched:
+        print(json.dumps(batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)), flush=True)
+        return
     t0 = time.time()
     case = build_case(args.size, "mrt", args.seed)
     N = args.size * args.size
@@ -156,6 +237,14 @@ def main():
         lbf._st = None
         torch.cuda.empty_cache()
 
+    batched = None
+    if not args.no_batched:
+        case["lbf"]._st = None
+        case["coll"]._state = None
+        torch.cuda.empty_cache()
+        batched = batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)
+        log(f"batched: {batched}")
+
     try:
         extra["copy_ceiling_gbs"] = round(copy_ceiling_gbs(), 1)
     except Exception as e:   # the bandwidth probe is informative only
@@ -187,6 +276,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_fluid_node": BYTES_POWER_LAW},
         "cpu_baseline": cpu,
+        "batched": batched,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 /* status codes */
 #define LBM_OK 0
@@ -47,7 +47,8 @@ enum { LBM_BGK = 0, LBM_TRT = 1, LBM_MRT = 2 };
 enum {
   LBM_STORE_F32 = 0,       /* f itself, fp32 */
   LBM_STORE_F16_SHIFT = 1, /* f - w_k, IEEE half; arithmetic in fp32 */
-  LBM_STORE_BF16_SHIFT = 2 /* f - w_k, bfloat16; arithmetic in fp32 */
+  LBM_STORE_BF16_SHIFT = 2, /* f - w_k, bfloat16; arithmetic in fp32 */
+  LBM_STORE_F64 = 3         /* f itself, fp64; arithmetic in fp64 (high-precision path) */
 };
 
 /* Shan-Chen variants (ExternalForce.py:55-145) */
@@ -152,6 +153,49 @@ int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* 
  * total_rho (BoundaryCondition.py:161-163, 222-223) to macro[parity].        */
 int lbm_finish(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
                int32_t parity, void* stream);
+
+/* ---- batches of independent lattices --------------------------------------
+ * Many lattices (each with its own lbm_lattice, lbm_fields and NEEM rules;
+ * same nc and storage, no Shan-Chen) advance together in one persistent
+ * launch per step over their 256-node tiles.  No counterpart in the
+ * reference, which steps one lattice per kernel sequence; each lattice still
+ * follows lbm_run's semantics exactly.  The batch object owns a device copy
+ * of the descriptors, the tile table and residual accumulators.          */
+typedef struct lbm_batch lbm_batch;
+/* neem: count x LBM_MAX_NEEM rules (lattice l's rules start at l*LBM_MAX_NEEM) */
+int lbm_batch_create(const lbm_lattice* lats, const lbm_fields* flds, const lbm_neem_rule* neem, int32_t count,
+                     lbm_batch** out);
+/* Replace lattice `index` (its buffers / parameters); takes effect at commit. */
+int lbm_batch_update(lbm_batch* b, int32_t index, const lbm_lattice* lat, const lbm_fields* fld,
+                     const lbm_neem_rule* neem);
+/* active: include in the tile list; start_step: launch index of a run at which
+ * this lattice's populations are post-stream f read in place (-1: never);
+ * vel_old: 2 planes for lbm_batch_residual (may be NULL).                  */
+int lbm_batch_set(lbm_batch* b, int32_t index, int32_t active, int32_t start_step, float* vel_old);
+/* Rebuild the tile list and upload descriptors (synchronous on `stream`).  A
+ * graph captured earlier stays valid while the tile count fits the table
+ * capacity of its capture (lbm_batch_capacity).                           */
+int lbm_batch_commit(lbm_batch* b, void* stream);
+int lbm_batch_tiles(const lbm_batch* b);
+int lbm_batch_capacity(const lbm_batch* b);
+/* Reserve room for `tiles` tiles in the tile table (before capturing a graph). */
+int lbm_batch_reserve(lbm_batch* b, int32_t tiles);
+/* nsteps launches (launch t reads parity (parity + t) % 2; the last writes
+ * every node's macro fields).  grid_blocks: persistent grid, multiple of 8
+ * (0 = 2048).                                                              */
+int lbm_batch_run(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream);
+/* lbm_finish for every active lattice of the batch. */
+int lbm_batch_finish(lbm_batch* b, int32_t parity, int32_t grid_blocks, void* stream);
+/* Capture lbm_batch_run(b, nsteps, parity, grid_blocks) into an executable
+ * graph (stream must not be NULL); replay with lbm_graph_launch.          */
+int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream,
+                    void** graph_exec);
+int lbm_graph_launch(void* graph_exec, void* stream);
+int lbm_graph_destroy(void* graph_exec);
+/* out[2l] = sum |u - u_old|^2, out[2l+1] = sum |u|^2 over lattice l's active
+ * nodes (u from macro[parity]); then u_old = u.  Synchronous.              */
+int lbm_batch_residual(lbm_batch* b, int32_t parity, double* out, void* stream);
+int lbm_batch_destroy(lbm_batch* b);
 
 /* ---- reductions (FP64) --------------------------------------------------*/
 /* out[c] = sum over active nodes of rho_c; out[nc], out[nc+1] = sum of

@@ -27,59 +27,69 @@ __host__ __device__ constexpr int opp(int k) {
 static_assert(opp(1) == 3 && opp(2) == 4 && opp(3) == 1 && opp(4) == 2, "opp");
 static_assert(opp(5) == 7 && opp(6) == 8 && opp(7) == 5 && opp(8) == 6, "opp");
 
-// lbm.py:44: an fp32 vector (4,1,1,1,1,1/4,1/4,1/4,1/4) divided by 9.0 in fp32.
-__host__ __device__ constexpr float wgt(int k) {
-  return k == 0 ? 4.0f / 9.0f : (k <= 4 ? 1.0f / 9.0f : 0.25f / 9.0f);
+// lbm.py:44: a vector (4,1,1,1,1,1/4,1/4,1/4,1/4) divided by 9.0 in the
+// working precision (fp32 as the reference's `float`, or fp64).
+template <typename R = float>
+__host__ __device__ constexpr R wgt(int k) {
+  return k == 0 ? R(4) / R(9) : (k <= 4 ? R(1) / R(9) : R(0.25) / R(9));
 }
 
 // Collision.f_eq, CollisionEngine.py:14-18: w*rho*(1+3eu+4.5eu^2-1.5uu).
-__device__ __forceinline__ float feq_k(int k, float ux, float uy, float rho, float uv) {
-  const float eu = (float)cx(k) * ux + (float)cy(k) * uy;
-  return wgt(k) * rho * (1.0f + 3.0f * eu + 4.5f * eu * eu - 1.5f * uv);
+template <typename R>
+__device__ __forceinline__ R feq_k(int k, R ux, R uy, R rho, R uv) {
+  const R eu = (R)cx(k) * ux + (R)cy(k) * uy;
+  return wgt<R>(k) * rho * (R(1) + R(3) * eu + R(4.5) * eu * eu - R(1.5) * uv);
 }
 
 // Collision.f_force (Guo), CollisionEngine.py:20-25: w*(3cF + 9 cF cu - 3 uF).
-__device__ __forceinline__ float fforce_k(int k, float Fx, float Fy, float ux, float uy, float uF) {
-  const float cF = (float)cx(k) * Fx + (float)cy(k) * Fy;
-  const float cu = (float)cx(k) * ux + (float)cy(k) * uy;
-  return wgt(k) * (3.0f * cF + 9.0f * cF * cu - 3.0f * uF);
+template <typename R>
+__device__ __forceinline__ R fforce_k(int k, R Fx, R Fy, R ux, R uy, R uF) {
+  const R cF = (R)cx(k) * Fx + (R)cy(k) * Fy;
+  const R cu = (R)cx(k) * ux + (R)cy(k) * uy;
+  return wgt<R>(k) * (R(3) * cF + R(9) * cF * cu - R(3) * uF);
 }
 
 // Collision.tau_max_min, CollisionEngine.py:28-34.
-__device__ __forceinline__ float clamp_tau(float t, float tmin, float tmax) {
+template <typename R>
+__device__ __forceinline__ R clamp_tau(R t, R tmin, R tmax) {
   if (t < tmin) t = tmin;
   else if (t > tmax) t = tmax;
   return t;
 }
 
 // ---- population storage ---------------------------------------------------
-// Shifted formats store g = f - w_k so that the stored magnitude is the
-// deviation from rest; arithmetic is always fp32.
+// Pop<S>::T is the stored type, Pop<S>::R the arithmetic type.  Shifted
+// formats store g = f - w_k so that the stored magnitude is the deviation
+// from rest; their arithmetic is fp32.
 template <int S> struct Pop;
 template <> struct Pop<LBM_STORE_F32> {
   using T = float;
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i, int) { return p[i]; }
-  static __device__ __forceinline__ void st(T* p, uint32_t i, int, float v) {
-    __builtin_nontemporal_store(v, p + i);
-  }
-  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int, float v) { p[i] = v; }
+  using R = float;
+  static __device__ __forceinline__ R ld(const T* p, uint32_t i, int) { return p[i]; }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, int, R v) { __builtin_nontemporal_store(v, p + i); }
+  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int, R v) { p[i] = v; }
+};
+template <> struct Pop<LBM_STORE_F64> {
+  using T = double;
+  using R = double;
+  static __device__ __forceinline__ R ld(const T* p, uint32_t i, int) { return p[i]; }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, int, R v) { __builtin_nontemporal_store(v, p + i); }
+  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int, R v) { p[i] = v; }
 };
 template <> struct Pop<LBM_STORE_F16_SHIFT> {
   using T = _Float16;
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i, int k) {
-    return (float)p[i] + wgt(k);
+  using R = float;
+  static __device__ __forceinline__ R ld(const T* p, uint32_t i, int k) { return (float)p[i] + wgt<float>(k); }
+  static __device__ __forceinline__ void st(T* p, uint32_t i, int k, R v) {
+    __builtin_nontemporal_store((T)(v - wgt<float>(k)), p + i);
   }
-  static __device__ __forceinline__ void st(T* p, uint32_t i, int k, float v) {
-    __builtin_nontemporal_store((T)(v - wgt(k)), p + i);
-  }
-  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, float v) {
-    p[i] = (T)(v - wgt(k));
-  }
+  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, R v) { p[i] = (T)(v - wgt<float>(k)); }
 };
 template <> struct Pop<LBM_STORE_BF16_SHIFT> {
   using T = uint16_t;  // raw bfloat16 bits
-  static __device__ __forceinline__ float ld(const T* p, uint32_t i, int k) {
-    return __uint_as_float(((uint32_t)p[i]) << 16) + wgt(k);
+  using R = float;
+  static __device__ __forceinline__ R ld(const T* p, uint32_t i, int k) {
+    return __uint_as_float(((uint32_t)p[i]) << 16) + wgt<float>(k);
   }
   static __device__ __forceinline__ uint16_t enc(float x) {
     // round to nearest even; NaN stays NaN
@@ -88,12 +98,10 @@ template <> struct Pop<LBM_STORE_BF16_SHIFT> {
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
   }
-  static __device__ __forceinline__ void st(T* p, uint32_t i, int k, float v) {
-    __builtin_nontemporal_store(enc(v - wgt(k)), p + i);
+  static __device__ __forceinline__ void st(T* p, uint32_t i, int k, R v) {
+    __builtin_nontemporal_store(enc(v - wgt<float>(k)), p + i);
   }
-  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, float v) {
-    p[i] = enc(v - wgt(k));
-  }
+  static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, R v) { p[i] = enc(v - wgt<float>(k)); }
 };
 
 // n / ny via a 64-bit multiply-high: mdiv = floor((2^64-1)/ny) + 1 is exact for

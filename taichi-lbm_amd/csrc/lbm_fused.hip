@@ -20,38 +20,41 @@
 // misaligned) 256 B access and every store is aligned and non-temporal.
 // Solid nodes read only their code (2 B).  No LDS and no MFMA: the step is a
 // bandwidth-bound stencil (about 2-5 flop per byte).
+//
+// The per-node body (node_update) is shared by the single-lattice kernel
+// (k_fused: one block per 256 nodes) and the batched kernel (k_batch: a
+// persistent grid over the 256-node tiles of many independent lattices,
+// each described by a device-resident Lat record, replayable from a graph).
 
 #include "lbm_physics.h"
 
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+
 #include <string>
+#include <vector>
 
 namespace lbm {
 
-struct StepArgs {
-  const void* src;     // populations read this launch
-  void* dst;           // populations written this launch
-  const float* mac_in;  // macro planes of the previous step (NEEM inputs)
-  float* mac_out;       // macro planes of this step
-  float* tau;           // per-node relaxation state (in place for power law)
-  const float* psi;     // Shan-Chen psi of this step (nc planes)
+// One lattice as the kernels see it.  Single-lattice launches pass it by
+// value; batched launches read it from a device array.
+struct Lat {
+  void* pop[2];          // population ping-pong (nc*9 planes each)
+  float* mac[2];         // macro ping-pong: rho_0..rho_{nc-1}, total_rho, ux, uy
+  void* tau;             // per-node relaxation state (R), nc planes
+  void* psi[2];          // Shan-Chen psi (R), nc planes each
   float* body_force;
   float* sc_force;
   float* shear_rate;
+  float* vel_old;        // 2 planes: velocity of the last residual check (batches)
   const uint16_t* code;
   uint64_t mdiv;
-  uint32_t nx, ny, N;
-  uint32_t ps;          // plane stride (elements)
-  int32_t local_read;   // populations are post-stream f: read in place
-  int32_t write_macro;  // write every node's macro fields (last step)
-  int32_t tau_mode;
-  int32_t force;        // gravity non-zero
-  int32_t sc;           // LBM_SC_*
-  int32_t n_neem;
-  int32_t xcd_remap;
-  uint32_t nblocks;
+  uint32_t nx, ny, N, ps;
+  int32_t coll, tau_mode, force, sc, n_neem;
+  int32_t start_step;    // batched runs: launch index whose populations are read in place
+  uint32_t block0;       // batched runs: first 256-node tile of this lattice
+  uint32_t active;
   float tau0[LBM_MAX_NC], tau_anti0[LBM_MAX_NC], magic[LBM_MAX_NC];
   float s_shear[LBM_MAX_NC], s_bulk[LBM_MAX_NC];
   float visc_m, visc_nm1, tau_min, tau_max, gx, gy;
@@ -60,22 +63,34 @@ struct StepArgs {
   lbm_neem_rule neem[LBM_MAX_NEEM];
 };
 
+// Per-launch switches.
+struct Launch {
+  uint32_t parity;       // pop[parity] / mac[parity] are read
+  int32_t step;          // launch index within the run (batches: in-place read when == start_step)
+  int32_t local_read;    // single lattice: populations are post-stream f, read in place
+  int32_t write_macro;   // write every node's macro fields (last step of a run)
+  int32_t xcd_remap;
+  uint32_t nblocks;
+};
+
+enum { MODE_STEP = 0, MODE_FINISH = 1, MODE_PSI = 2 };
+
 // Block index with XCD-aware remapping: the hardware dispatches consecutive
 // workgroups round-robin over the 8 XCDs; remapping gives each XCD one
 // contiguous range of the lattice so the lines shared by neighbouring blocks
 // stay in one L2.
-__device__ __forceinline__ uint32_t logical_block(const StepArgs& a) {
+__device__ __forceinline__ uint32_t logical_block(const Launch& q) {
   const uint32_t b = blockIdx.x;
-  if (!a.xcd_remap) return b;
-  const uint32_t per = a.nblocks >> 3;  // host enables remap only if nblocks % 8 == 0
+  if (!q.xcd_remap) return b;
+  const uint32_t per = q.nblocks >> 3;  // host enables remap only if nblocks % 8 == 0
   return (b & 7u) * per + (b >> 3);
 }
 
 // Post-stream populations of node nb (all components): pull f2(x - c_k, k),
 // or f2(x, opp k) where the code's bounce bit is set.
 template <int NC, int S>
-__device__ __forceinline__ void gather(const typename Pop<S>::T* __restrict__ src, uint32_t ps,
-                                       const Nbr& nb, uint32_t bmask, float (&f)[NC][9]) {
+__device__ __forceinline__ void gather(const typename Pop<S>::T* __restrict__ src, uint32_t ps, const Nbr& nb,
+                                       uint32_t bmask, typename Pop<S>::R (&f)[NC][9]) {
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -90,12 +105,13 @@ __device__ __forceinline__ void gather(const typename Pop<S>::T* __restrict__ sr
 
 // Apply NEEM rule `r` at node (i, j) whose pre-rule populations are f.
 // Reads the interior neighbour's post-stream populations (re-gathered) and
-// its rho/u of the previous step; on `finish` also writes the reference's
-// vel / rho / total_rho overrides.
+// its rho/u of the previous step (mac_in); on `finish` also writes the
+// reference's vel / rho / total_rho overrides into mac_override.
 template <int NC, int S>
-__device__ void apply_neem(const StepArgs& a, const lbm_neem_rule& r, uint32_t i, uint32_t j,
-                           uint32_t n, float (&f)[NC][9], float* mac_override) {
-  using T = typename Pop<S>::T;
+__device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const float* mi,
+                           const lbm_neem_rule& r, uint32_t i, uint32_t j, uint32_t n,
+                           typename Pop<S>::R (&f)[NC][9], float* mac_override) {
+  using R = typename Pop<S>::R;
   const int d = r.direction;
   int di = 0, dj = 0;
   switch (d) {
@@ -107,72 +123,70 @@ __device__ void apply_neem(const StepArgs& a, const lbm_neem_rule& r, uint32_t i
   const uint32_t inb = (uint32_t)((int)i - di), jnb = (uint32_t)((int)j - dj);
   const Nbr nb(inb, jnb, a.nx, a.ny);
   const uint32_t code_nb = a.code[nb.n];
-  float fn[NC][9];
-  gather<NC, S>((const T*)a.src, a.ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, fn);
-  const float* mi = a.mac_in;
+  R fn[NC][9];
+  gather<NC, S>(src, a.ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, fn);
   const uint32_t ps = a.ps;
-  const float ux_nb = mi[(size_t)(NC + 1) * ps + nb.n];
-  const float uy_nb = mi[(size_t)(NC + 2) * ps + nb.n];
-  const float uv_nb = ux_nb * ux_nb + uy_nb * uy_nb;
+  const R ux_nb = mi[(size_t)(NC + 1) * ps + nb.n];
+  const R uy_nb = mi[(size_t)(NC + 2) * ps + nb.n];
+  const R uv_nb = ux_nb * ux_nb + uy_nb * uy_nb;
   if (r.kind == LBM_NEEM_VELOCITY) {
-    const float uxw = r.ux, uyw = r.uy;
-    const float uvw = uxw * uxw + uyw * uyw;
-    float rw[NC];
+    const R uxw = r.ux, uyw = r.uy;
+    const R uvw = uxw * uxw + uyw * uyw;
+    R rw[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) rw[c] = neem_rhow(d, uxw, uyw, f[c]);
+    for (int c = 0; c < NC; ++c) rw[c] = neem_rhow<R>(d, uxw, uyw, f[c]);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const float rho_nb = mi[(size_t)c * ps + nb.n];
+      const R rho_nb = mi[(size_t)c * ps + nb.n];
 #pragma unroll
       for (int k = 0; k < 9; ++k)
-        f[c][k] = feq_k(k, uxw, uyw, rw[c], uvw) + fn[c][k] - feq_k(k, ux_nb, uy_nb, rho_nb, uv_nb);
+        f[c][k] = feq_k<R>(k, uxw, uyw, rw[c], uvw) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
     }
     if (mac_override) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) mac_override[(size_t)c * ps + n] = rw[c];
-      mac_override[(size_t)(NC + 1) * ps + n] = uxw;
-      mac_override[(size_t)(NC + 2) * ps + n] = uyw;
+      for (int c = 0; c < NC; ++c) mac_override[(size_t)c * ps + n] = (float)rw[c];
+      mac_override[(size_t)(NC + 1) * ps + n] = (float)uxw;
+      mac_override[(size_t)(NC + 2) * ps + n] = (float)uyw;
     }
   } else {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const float rho_own = mi[(size_t)c * ps + n];
-      const float rho_nb = mi[(size_t)c * ps + nb.n];
+      const R rho_own = mi[(size_t)c * ps + n];
+      const R rho_nb = mi[(size_t)c * ps + nb.n];
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         if (neem_unknown(d, k))
-          f[c][k] = feq_k(k, ux_nb, uy_nb, rho_own, uv_nb) + fn[c][k] - feq_k(k, ux_nb, uy_nb, rho_nb, uv_nb);
+          f[c][k] = feq_k<R>(k, ux_nb, uy_nb, rho_own, uv_nb) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
     }
     if (mac_override) {
       mac_override[(size_t)NC * ps + n] = r.rho;
-      mac_override[(size_t)(NC + 1) * ps + n] = ux_nb;
-      mac_override[(size_t)(NC + 2) * ps + n] = uy_nb;
+      mac_override[(size_t)(NC + 1) * ps + n] = (float)ux_nb;
+      mac_override[(size_t)(NC + 2) * ps + n] = (float)uy_nb;
     }
   }
 }
 
-enum { MODE_STEP = 0, MODE_FINISH = 1, MODE_PSI = 2 };
-
 template <int COLL, int NC, int S, int MODE>
-__global__ __launch_bounds__(kBlock) void k_fused(const StepArgs a) {
+__device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint32_t n, bool local) {
   using T = typename Pop<S>::T;
-  const uint32_t n = logical_block(a) * kBlock + threadIdx.x;
-  if (n >= a.N) return;
+  using R = typename Pop<S>::R;
   const uint32_t ps = a.ps;
+  const uint32_t p = q.parity;
   const uint32_t code = a.code[n];
   if (!(code & LBM_CODE_ACTIVE)) {
-    if (MODE == MODE_STEP && a.write_macro) {
+    if (MODE == MODE_STEP && q.write_macro) {
+      float* mo = a.mac[p ^ 1u];
 #pragma unroll
-      for (int p = 0; p < NC + 3; ++p) a.mac_out[(size_t)p * ps + n] = 0.0f;
+      for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = 0.0f;
     }
     return;
   }
   const uint32_t i = div_ny(n, a.mdiv);
   const uint32_t j = n - i * a.ny;
-  const T* __restrict__ src = (const T*)a.src;
+  const T* __restrict__ src = (const T*)a.pop[p];
 
-  float f[NC][9];
-  if (MODE == MODE_STEP && a.local_read) {
+  R f[NC][9];
+  if (MODE == MODE_STEP && local) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -181,11 +195,12 @@ __global__ __launch_bounds__(kBlock) void k_fused(const StepArgs a) {
     const Nbr nb(i, j, a.nx, a.ny);
     gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
     const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
-    if (e) apply_neem<NC, S>(a, a.neem[e - 1], i, j, n, f, MODE == MODE_FINISH ? a.mac_out : nullptr);
+    if (e)
+      apply_neem<NC, S>(a, src, a.mac[p], a.neem[e - 1], i, j, n, f, MODE == MODE_FINISH ? a.mac[p] : nullptr);
   }
 
   if constexpr (MODE == MODE_FINISH) {
-    T* dst = (T*)a.dst;
+    T* dst = (T*)a.pop[p ^ 1u];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -194,11 +209,11 @@ __global__ __launch_bounds__(kBlock) void k_fused(const StepArgs a) {
   }
 
   // S1 density (lbm.py:153-164)
-  float rho[NC];
-  float rtot = 0.0f;
+  R rho[NC];
+  R rtot = R(0);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    float s = 0.0f;
+    R s = R(0);
 #pragma unroll
     for (int k = 0; k < 9; ++k) s += f[c][k];
     rho[c] = s;
@@ -206,47 +221,47 @@ __global__ __launch_bounds__(kBlock) void k_fused(const StepArgs a) {
   }
 
   if constexpr (MODE == MODE_PSI) {
-    float* psi_out = (float*)a.dst;
+    R* psi_out = (R*)a.psi[p ^ 1u];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) psi_out[(size_t)c * ps + n] = sc_psi(rho[c], a.psi_rho0[c]);
+    for (int c = 0; c < NC; ++c) psi_out[(size_t)c * ps + n] = sc_psi<R>(rho[c], (R)a.psi_rho0[c]);
     return;
   }
 
   // Shan-Chen force (ExternalForce.py:78-91 / 124-145); psi read at x + c_k.
-  float scx[NC], scy[NC];
+  R scx[NC], scy[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) scx[c] = scy[c] = 0.0f;
+  for (int c = 0; c < NC; ++c) scx[c] = scy[c] = R(0);
   if (a.sc != LBM_SC_NONE) {
     const Nbr nb(i, j, a.nx, a.ny);
-    const float* psi = a.psi;
+    const R* psi = (const R*)a.psi[p ^ 1u];
     if (a.sc == LBM_SC_C1) {
-      float Fx = 0.0f, Fy = 0.0f;
+      R Fx = R(0), Fy = R(0);
 #pragma unroll
       for (int k = 1; k < 9; ++k) {
-        const float p = psi[nb.src(opp(k))];  // x - c_opp(k) = x + c_k
-        Fx += (wgt(k) * (float)cx(k)) * p;
-        Fy += (wgt(k) * (float)cy(k)) * p;
+        const R v = psi[nb.src(opp(k))];  // x - c_opp(k) = x + c_k
+        Fx += (wgt<R>(k) * (R)cx(k)) * v;
+        Fy += (wgt<R>(k) * (R)cy(k)) * v;
       }
-      const float s = -a.g_coh[0] * psi[n];
+      const R s = -(R)a.g_coh[0] * psi[n];
       scx[0] = Fx * s;
       scy[0] = Fy * s;
     } else {
 #pragma unroll
       for (int c1 = 0; c1 < NC; ++c1) {
-        float Fx = 0.0f, Fy = 0.0f;
+        R Fx = R(0), Fy = R(0);
 #pragma unroll
         for (int k = 1; k < 9; ++k) {
           const uint32_t xk = nb.src(opp(k));
 #pragma unroll
           for (int c2 = 0; c2 < NC; ++c2) {
             if (c2 == c1) continue;
-            const float p = psi[(size_t)c2 * ps + xk];
-            const float G = a.g_coh[c1 * LBM_MAX_NC + c2];
-            Fx += (wgt(k) * (float)cx(k)) * p * G;
-            Fy += (wgt(k) * (float)cy(k)) * p * G;
+            const R v = psi[(size_t)c2 * ps + xk];
+            const R G = a.g_coh[c1 * LBM_MAX_NC + c2];
+            Fx += (wgt<R>(k) * (R)cx(k)) * v * G;
+            Fy += (wgt<R>(k) * (R)cy(k)) * v * G;
           }
         }
-        const float s = -psi[(size_t)c1 * ps + n];
+        const R s = -psi[(size_t)c1 * ps + n];
         scx[c1] = s * Fx;
         scy[c1] = s * Fy;
       }
@@ -255,89 +270,175 @@ __global__ __launch_bounds__(kBlock) void k_fused(const StepArgs a) {
 
   // S3 force_density (lbm.py:215-220)
   const bool force = a.force || a.sc != LBM_SC_NONE;
-  float Fx[NC], Fy[NC];
+  R Fx[NC], Fy[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const float frac = rho[c] / rtot;
-    Fx[c] = scx[c] + frac * a.gx;
-    Fy[c] = scy[c] + frac * a.gy;
+    const R frac = rho[c] / rtot;
+    Fx[c] = scx[c] + frac * (R)a.gx;
+    Fy[c] = scy[c] + frac * (R)a.gy;
   }
 
   // S4 velocity (lbm.py:223-237)
-  float vx = 0.0f, vy = 0.0f;
+  R vx = R(0), vy = R(0);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const float* q = f[c];
-    float tx = q[1] - q[3] + q[5] - q[6] - q[7] + q[8];
-    float ty = q[2] - q[4] + q[5] + q[6] - q[7] - q[8];
+    const R* g = f[c];
+    R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
+    R ty = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
     if (force) {
-      tx = tx + 0.5f * Fx[c];
-      ty = ty + 0.5f * Fy[c];
+      tx = tx + R(0.5) * Fx[c];
+      ty = ty + R(0.5) * Fy[c];
     }
     vx += tx;
     vy += ty;
   }
-  const float ux = vx / rtot, uy = vy / rtot;
+  const R ux = vx / rtot, uy = vy / rtot;
 
   // S5 power law + S6 collide, per component
-  T* __restrict__ dst = (T*)a.dst;
-  float gam[NC];
+  T* __restrict__ dst = (T*)a.pop[p ^ 1u];
+  R gam[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    float tau = a.tau0[c], tau_anti = a.tau_anti0[c], s_shear = a.s_shear[c];
-    gam[c] = 0.0f;
+    R tau = a.tau0[c], tau_anti = a.tau_anti0[c], s_shear = a.s_shear[c];
+    const R s_bulk = a.s_bulk[c];
+    gam[c] = R(0);
     if (a.tau_mode == LBM_TAU_FIELD) {
-      const float state = a.tau[(size_t)c * ps + n];
+      const R state = ((const R*)a.tau)[(size_t)c * ps + n];
       if constexpr (COLL == LBM_MRT) {
         s_shear = state;
       } else {
         tau = state;
-        if constexpr (COLL == LBM_TRT) tau_anti = a.magic[c] / (state - 0.5f) + 0.5f;
+        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
       }
     } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
-      float* tp = a.tau + (size_t)c * ps + n;
-      const float state = *tp;
-      const float g = shear_rate_of<COLL>(f[c], rho[c], ux, uy, state, a.s_bulk[c]);
+      R* tp = (R*)a.tau + (size_t)c * ps + n;
+      const R state = *tp;
+      const R g = shear_rate_of<COLL, R>(f[c], rho[c], ux, uy, state, s_bulk);
       gam[c] = g;
-      const float nu = a.visc_m * powf(g, a.visc_nm1);
-      const float t = clamp_tau(nu * 3.0f + 0.5f, a.tau_min, a.tau_max);
+      const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
+      const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
       if constexpr (COLL == LBM_MRT) {
-        s_shear = 1.0f / t;
+        s_shear = R(1) / t;
         *tp = s_shear;
       } else {
         tau = t;
         *tp = t;
-        if constexpr (COLL == LBM_TRT) tau_anti = a.magic[c] / (t - 0.5f) + 0.5f;
+        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
       }
     }
-    float out[9];
-    collide<COLL>(f[c], out, rho[c], ux, uy, Fx[c], Fy[c], force, tau, tau_anti, s_shear, a.s_bulk[c]);
+    R out[9];
+    collide<COLL, R>(f[c], out, rho[c], ux, uy, Fx[c], Fy[c], force, tau, tau_anti, s_shear, s_bulk);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, out[k]);
   }
 
-  if (a.write_macro || (code & LBM_CODE_KEEP_MACRO)) {
-    float* mo = a.mac_out;
+  if (q.write_macro || (code & LBM_CODE_KEEP_MACRO)) {
+    float* mo = a.mac[p ^ 1u];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = rho[c];
-    mo[(size_t)NC * ps + n] = rtot;
-    mo[(size_t)(NC + 1) * ps + n] = ux;
-    mo[(size_t)(NC + 2) * ps + n] = uy;
+    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = (float)rho[c];
+    mo[(size_t)NC * ps + n] = (float)rtot;
+    mo[(size_t)(NC + 1) * ps + n] = (float)ux;
+    mo[(size_t)(NC + 2) * ps + n] = (float)uy;
   }
-  if (a.write_macro) {
+  if (q.write_macro) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      if (force) {
-        a.body_force[(size_t)(2 * c) * ps + n] = Fx[c];
-        a.body_force[(size_t)(2 * c + 1) * ps + n] = Fy[c];
+      if (force && a.body_force) {
+        a.body_force[(size_t)(2 * c) * ps + n] = (float)Fx[c];
+        a.body_force[(size_t)(2 * c + 1) * ps + n] = (float)Fy[c];
       }
       if (a.sc != LBM_SC_NONE) {
-        a.sc_force[(size_t)(2 * c) * ps + n] = scx[c];
-        a.sc_force[(size_t)(2 * c + 1) * ps + n] = scy[c];
+        a.sc_force[(size_t)(2 * c) * ps + n] = (float)scx[c];
+        a.sc_force[(size_t)(2 * c + 1) * ps + n] = (float)scy[c];
       }
-      if (a.tau_mode == LBM_TAU_POWER_LAW) a.shear_rate[(size_t)c * ps + n] = gam[c];
+      if (a.tau_mode == LBM_TAU_POWER_LAW && a.shear_rate) a.shear_rate[(size_t)c * ps + n] = (float)gam[c];
     }
   }
+}
+
+template <int COLL, int NC, int S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
+  const uint32_t n = logical_block(q) * kBlock + threadIdx.x;
+  if (n >= a.N) return;
+  node_update<COLL, NC, S, MODE>(a, q, n, q.local_read != 0);
+}
+
+// Persistent batched step over the 256-node tiles of many lattices.  Each XCD
+// takes one contiguous range of the tile list; its blocks stride through it.
+// The tile -> lattice table and the tile count live in device memory so a
+// captured graph keeps working after the host changes the batch.
+template <int NC, int S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_batch(const Lat* __restrict__ items,
+                                                  const uint32_t* __restrict__ tile_item,
+                                                  const uint32_t* __restrict__ tile_count, const Launch q) {
+  const uint32_t total = *tile_count;
+  const uint32_t xcd = blockIdx.x & 7u, slot = blockIdx.x >> 3, per = gridDim.x >> 3;
+  const uint32_t lo = (uint32_t)(((uint64_t)total * xcd) >> 3);
+  const uint32_t hi = (uint32_t)(((uint64_t)total * (xcd + 1)) >> 3);
+  for (uint32_t w = lo + slot; w < hi; w += per) {
+    const uint32_t l = __builtin_amdgcn_readfirstlane(tile_item[w]);
+    const Lat& a = items[l];
+    const uint32_t n = (w - a.block0) * kBlock + threadIdx.x;
+    if (n >= a.N) continue;
+    const bool local = (a.start_step == q.step);
+    if constexpr (MODE == MODE_STEP) {
+      switch (a.coll) {
+        case LBM_BGK: node_update<LBM_BGK, NC, S, MODE>(a, q, n, local); break;
+        case LBM_TRT: node_update<LBM_TRT, NC, S, MODE>(a, q, n, local); break;
+        default: node_update<LBM_MRT, NC, S, MODE>(a, q, n, local); break;
+      }
+    } else {
+      node_update<LBM_BGK, NC, S, MODE>(a, q, n, local);
+    }
+  }
+}
+
+// Velocity residual of every lattice of a batch: acc[2l] += sum |u - u_old|^2,
+// acc[2l+1] += sum |u|^2 over its active nodes, then u_old = u.
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_batch_residual(const Lat* __restrict__ items,
+                                                           const uint32_t* __restrict__ tile_item,
+                                                           const uint32_t* __restrict__ tile_count,
+                                                           uint32_t parity, double* acc) {
+  const uint32_t total = *tile_count;
+  __shared__ double part[kBlock / 64][2];
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint32_t l = __builtin_amdgcn_readfirstlane(tile_item[w]);
+    const Lat& a = items[l];
+    const uint32_t n = (w - a.block0) * kBlock + threadIdx.x;
+    double d2 = 0.0, u2 = 0.0;
+    if (n < a.N && (a.code[n] & LBM_CODE_ACTIVE) && a.vel_old) {
+      const float* m = a.mac[parity];
+      const float ux = m[(size_t)(NC + 1) * a.ps + n], uy = m[(size_t)(NC + 2) * a.ps + n];
+      const float ox = a.vel_old[n], oy = a.vel_old[(size_t)a.ps + n];
+      d2 = (double)(ux - ox) * (ux - ox) + (double)(uy - oy) * (uy - oy);
+      u2 = (double)ux * ux + (double)uy * uy;
+      a.vel_old[n] = ux;
+      a.vel_old[(size_t)a.ps + n] = uy;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      d2 += __shfl_down(d2, off, 64);
+      u2 += __shfl_down(u2, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      part[threadIdx.x >> 6][0] = d2;
+      part[threadIdx.x >> 6][1] = u2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      double s = 0.0;
+#pragma unroll
+      for (int wv = 0; wv < kBlock / 64; ++wv) s += part[wv][threadIdx.x];
+      atomicAdd(acc + 2 * l + threadIdx.x, s);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_f64(double* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +470,7 @@ int hip_check(hipError_t e, const char* what) {
   return LBM_E_HIP;
 }
 
-int validate(const lbm_lattice* lat, const lbm_fields* fld) {
+int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem) {
   if (!lat || !fld) return fail("null lattice or fields");
   if (lat->nx < 1 || lat->ny < 1) return fail("bad lattice size %d x %d", lat->nx, lat->ny);
   if (lat->nc < 1 || lat->nc > LBM_MAX_NC) return fail("num_components %d unsupported (1..%d)", lat->nc, LBM_MAX_NC);
@@ -379,38 +480,51 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld) {
   if ((uint64_t)lat->plane_stride * (uint64_t)(lat->nc * 9) >= (1ull << 32))
     return fail("population buffer exceeds 2^32 elements");
   if (lat->collision < LBM_BGK || lat->collision > LBM_MRT) return fail("bad collision kind %d", lat->collision);
-  if (lat->storage < LBM_STORE_F32 || lat->storage > LBM_STORE_BF16_SHIFT) return fail("bad storage %d", lat->storage);
+  if (lat->storage < LBM_STORE_F32 || lat->storage > LBM_STORE_F64) return fail("bad storage %d", lat->storage);
   if (lat->shan_chen < LBM_SC_NONE || lat->shan_chen > LBM_SC_C2) return fail("bad shan_chen %d", lat->shan_chen);
   if (lat->shan_chen == LBM_SC_C2 && lat->nc != 2) return fail("ShanChenForceC2 needs 2 components");
   if (lat->n_neem < 0 || lat->n_neem > LBM_MAX_NEEM) return fail("n_neem %d out of range", lat->n_neem);
+  if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
+  for (int r = 0; r < lat->n_neem; ++r) {
+    if (neem[r].direction < 1 || neem[r].direction > 4) return fail("NEEM rule %d: direction %d", r, neem[r].direction);
+    if (neem[r].kind != LBM_NEEM_VELOCITY && neem[r].kind != LBM_NEEM_PRESSURE)
+      return fail("NEEM rule %d: kind %d", r, neem[r].kind);
+  }
   if (!fld->f[0] || !fld->f[1] || !fld->macro[0] || !fld->macro[1] || !fld->code)
     return fail("missing population / macro / code buffer");
-  if (lat->tau_mode < LBM_TAU_CONST || lat->tau_mode > LBM_TAU_POWER_LAW) return fail("bad tau_mode %d", lat->tau_mode);
+  if (lat->tau_mode < LBM_TAU_CONST || lat->tau_mode > LBM_TAU_POWER_LAW)
+    return fail("bad tau_mode %d", lat->tau_mode);
   if (lat->tau_mode != LBM_TAU_CONST && !fld->tau) return fail("tau_mode %d needs the tau buffer", lat->tau_mode);
-  if (lat->tau_mode == LBM_TAU_POWER_LAW && !fld->shear_rate) return fail("power law needs the shear_rate buffer");
   if (lat->shan_chen && (!fld->psi[0] || !fld->psi[1] || !fld->sc_force))
     return fail("Shan-Chen needs psi and sc_force buffers");
-  if (!fld->body_force) return fail("missing body_force buffer");
   return LBM_OK;
 }
 
-StepArgs make_args(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem) {
-  StepArgs a;
+Lat make_lat(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem) {
+  Lat a;
   memset(&a, 0, sizeof a);
+  for (int q = 0; q < 2; ++q) {
+    a.pop[q] = fld->f[q];
+    a.mac[q] = fld->macro[q];
+    a.psi[q] = fld->psi[q];
+  }
+  a.tau = fld->tau;
+  a.body_force = fld->body_force;
+  a.sc_force = fld->sc_force;
+  a.shear_rate = fld->shear_rate;
+  a.code = fld->code;
   a.nx = (uint32_t)lat->nx;
   a.ny = (uint32_t)lat->ny;
   a.N = a.nx * a.ny;
   a.ps = (uint32_t)lat->plane_stride;
   a.mdiv = lat->ny == 1 ? 0ull : (~0ull) / (uint64_t)lat->ny + 1ull;
-  a.code = fld->code;
-  a.tau = fld->tau;
-  a.body_force = fld->body_force;
-  a.sc_force = fld->sc_force;
-  a.shear_rate = fld->shear_rate;
+  a.coll = lat->collision;
   a.tau_mode = lat->tau_mode;
   a.force = (lat->gx != 0.0f || lat->gy != 0.0f);
   a.sc = lat->shan_chen;
   a.n_neem = lat->n_neem;
+  a.start_step = -1;
+  a.active = 1;
   for (int c = 0; c < LBM_MAX_NC; ++c) {
     a.tau0[c] = lat->tau[c];
     a.tau_anti0[c] = lat->tau_anti[c];
@@ -427,12 +541,18 @@ StepArgs make_args(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem
   a.gx = lat->gx;
   a.gy = lat->gy;
   for (int r = 0; r < lat->n_neem && neem; ++r) a.neem[r] = neem[r];
-  a.nblocks = (a.N + kBlock - 1) / kBlock;
-  a.xcd_remap = (a.nblocks % 8 == 0 && a.nblocks >= 64) ? 1 : 0;
   return a;
 }
 
-using KernelFn = void (*)(const StepArgs);
+Launch make_launch(uint32_t N) {
+  Launch q;
+  memset(&q, 0, sizeof q);
+  q.nblocks = (N + kBlock - 1) / kBlock;
+  q.xcd_remap = (q.nblocks % 8 == 0 && q.nblocks >= 64) ? 1 : 0;
+  return q;
+}
+
+using KernelFn = void (*)(const Lat, const Launch);
 
 template <int COLL, int NC, int S>
 KernelFn pick_mode(int mode) {
@@ -457,48 +577,74 @@ KernelFn pick_store(int store, int coll, int mode) {
   switch (store) {
     case LBM_STORE_F32: return pick_coll<NC, LBM_STORE_F32>(coll, mode);
     case LBM_STORE_F16_SHIFT: return pick_coll<NC, LBM_STORE_F16_SHIFT>(coll, mode);
-    default: return pick_coll<NC, LBM_STORE_BF16_SHIFT>(coll, mode);
+    case LBM_STORE_BF16_SHIFT: return pick_coll<NC, LBM_STORE_BF16_SHIFT>(coll, mode);
+    default: return pick_coll<NC, LBM_STORE_F64>(coll, mode);
   }
 }
 KernelFn pick(int nc, int store, int coll, int mode) {
   return nc == 1 ? pick_store<1>(store, coll, mode) : pick_store<2>(store, coll, mode);
 }
 
-int launch(KernelFn fn, const StepArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(fn, dim3(a.nblocks), dim3(kBlock), 0, s, a);
+int launch(KernelFn fn, const Lat& a, const Launch& q, hipStream_t s) {
+  hipLaunchKernelGGL(fn, dim3(q.nblocks), dim3(kBlock), 0, s, a, q);
   return hip_check(hipGetLastError(), "kernel launch");
+}
+
+// ---- batches ------------------------------------------------------------------
+using BatchFn = void (*)(const Lat*, const uint32_t*, const uint32_t*, const Launch);
+
+template <int NC, int S>
+BatchFn pick_batch_mode(int mode) {
+  return mode == MODE_STEP ? k_batch<NC, S, MODE_STEP> : k_batch<NC, S, MODE_FINISH>;
+}
+template <int NC>
+BatchFn pick_batch_store(int store, int mode) {
+  switch (store) {
+    case LBM_STORE_F32: return pick_batch_mode<NC, LBM_STORE_F32>(mode);
+    case LBM_STORE_F16_SHIFT: return pick_batch_mode<NC, LBM_STORE_F16_SHIFT>(mode);
+    case LBM_STORE_BF16_SHIFT: return pick_batch_mode<NC, LBM_STORE_BF16_SHIFT>(mode);
+    default: return pick_batch_mode<NC, LBM_STORE_F64>(mode);
+  }
+}
+BatchFn pick_batch(int nc, int store, int mode) {
+  return nc == 1 ? pick_batch_store<1>(store, mode) : pick_batch_store<2>(store, mode);
 }
 
 }  // namespace lbm
 
+struct lbm_batch {
+  int nc = 1, storage = 0, count = 0;
+  std::vector<lbm::Lat> host;        // descriptors (host copy)
+  lbm::Lat* items = nullptr;         // device copy
+  uint32_t* tile_item = nullptr;     // device: tile -> lattice
+  uint32_t* tile_count = nullptr;    // device: number of tiles
+  double* acc = nullptr;             // device: residual accumulators, 2 per lattice
+  size_t tile_cap = 0;
+  uint32_t tiles = 0;
+  std::vector<uint32_t> host_tiles;
+};
+
 extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
                        int32_t nsteps, int32_t parity, int32_t first_local, void* stream) {
   using namespace lbm;
-  int rc = validate(lat, fld);
+  int rc = validate(lat, fld, neem);
   if (rc) return rc;
   if (nsteps < 0) return fail("nsteps %d < 0", nsteps);
   if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
-  if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
   hipStream_t s = (hipStream_t)stream;
-  StepArgs a = make_args(lat, fld, neem);
+  const Lat a = make_lat(lat, fld, neem);
+  Launch q = make_launch(a.N);
   KernelFn step = pick(lat->nc, lat->storage, lat->collision, MODE_STEP);
   KernelFn psik = pick(lat->nc, lat->storage, lat->collision, MODE_PSI);
-  int p = parity;
+  uint32_t p = (uint32_t)parity;
   for (int t = 0; t < nsteps; ++t) {
-    a.local_read = (t == 0 && first_local) ? 1 : 0;
-    a.src = fld->f[p];
-    a.mac_in = fld->macro[p];
-    a.mac_out = fld->macro[1 - p];
-    a.write_macro = (t == nsteps - 1) ? 1 : 0;
-    if (lat->shan_chen) {
-      StepArgs b = a;
-      b.dst = fld->psi[1 - p];
-      if ((rc = launch(psik, b, s))) return rc;
-      a.psi = fld->psi[1 - p];
-    }
-    a.dst = fld->f[1 - p];
-    if ((rc = launch(step, a, s))) return rc;
-    p = 1 - p;
+    q.parity = p;
+    q.step = t;
+    q.local_read = (t == 0 && first_local) ? 1 : 0;
+    q.write_macro = (t == nsteps - 1) ? 1 : 0;
+    if (lat->shan_chen && (rc = launch(psik, a, q, s))) return rc;
+    if ((rc = launch(step, a, q, s))) return rc;
+    p ^= 1u;
   }
   return LBM_OK;
 }
@@ -506,14 +652,227 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
 extern "C" int lbm_finish(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
                           int32_t parity, void* stream) {
   using namespace lbm;
-  int rc = validate(lat, fld);
+  int rc = validate(lat, fld, neem);
   if (rc) return rc;
   if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
-  if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
-  StepArgs a = make_args(lat, fld, neem);
-  a.src = fld->f[parity];
-  a.dst = fld->f[1 - parity];
-  a.mac_in = fld->macro[parity];
-  a.mac_out = fld->macro[parity];
-  return launch(pick(lat->nc, lat->storage, lat->collision, MODE_FINISH), a, (hipStream_t)stream);
+  const Lat a = make_lat(lat, fld, neem);
+  Launch q = make_launch(a.N);
+  q.parity = (uint32_t)parity;
+  return launch(pick(lat->nc, lat->storage, lat->collision, MODE_FINISH), a, q, (hipStream_t)stream);
+}
+
+// ---- batch entry points ---------------------------------------------------------
+extern "C" int lbm_batch_create(const lbm_lattice* lats, const lbm_fields* flds, const lbm_neem_rule* neem,
+                                int32_t count, lbm_batch** out) {
+  using namespace lbm;
+  if (!lats || !flds || !out || count < 1) return fail("bad lbm_batch_create arguments");
+  for (int l = 0; l < count; ++l) {
+    int rc = validate(&lats[l], &flds[l], neem ? neem + (size_t)l * LBM_MAX_NEEM : nullptr);
+    if (rc) return rc;
+    if (lats[l].shan_chen) return fail("batched lattices cannot use Shan-Chen forces");
+    if (lats[l].nc != lats[0].nc || lats[l].storage != lats[0].storage)
+      return fail("every lattice of a batch needs the same num_components and storage");
+  }
+  lbm_batch* b = new lbm_batch();
+  b->nc = lats[0].nc;
+  b->storage = lats[0].storage;
+  b->count = count;
+  b->host.resize(count);
+  for (int l = 0; l < count; ++l) b->host[l] = make_lat(&lats[l], &flds[l], neem ? neem + (size_t)l * LBM_MAX_NEEM : nullptr);
+  int rc = hip_check(hipMalloc((void**)&b->items, sizeof(Lat) * count), "hipMalloc(batch items)");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->tile_count, sizeof(uint32_t)), "hipMalloc(tile count)");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->acc, sizeof(double) * 2 * count), "hipMalloc(residuals)");
+  if (rc) {
+    (void)hipFree(b->items);
+    (void)hipFree(b->tile_count);
+    (void)hipFree(b->acc);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return LBM_OK;
+}
+
+extern "C" int lbm_batch_set(lbm_batch* b, int32_t index, int32_t active, int32_t start_step, float* vel_old) {
+  using namespace lbm;
+  if (!b || index < 0 || index >= b->count) return fail("bad batch index %d", index);
+  b->host[index].active = active ? 1u : 0u;
+  b->host[index].start_step = start_step;
+  b->host[index].vel_old = vel_old;
+  return LBM_OK;
+}
+
+extern "C" int lbm_batch_update(lbm_batch* b, int32_t index, const lbm_lattice* lat, const lbm_fields* fld,
+                                const lbm_neem_rule* neem) {
+  using namespace lbm;
+  if (!b || index < 0 || index >= b->count) return fail("bad batch index %d", index);
+  int rc = validate(lat, fld, neem);
+  if (rc) return rc;
+  if (lat->nc != b->nc || lat->storage != b->storage || lat->shan_chen)
+    return fail("lattice does not match the batch (components, storage, no Shan-Chen)");
+  const Lat keep = b->host[index];
+  b->host[index] = make_lat(lat, fld, neem);
+  b->host[index].active = keep.active;
+  b->host[index].start_step = keep.start_step;
+  b->host[index].vel_old = keep.vel_old;
+  return LBM_OK;
+}
+
+extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
+  using namespace lbm;
+  if (!b) return fail("null batch");
+  hipStream_t s = (hipStream_t)stream;
+  b->host_tiles.clear();
+  for (int l = 0; l < b->count; ++l) {
+    Lat& a = b->host[l];
+    a.block0 = (uint32_t)b->host_tiles.size();
+    if (!a.active) continue;
+    const uint32_t nt = (a.N + kBlock - 1) / kBlock;
+    for (uint32_t t = 0; t < nt; ++t) b->host_tiles.push_back((uint32_t)l);
+  }
+  b->tiles = (uint32_t)b->host_tiles.size();
+  if (b->tiles > b->tile_cap) {
+    // a captured graph holds the old table's address: grow only before capture
+    if (b->tile_item) (void)hipFree(b->tile_item);
+    b->tile_cap = b->tiles + b->tiles / 4 + 1024;
+    int rc = hip_check(hipMalloc((void**)&b->tile_item, sizeof(uint32_t) * b->tile_cap), "hipMalloc(tile table)");
+    if (rc) {
+      b->tile_item = nullptr;
+      b->tile_cap = 0;
+      return rc;
+    }
+  }
+  int rc = hip_check(hipMemcpyAsync(b->items, b->host.data(), sizeof(Lat) * b->count, hipMemcpyHostToDevice, s),
+                     "upload batch items");
+  if (!rc && b->tiles)
+    rc = hip_check(hipMemcpyAsync(b->tile_item, b->host_tiles.data(), sizeof(uint32_t) * b->tiles,
+                                  hipMemcpyHostToDevice, s), "upload tile table");
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->tile_count, &b->tiles, sizeof(uint32_t), hipMemcpyHostToDevice, s),
+                   "upload tile count");
+  // the host vectors are read by the copies above: wait before they can change
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return rc;
+}
+
+extern "C" int lbm_batch_tiles(const lbm_batch* b) { return b ? (int)b->tiles : -1; }
+extern "C" int lbm_batch_capacity(const lbm_batch* b) { return b ? (int)b->tile_cap : -1; }
+
+extern "C" int lbm_batch_reserve(lbm_batch* b, int32_t tiles) {
+  using namespace lbm;
+  if (!b || tiles < 0) return fail("bad lbm_batch_reserve arguments");
+  if ((size_t)tiles <= b->tile_cap) return LBM_OK;
+  uint32_t* t = nullptr;
+  int rc = hip_check(hipMalloc((void**)&t, sizeof(uint32_t) * (size_t)tiles), "hipMalloc(tile table)");
+  if (rc) return rc;
+  if (b->tile_item) {
+    rc = hip_check(hipMemcpy(t, b->tile_item, sizeof(uint32_t) * b->tiles, hipMemcpyDeviceToDevice), "copy tiles");
+    (void)hipFree(b->tile_item);
+  }
+  b->tile_item = t;
+  b->tile_cap = (size_t)tiles;
+  return rc;
+}
+
+namespace {
+int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t write_macro, int32_t grid,
+                 hipStream_t s) {
+  using namespace lbm;
+  if (!b->tile_item) return fail("commit the batch before stepping it");
+  if (grid <= 0) grid = 256 * 8;
+  if (grid % 8) return fail("grid_blocks must be a multiple of 8 (one range per XCD)");
+  Launch q;
+  memset(&q, 0, sizeof q);
+  q.parity = parity;
+  q.step = step;
+  q.write_macro = write_macro;
+  hipLaunchKernelGGL(pick_batch(b->nc, b->storage, mode), dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item,
+                     b->tile_count, q);
+  return hip_check(hipGetLastError(), "batch launch");
+}
+}  // namespace
+
+extern "C" int lbm_batch_run(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream) {
+  using namespace lbm;
+  if (!b || nsteps < 0 || (parity != 0 && parity != 1)) return fail("bad lbm_batch_run arguments");
+  hipStream_t s = (hipStream_t)stream;
+  for (int t = 0; t < nsteps; ++t) {
+    int rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s);
+    if (rc) return rc;
+  }
+  return LBM_OK;
+}
+
+extern "C" int lbm_batch_finish(lbm_batch* b, int32_t parity, int32_t grid_blocks, void* stream) {
+  using namespace lbm;
+  if (!b || (parity != 0 && parity != 1)) return fail("bad lbm_batch_finish arguments");
+  return batch_launch(b, MODE_FINISH, (uint32_t)parity, -2, 0, grid_blocks, (hipStream_t)stream);
+}
+
+extern "C" int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream,
+                               void** graph_exec) {
+  using namespace lbm;
+  if (!b || nsteps < 1 || !graph_exec || (parity != 0 && parity != 1)) return fail("bad lbm_batch_graph arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) return fail("graph capture needs a non-null stream");
+  int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (rc) return rc;
+  for (int t = 0; t < nsteps && !rc; ++t)
+    rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s);
+  hipGraph_t g = nullptr;
+  const int rc2 = hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  if (rc || rc2) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : rc2;
+  }
+  hipGraphExec_t ge = nullptr;
+  rc = hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  *graph_exec = (void*)ge;
+  return LBM_OK;
+}
+
+extern "C" int lbm_graph_launch(void* graph_exec, void* stream) {
+  using namespace lbm;
+  if (!graph_exec) return fail("null graph");
+  return hip_check(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+extern "C" int lbm_graph_destroy(void* graph_exec) {
+  using namespace lbm;
+  if (!graph_exec) return LBM_OK;
+  return hip_check(hipGraphExecDestroy((hipGraphExec_t)graph_exec), "hipGraphExecDestroy");
+}
+
+extern "C" int lbm_batch_residual(lbm_batch* b, int32_t parity, double* out, void* stream) {
+  using namespace lbm;
+  if (!b || !out || (parity != 0 && parity != 1)) return fail("bad lbm_batch_residual arguments");
+  if (!b->tile_item) return fail("commit the batch first");
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t n = 2u * (uint32_t)b->count;
+  hipLaunchKernelGGL(k_zero_f64, dim3((n + 255) / 256), dim3(256), 0, s, b->acc, n);
+  int rc = hip_check(hipGetLastError(), "zero launch");
+  if (rc) return rc;
+  const uint32_t grid = b->tiles < 4096u ? (b->tiles ? b->tiles : 1u) : 4096u;
+  if (b->nc == 1)
+    hipLaunchKernelGGL(k_batch_residual<1>, dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count,
+                       (uint32_t)parity, b->acc);
+  else
+    hipLaunchKernelGGL(k_batch_residual<2>, dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count,
+                       (uint32_t)parity, b->acc);
+  if ((rc = hip_check(hipGetLastError(), "residual launch"))) return rc;
+  if ((rc = hip_check(hipMemcpyAsync(out, b->acc, sizeof(double) * n, hipMemcpyDeviceToHost, s), "copy out")))
+    return rc;
+  return hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+extern "C" int lbm_batch_destroy(lbm_batch* b) {
+  if (!b) return LBM_OK;
+  (void)hipFree(b->items);
+  (void)hipFree(b->tile_item);
+  (void)hipFree(b->tile_count);
+  (void)hipFree(b->acc);
+  delete b;
+  return LBM_OK;
 }

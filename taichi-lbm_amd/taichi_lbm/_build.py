@@ -19,6 +19,7 @@ CSRC = ROOT / "csrc"
 INCLUDE = ROOT.parent / "include"
 LIB_DIR = PKG_DIR / "_lib"
 LIB_PATH = LIB_DIR / "liblbm_hip.so"
+STAMP_PATH = LIB_DIR / "liblbm_hip.stamp"
 SOURCES = ["lbm_fused.hip", "lbm_granular.hip", "lbm_reduce.hip"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
@@ -29,6 +30,17 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+
+
+def source_hash() -> str:
+    """Hash of every source the library is built from (HIP, headers, C-ABI header)."""
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.h")) + [INCLUDE / "lbm_abi.h"]
+    for f in files:
+        h.update(f.name.encode())
+        h.update(f.read_bytes())
+    return h.hexdigest()
 
 
 def hipcc() -> str:
@@ -69,6 +81,7 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{_filter(r.stdout + r.stderr)}")
     os.replace(tmp, LIB_PATH)
+    STAMP_PATH.write_text(source_hash() + "\n")
     return LIB_PATH
 
 

@@ -8,10 +8,10 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from ._build import LIB_PATH
+from ._build import LIB_PATH, STAMP_PATH, source_hash
 
 LBM_BGK, LBM_TRT, LBM_MRT = 0, 1, 2
-STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT = 0, 1, 2
+STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT, STORE_F64 = 0, 1, 2, 3
 SC_NONE, SC_C1, SC_C2 = 0, 1, 2
 TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
 NEEM_VELOCITY, NEEM_PRESSURE = 1, 2
@@ -24,7 +24,7 @@ CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class NeemRule(C.Structure):
@@ -79,6 +79,12 @@ def lib():
         raise RuntimeError(
             f"HIP library {path} is missing: build it with taichi_lbm._build.build() "
             "(or __graft_entry__.build()); there is no CPU fallback")
+    if "LBM_HIP_LIB" not in os.environ:
+        want = source_hash()
+        have = STAMP_PATH.read_text().strip() if STAMP_PATH.exists() else ""
+        if have != want:
+            raise RuntimeError(f"HIP library {path} was built from other sources than the ones in this tree: "
+                               "rebuild it with taichi_lbm._build.build() (or __graft_entry__.build())")
     L = C.CDLL(str(path))
     P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
     LP, FP, NP, GP = C.POINTER(Lattice), C.POINTER(Fields), C.POINTER(NeemRule), C.POINTER(GFields)
@@ -97,6 +103,23 @@ def lib():
                  "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_sc_update_psi",
                  "lbm_g_sc_force"):
         sig[name] = ([LP, GP, P, I32, P], C.c_int)
+    BP = C.c_void_p
+    sig.update({
+        "lbm_batch_create": ([LP, FP, NP, I32, C.POINTER(C.c_void_p)], C.c_int),
+        "lbm_batch_update": ([BP, I32, LP, FP, NP], C.c_int),
+        "lbm_batch_set": ([BP, I32, I32, I32, P], C.c_int),
+        "lbm_batch_commit": ([BP, P], C.c_int),
+        "lbm_batch_tiles": ([BP], C.c_int),
+        "lbm_batch_capacity": ([BP], C.c_int),
+        "lbm_batch_reserve": ([BP, I32], C.c_int),
+        "lbm_batch_run": ([BP, I32, I32, I32, P], C.c_int),
+        "lbm_batch_finish": ([BP, I32, I32, P], C.c_int),
+        "lbm_batch_graph": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p)], C.c_int),
+        "lbm_graph_launch": ([P, P], C.c_int),
+        "lbm_graph_destroy": ([P], C.c_int),
+        "lbm_batch_residual": ([BP, I32, D, P], C.c_int),
+        "lbm_batch_destroy": ([BP], C.c_int),
+    })
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -112,7 +135,10 @@ def exported_symbols():
     return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_finish", "lbm_reduce_macro", "lbm_residual",
             "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
-            "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force"]
+            "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
+            "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
+            "lbm_batch_reserve", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph", "lbm_graph_launch",
+            "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
 
 
 def check(rc: int, what: str = ""):

@@ -1,0 +1,406 @@
+"""Batched lattices: many independent solves in one persistent launch per step.
+
+The reference runs one lattice per sequence of Taichi kernels.  ``BatchRunner``
+keeps any number of independent lattices (different sizes, collision models,
+relaxation times and boundary values) resident in device arenas and advances
+all active ones with one launch per step (``lbm_batch_run``) or a replayed
+HIP graph (``lbm_batch_graph``).  Each lattice follows exactly the semantics
+of ``Stepper.run`` (the same per-node body runs in both kernels).
+
+Lattices can be added and removed between launches (slots + a first-fit
+arena allocator), which is what the serving loop (serving.py) builds on.
+
+Population storage is selectable: ``"f32"``, ``"f16"`` / ``"bf16"`` (shifted:
+f - w_k is stored, arithmetic in fp32) or ``"f64"`` (fp64 throughout, the
+high-precision path).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _hip as H
+from ._codes import compile_codes
+from ._device import device, plane_stride, stream, torch
+
+STORAGE = {"f32": H.STORE_F32, "f16": H.STORE_F16_SHIFT, "bf16": H.STORE_BF16_SHIFT, "f64": H.STORE_F64}
+POP_BYTES = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}
+_W64 = np.array([4, 1, 1, 1, 1, 1 / 4, 1 / 4, 1 / 4, 1 / 4]) / 9.0
+_W32 = np.array([4, 1, 1, 1, 1, .25, .25, .25, .25], np.float32) / np.float32(9)
+_CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+_CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+
+
+def _torch_dtype(storage):
+    t = torch()
+    return {"f32": t.float32, "f16": t.float16, "bf16": t.bfloat16, "f64": t.float64}[storage]
+
+
+@dataclass
+class LatticeSpec:
+    """Host description of one lattice of a batch (compiled from the drop-in API)."""
+    NX: int
+    NY: int
+    code: np.ndarray                  # uint16 (NX*NY,)
+    neem: list                        # [(kind, direction, ux, uy, rho)]
+    lattice: H.Lattice                # parameters (collision, tau, fluid model, gravity)
+    rho0: np.ndarray                  # (NX, NY) initial density (fluid nodes)
+    u0: np.ndarray                    # (NX, NY, 2) initial velocity
+    tag: dict = field(default_factory=dict)
+
+    @property
+    def N(self):
+        return self.NX * self.NY
+
+
+def spec_from_case(case, power_law=None) -> LatticeSpec:
+    """Compile a case built through the drop-in API (cases.py) into a LatticeSpec."""
+    from .lbm import lattice_struct
+    lbf, coll = case["lbf"], case["coll"]
+    if lbf.nc != 1 or lbf.sc_field is not None:
+        raise NotImplementedError("batched lattices are single-component without Shan-Chen")
+    code, rules = compile_codes(lbf.NX, lbf.NY, case["macro"].group.nodes, case["bce"].fused_list(),
+                                lbf.neighbor_valid())
+    pl = case["power_law"] if power_law is None else power_law
+    L = lattice_struct(lbf, collision=coll, tau_mode=H.TAU_POWER_LAW if pl else H.TAU_CONST, neem=rules)
+    return LatticeSpec(lbf.NX, lbf.NY, code, rules, L, case["rho"][..., 0].copy(), case["vel"].copy(),
+                       tag=dict(case.get("spec", {})))
+
+
+def footprint_bytes(spec_or_n, storage="f32") -> int:
+    """Device bytes one lattice occupies in a batch (its arena segment)."""
+    n = spec_or_n if isinstance(spec_or_n, int) else spec_or_n.N
+    ps = plane_stride(n)
+    r = 8 if storage == "f64" else 4
+    return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * 4 + 2 * 4 + 2 + r)
+
+
+class _Arena:
+    """First-fit allocator over node offsets (units of 64 nodes)."""
+
+    def __init__(self, capacity: int):
+        self.free = [(0, capacity)]
+
+    def alloc(self, n: int) -> Optional[int]:
+        for q, (s, l) in enumerate(self.free):
+            if l >= n:
+                if l == n:
+                    self.free.pop(q)
+                else:
+                    self.free[q] = (s + n, l - n)
+                return s
+        return None
+
+    def release(self, s: int, n: int):
+        self.free.append((s, n))
+        self.free.sort()
+        merged = []
+        for a, l in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + l)
+            else:
+                merged.append((a, l))
+        self.free = merged
+
+
+class BatchRunner:
+    """Resident lattices advanced together.
+
+    ``BatchRunner(specs)`` sizes the arena for the given lattices and adds them;
+    ``BatchRunner([], capacity_nodes=..., max_slots=...)`` starts empty for
+    dynamic use (``add`` / ``remove``)."""
+
+    def __init__(self, specs: Sequence[LatticeSpec] = (), storage: str = "f32", grid_blocks: int = 0,
+                 capacity_nodes: int = 0, max_slots: int = 0):
+        if storage not in STORAGE:
+            raise ValueError(f"storage must be one of {sorted(STORAGE)}")
+        t = torch()
+        dev = device()
+        specs = list(specs)
+        self.storage = storage
+        self.grid = int(grid_blocks)
+        cap = int(capacity_nodes) or sum(plane_stride(s.N) for s in specs)
+        cap = max(64, (cap + 63) // 64 * 64)
+        self.capacity = cap
+        self.max_slots = int(max_slots) or max(1, len(specs))
+        pdt = _torch_dtype(storage)
+        rdt = t.float64 if storage == "f64" else t.float32
+        # arenas indexed by node offset: lattice l at offset o uses populations
+        # [9*o, 9*(o+ps)), macro [4*o, 4*(o+ps)), code / tau [o, o+ps), u_old [2*o, ...)
+        self.popseg = t.zeros((2, 9 * cap), dtype=pdt, device=dev)
+        self.macseg = t.zeros((2, 4 * cap), dtype=t.float32, device=dev)
+        self.velseg = t.zeros(2 * cap, dtype=t.float32, device=dev)
+        self.code = t.zeros(cap, dtype=t.int16, device=dev)
+        self.tau = t.zeros(cap, dtype=rdt, device=dev)
+        self.arena = _Arena(cap)
+        n = self.max_slots
+        self.lats = (H.Lattice * n)()
+        self.flds = (H.Fields * n)()
+        self.neem = (H.NeemRule * (n * H.MAX_NEEM))()
+        for l in range(n):            # empty slots: 1x1 placeholders, inactive
+            L = self.lats[l]
+            L.nx = L.ny = L.nc = 1
+            L.plane_stride = 64
+            L.storage = STORAGE[storage]
+            L.tau_min, L.tau_max = 0.5000001, 4.0
+            self._point(l, 0)
+        h = ctypes.c_void_p()
+        H.check(H.lib().lbm_batch_create(self.lats, self.flds, self.neem, n, ctypes.byref(h)), "lbm_batch_create")
+        self.h = h
+        for l in range(n):
+            H.check(H.lib().lbm_batch_set(h, l, 0, -1, None), "lbm_batch_set")
+        H.check(H.lib().lbm_batch_reserve(h, cap // 256 + n + 8), "lbm_batch_reserve")
+        self.slots: Dict[int, dict] = {}
+        self.parity = 0
+        self.finished = False
+        self._dirty = True
+        self._pending_local = set()    # slots whose first launch reads in place
+        self.ps = {}
+        for s in specs:
+            self.add(s)
+        self.commit()
+
+    # ---- layout --------------------------------------------------------------
+    def _point(self, l, o, ps=64):
+        F = self.flds[l]
+        pe = self.popseg.element_size()
+        F.f[0] = self.popseg[0].data_ptr() + 9 * o * pe
+        F.f[1] = self.popseg[1].data_ptr() + 9 * o * pe
+        F.macro[0] = self.macseg[0].data_ptr() + 4 * o * 4
+        F.macro[1] = self.macseg[1].data_ptr() + 4 * o * 4
+        F.tau = self.tau.data_ptr() + o * self.tau.element_size()
+        F.code = self.code.data_ptr() + o * 2
+
+    def _vel_ptr(self, o):
+        return self.velseg.data_ptr() + 2 * o * 4
+
+    # ---- lattices --------------------------------------------------------------
+    def fits(self, spec: LatticeSpec) -> bool:
+        return len(self.slots) < self.max_slots and any(l >= plane_stride(spec.N) for _, l in self.arena.free)
+
+    def add(self, spec: LatticeSpec) -> int:
+        """Makes the lattice resident (f = feq(u0, rho0), init_LBM) and active
+        from the next launch on; returns its slot."""
+        ps = plane_stride(spec.N)
+        slot = next((l for l in range(self.max_slots) if l not in self.slots), None)
+        if slot is None:
+            raise RuntimeError("no free batch slot")
+        o = self.arena.alloc(ps)
+        if o is None:
+            raise RuntimeError("batch arena full")
+        self.slots[slot] = {"spec": spec, "off": o, "ps": ps}
+        self.ps[slot] = ps
+        L = self.lats[slot]
+        ctypes.memmove(ctypes.byref(L), ctypes.byref(spec.lattice), ctypes.sizeof(H.Lattice))
+        L.plane_stride = ps
+        L.storage = STORAGE[self.storage]
+        L.n_neem = len(spec.neem)
+        self._point(slot, o, ps)
+        for e, (kind, d, ux, uy, rho) in enumerate(spec.neem):
+            r = self.neem[slot * H.MAX_NEEM + e]
+            r.kind, r.direction, r.ux, r.uy, r.rho = kind, d, ux, uy, rho
+        self._init_slot(slot)
+        rules = ctypes.cast(ctypes.addressof(self.neem) + slot * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
+                            ctypes.POINTER(H.NeemRule))
+        H.check(H.lib().lbm_batch_update(self.h, slot, ctypes.byref(L), ctypes.byref(self.flds[slot]), rules),
+                "lbm_batch_update")
+        H.check(H.lib().lbm_batch_set(self.h, slot, 1, 0, self._vel_ptr(o)), "lbm_batch_set")
+        self._pending_local.add(slot)
+        self._dirty = True
+        return slot
+
+    def remove(self, slot: int):
+        info = self.slots.pop(slot)
+        H.check(H.lib().lbm_batch_set(self.h, slot, 0, -1, None), "lbm_batch_set")
+        self.arena.release(info["off"], info["ps"])
+        self._pending_local.discard(slot)
+        self._dirty = True
+
+    def _init_slot(self, slot):
+        """init_LBM: f = feq(u0, rho0) on active nodes into the populations the
+        next launch reads; relaxation state = initTau's value; u_old = 0."""
+        t = torch()
+        info = self.slots[slot]
+        s, o, ps = info["spec"], info["off"], info["ps"]
+        f64 = self.storage == "f64"
+        dt = np.float64 if f64 else np.float32
+        w = _W64 if f64 else _W32
+        act = (s.code & H.CODE_ACTIVE) != 0
+        rho = s.rho0.reshape(-1).astype(dt)
+        ux = s.u0[..., 0].reshape(-1).astype(dt)
+        uy = s.u0[..., 1].reshape(-1).astype(dt)
+        uv = ux * ux + uy * uy
+        host = np.zeros((9, ps), dt)
+        for k in range(9):
+            eu = _CX[k] * ux + _CY[k] * uy
+            fe = w[k] * rho * (1 + 3 * eu + 4.5 * eu * eu - 1.5 * uv)
+            if self.storage in ("f16", "bf16"):
+                fe = fe - w[k]                                  # shifted storage
+            host[k, :s.N] = np.where(act, fe, 0)
+        self.popseg[self.parity, 9 * o: 9 * (o + ps)].copy_(t.as_tensor(host.reshape(-1)).to(self.popseg.dtype))
+        self.popseg[self.parity ^ 1, 9 * o: 9 * (o + ps)].zero_()
+        self.macseg[:, 4 * o: 4 * (o + ps)].zero_()
+        self.velseg[2 * o: 2 * (o + ps)].zero_()
+        code = np.zeros(ps, np.uint16)
+        code[:s.N] = s.code
+        self.code[o:o + ps].copy_(t.as_tensor(code.view(np.int16)))
+        L = s.lattice
+        state = {0: L.tau[0], 1: L.tau[0], 2: L.s_shear[0]}[L.collision]
+        tau = np.zeros(ps, dt)
+        tau[:s.N] = np.where(act, state, 1.0 if L.collision == 2 else 0.0)
+        self.tau[o:o + ps].copy_(t.as_tensor(tau))
+
+    # ---- control -------------------------------------------------------------
+    def commit(self):
+        if self._dirty:
+            H.check(H.lib().lbm_batch_commit(self.h, stream()), "lbm_batch_commit")
+            self._dirty = False
+
+    @property
+    def tiles(self):
+        return H.lib().lbm_batch_tiles(self.h)
+
+    def set_active(self, slot, active):
+        info = self.slots[slot]
+        H.check(H.lib().lbm_batch_set(self.h, slot, int(active), -1, self._vel_ptr(info["off"])), "lbm_batch_set")
+        self._dirty = True
+
+    def _after_launches(self):
+        if self._pending_local:
+            for slot in self._pending_local:
+                H.check(H.lib().lbm_batch_set(self.h, slot, 1, -1, self._vel_ptr(self.slots[slot]["off"])),
+                        "lbm_batch_set")
+            self._pending_local.clear()
+            self._dirty = True
+
+    def run(self, nsteps: int):
+        if nsteps <= 0:
+            return
+        self.commit()
+        H.check(H.lib().lbm_batch_run(self.h, int(nsteps), self.parity, self.grid, stream()), "lbm_batch_run")
+        self.parity ^= (int(nsteps) & 1)
+        self.finished = False
+        self._after_launches()
+
+    def capture(self, nsteps: int) -> "BatchGraph":
+        """A graph of `nsteps` batched steps from the current parity.  nsteps
+        must be even so every replay starts on the same parity; descriptor and
+        tile-list changes committed between replays are picked up."""
+        if nsteps % 2:
+            raise ValueError("capture an even number of steps")
+        s = stream()
+        if s == 0:
+            raise RuntimeError("capture needs a non-default stream (torch.cuda.stream(...))")
+        self.commit()
+        g = ctypes.c_void_p()
+        H.check(H.lib().lbm_batch_graph(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g)),
+                "lbm_batch_graph")
+        return BatchGraph(self, g, int(nsteps))
+
+    def finish(self):
+        """Boundary pass for every active lattice (f, rho, vel as the reference reads them)."""
+        if not self.finished:
+            self.commit()
+            H.check(H.lib().lbm_batch_finish(self.h, self.parity, self.grid, stream()), "lbm_batch_finish")
+            self.finished = True
+
+    def residuals(self) -> np.ndarray:
+        """Per slot: (sum |u - u_old|^2, sum |u|^2) in FP64 over its active nodes; u_old <- u."""
+        self.commit()
+        out = (ctypes.c_double * (2 * self.max_slots))()
+        H.check(H.lib().lbm_batch_residual(self.h, self.parity, out, stream()), "lbm_batch_residual")
+        return np.array(out).reshape(-1, 2)
+
+    # ---- readback --------------------------------------------------------------
+    def fields(self, slot: int):
+        """Reference-shaped numpy arrays of a lattice after the boundary pass:
+        f, f2 (NX, NY, 1, 9), rho (NX, NY, 1), total_rho (NX, NY), vel (NX, NY, 2)."""
+        if slot in self._pending_local:
+            raise RuntimeError("run the batch at least one step before reading a newly added lattice")
+        self.finish()
+        t = torch()
+        info = self.slots[slot]
+        s, o, ps = info["spec"], info["off"], info["ps"]
+        NX, NY, p = s.NX, s.NY, self.parity
+
+        def pops(q):
+            a = self.popseg[q, 9 * o: 9 * (o + ps)].reshape(9, ps)[:, :s.N].to(t.float64).cpu().numpy()
+            if self.storage in ("f16", "bf16"):
+                act = (s.code & H.CODE_ACTIVE) != 0
+                a = a + np.where(act[None, :], _W32.astype(np.float64)[:, None], 0.0)
+            return a.reshape(9, NX, NY).transpose(1, 2, 0)[:, :, None, :]
+
+        m = self.macseg[p, 4 * o: 4 * (o + ps)].reshape(4, ps)[:, :s.N].cpu().numpy().reshape(4, NX, NY)
+        return {"f2": pops(p), "f": pops(p ^ 1), "rho": m[0][..., None], "total_rho": m[1],
+                "vel": np.stack([m[2], m[3]], -1)}
+
+    def velocity(self, slot: int) -> np.ndarray:
+        return self.fields(slot)["vel"]
+
+    def close(self):
+        if getattr(self, "h", None):
+            H.lib().lbm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchGraph:
+    """An instantiated graph of `nsteps` batched steps."""
+
+    def __init__(self, runner: BatchRunner, g, nsteps: int):
+        self.runner, self.g, self.nsteps = runner, g, nsteps
+        self.parity = runner.parity
+
+    def replay(self):
+        r = self.runner
+        if r.parity != self.parity:
+            raise RuntimeError("the batch parity moved since capture (an odd eager run in between)")
+        r.commit()
+        H.check(H.lib().lbm_graph_launch(self.g, stream()), "lbm_graph_launch")
+        r.finished = False
+        r._after_launches()
+
+    def close(self):
+        if self.g:
+            H.lib().lbm_graph_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune_grid(runner: BatchRunner, candidates=(2, 4, 8, 16, 32), steps: int = 10,
+                  cus: Optional[int] = None):
+    """Times the persistent grid size (workgroups per CU x CUs, a multiple of 8)
+    on the runner's own batch and keeps the fastest.  Advances the batch by
+    len(candidates) * (steps + 2) steps."""
+    t = torch()
+    cus = cus or t.cuda.get_device_properties(t.cuda.current_device()).multi_processor_count
+    best = None
+    timings = {}
+    for per_cu in candidates:
+        grid = max(8, (per_cu * cus) // 8 * 8)
+        runner.grid = grid
+        runner.run(2)
+        s = t.cuda.current_stream()
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(s)
+        runner.run(steps)
+        e1.record(s)
+        t.cuda.synchronize()
+        timings[grid] = e0.elapsed_time(e1) / steps
+        if best is None or timings[grid] < timings[best]:
+            best = grid
+    runner.grid = best
+    return best, timings

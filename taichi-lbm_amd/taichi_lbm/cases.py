@@ -247,3 +247,49 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
                 spec=dict(kind="shan_chen", NX=NX, NY=NY, nc=nc, G=G, gcross=gcross, seed=seed,
                           collision=collision, nu=nu))
 
+
+
+def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt", magic=0.25, tau_min=0.51,
+                tau_max=2.0, name="contraction"):
+    """4:1 planar contraction scaled from cases/contraction flow.py:16-33,80-81
+    (a 40 x 8 channel with solid blocks over x in (25, 35) leaving the gap
+    y in (3, 5)): VelocityNEEM inlet (d=3, u_in), PressureNEEM outlet (d=1,
+    rho=1), bounce-back walls, power-law fluid with m chosen so that the
+    apparent viscosity is nu_ref at the inlet shear rate u_in / (H/2), tau
+    clamped to [tau_min, tau_max]."""
+    NX, NY = int(W), int(H)
+    lbf = LBField(name, NX, NY, 1)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    sx, sy = NX / 40.0, NY / 8.0          # lattice nodes per reference length unit
+    bce.Mask_rectangle_identify(lbf, 25 * sx - 0.5, 35 * sx - 0.5, 5 * sy - 0.5, 8 * sy - 0.5)
+    bce.Mask_rectangle_identify(lbf, 25 * sx - 0.5, 35 * sx - 0.5, 0 * sy - 0.5, 3 * sy - 0.5)
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    periodic.precompute(clf)
+    bce.add_boundary_condition("periodic", periodic)
+    wmap = solid_neighbour_map(m, y_walls=True)
+    wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+    wall.precompute(clf)
+    bce.add_boundary_condition("wall", wall)
+    inlet = VelocityNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1)),
+                         (u_in, 0.0), 3)
+    inlet.precompute(clf)
+    bce.add_boundary_condition("inlet", inlet)
+    outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
+                          1.0, 1)
+    outlet.precompute(clf)
+    bce.add_boundary_condition("outlet", outlet)
+    lbf.neighbor_classify()
+    gamma_ref = u_in / (NY / 2.0)
+    power_m = nu_ref * gamma_ref ** (1.0 - power_n)
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [nu_ref], PowerLawFluid(power_m, power_n), magic=magic,
+                          nu_bulk=[nu_ref])
+    coll.tau_min, coll.tau_max = tau_min, tau_max
+    macro, rho, vel = _finish(lbf, bce, fluid, coll)
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
+                spec=dict(kind="contraction", NX=NX, NY=NY, power_n=power_n, power_m=power_m, u_in=u_in,
+                          nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max))

@@ -1,0 +1,172 @@
+"""Serving: continuous batching of independent solves on one GPU.
+
+Requests (one lattice each, e.g. a contraction-flow power-law solve) arrive
+over time.  The server keeps admitted solves resident in a BatchRunner and
+advances all of them together, ``tick_steps`` steps per tick (a replayed HIP
+graph).  Between ticks it
+
+* checks each solve's convergence -- the relative velocity change over the
+  tick, ||u - u_prev||_2 / ||u||_2 over fluid nodes in FP64, against
+  ``tol`` -- and its step budget, retires finished solves (reading their
+  velocity field) and frees their HBM;
+* admits waiting requests in arrival order while their footprint fits the
+  HBM budget and a slot is free, so work that arrives while others run
+  joins the next tick.
+
+Each solve's result is independent of what else is resident (the batched
+kernel touches only the lattice's own arena segment); the tests check that
+a solve returns the same field alone and in a mix.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from .batch import BatchRunner, LatticeSpec, footprint_bytes
+from ._device import torch
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                  # seconds after the server starts
+    spec: LatticeSpec
+    max_steps: int = 50_000
+    tol: float = 1e-6
+    # filled by the server
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    steps: int = 0
+    residual: float = float("inf")
+    converged: bool = False
+    result: Optional[np.ndarray] = None
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+@dataclass
+class ServeStats:
+    completed: List[Request] = field(default_factory=list)
+    wall: float = 0.0
+    ticks: int = 0
+    peak_resident: int = 0
+    peak_bytes: int = 0
+
+    def summary(self):
+        lat = np.array([r.latency for r in self.completed])
+        return {
+            "solves": len(self.completed),
+            "wall_s": round(self.wall, 3),
+            "solves_per_s": round(len(self.completed) / self.wall, 3) if self.wall > 0 else None,
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
+            "converged": int(sum(r.converged for r in self.completed)),
+            "ticks": self.ticks,
+            "peak_resident": self.peak_resident,
+            "peak_resident_bytes": self.peak_bytes,
+        }
+
+
+class SolveServer:
+    def __init__(self, storage: str = "f32", hbm_budget_bytes: int = 8 << 30, max_slots: int = 256,
+                 tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True):
+        if tick_steps % 2:
+            raise ValueError("tick_steps must be even (graph replays keep one parity schedule)")
+        self.storage = storage
+        self.budget = int(hbm_budget_bytes)
+        # arena sized from the budget: bytes per node of a resident lattice
+        per_node = footprint_bytes(64, storage) // 64
+        self.capacity_nodes = self.budget // per_node // 64 * 64
+        self.runner = BatchRunner([], storage=storage, grid_blocks=grid_blocks,
+                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots)
+        self.tick_steps = int(tick_steps)
+        self.keep_results = keep_results
+        self.graph = None
+        self.resident: Dict[int, Request] = {}      # slot -> request
+        self.used_bytes = 0
+
+    def _admit(self, req: Request, now: float):
+        slot = self.runner.add(req.spec)
+        req.admitted = now
+        self.resident[slot] = req
+        self.used_bytes += footprint_bytes(req.spec, self.storage)
+
+    def _retire(self, slot: int, now: float):
+        req = self.resident.pop(slot)
+        if self.keep_results:
+            req.result = self.runner.velocity(slot)
+        self.runner.remove(slot)
+        self.used_bytes -= footprint_bytes(req.spec, self.storage)
+        req.finished = now
+
+    def serve(self, requests: List[Request], clock: Callable[[], float] = time.perf_counter,
+              on_tick: Optional[Callable] = None) -> ServeStats:
+        """Runs until every request has finished; arrivals are honoured in real time."""
+        t = torch()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        stats = ServeStats()
+        stream = t.cuda.Stream()
+        t0 = clock()
+        with t.cuda.stream(stream):
+            while pending or self.resident:
+                now = clock() - t0
+                # admission in arrival order while the footprint fits
+                while pending and pending[0].arrival <= now and self.runner.fits(pending[0].spec) and \
+                        self.used_bytes + footprint_bytes(pending[0].spec, self.storage) <= self.budget:
+                    self._admit(pending.pop(0), now)
+                if not self.resident:
+                    if pending:        # idle until the next arrival
+                        time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                    continue
+                stats.peak_resident = max(stats.peak_resident, len(self.resident))
+                stats.peak_bytes = max(stats.peak_bytes, self.used_bytes)
+                if self.graph is None:
+                    # first tick runs eagerly (fresh lattices read in place), then capture
+                    self.runner.run(self.tick_steps)
+                    self.graph = self.runner.capture(self.tick_steps)
+                else:
+                    self.graph.replay()
+                stats.ticks += 1
+                res = self.runner.residuals()            # synchronises the stream
+                now = clock() - t0
+                for slot, req in list(self.resident.items()):
+                    req.steps += self.tick_steps
+                    d2, u2 = res[slot]
+                    req.residual = float(np.sqrt(d2 / u2)) if u2 > 0 else float("inf")
+                    if req.residual < req.tol:
+                        req.converged = True
+                    if req.converged or req.steps >= req.max_steps:
+                        self._retire(slot, now)
+                        stats.completed.append(req)
+                if on_tick:
+                    on_tick(self, stats)
+        stats.wall = clock() - t0
+        return stats
+
+    def close(self):
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
+        self.runner.close()
+
+
+def solve_alone(spec: LatticeSpec, max_steps: int, tol: float, tick_steps: int, storage: str = "f32",
+                grid_blocks: int = 0):
+    """The same request on its own: one lattice, the same tick / residual schedule."""
+    r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks)
+    steps, res = 0, float("inf")
+    while True:
+        r.run(tick_steps)
+        steps += tick_steps
+        d2, u2 = r.residuals()[0]
+        res = float(np.sqrt(d2 / u2)) if u2 > 0 else float("inf")
+        if res < tol or steps >= max_steps:
+            break
+    u = r.velocity(0)
+    r.close()
+    return u, steps, res

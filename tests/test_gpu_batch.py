@@ -1,0 +1,146 @@
+"""Batched lattices (one persistent launch per step, graph replay) against the
+single-lattice fused path and the oracle."""
+import numpy as np
+import pytest
+
+from taichi_lbm import cases
+from taichi_lbm.batch import BatchRunner, autotune_grid, spec_from_case
+from taichi_lbm.stepper import Stepper
+from oracle_bridge import oracle_solver
+from tolerances import check
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    out = []
+    params = [(48, 24, "bgk", 0.6, 0.05), (40, 32, "trt", 0.7, 0.03), (64, 20, "mrt", 0.55, 0.08),
+              (33, 17, "bgk", 0.8, 0.01), (50, 26, "trt", 0.52, 0.06)]
+    for q, (NX, NY, coll, tau, u) in enumerate(params):
+        r = max(2, NY // 8)
+        out.append(cases.cylinder(NX=NX, NY=NY, cx=NX // 5, cy=NY // 2, r=r, tau=tau, u_in=u, collision=coll,
+                                  tau_bulk=0.8, name=f"c{q}"))
+    return out
+
+
+def test_batch_equals_single_lattice_runs():
+    import torch
+    cs = _mix()
+    runner = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    n = 40
+    runner.run(15)
+    runner.run(n - 15)
+    for q, c in enumerate(cs):
+        print(f"lattice {q}", flush=True)
+        cases.initialise(c)
+        st = Stepper(c["lbf"], c["macro"], c["coll"], c["bce"])
+        st.run(n)
+        got = runner.fields(q)
+        for name in ("f", "f2", "rho", "vel"):
+            ref = getattr(c["lbf"], name).to_numpy()
+            assert np.array_equal(got[name].astype(np.float32), ref), (q, name)
+    torch.cuda.synchronize()
+
+
+def test_batch_matches_oracle_fp32():
+    cs = _mix()[:2]
+    runner = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    n = 30
+    runner.run(n)
+    for q, c in enumerate(cs):
+        sol = oracle_solver(c, np.float32)
+        sol.run(n)
+        got = runner.fields(q)
+        for name in ("f", "f2", "rho", "vel"):
+            check(name, got[name], getattr(sol.lat, name))
+
+
+def test_graph_replay_equals_run():
+    import torch
+    cs = _mix()
+    a = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    b = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a.run(2)
+        b.run(2)
+        g = b.capture(6)
+        for rep in range(4):
+            g.replay()
+            torch.cuda.synchronize()          # eager work between replays
+            _ = b.residuals()
+        a.run(24)
+        torch.cuda.synchronize()
+        for q in range(len(cs)):
+            fa, fb = a.fields(q), b.fields(q)
+            for name in ("f", "f2", "rho", "vel"):
+                assert np.array_equal(fa[name], fb[name]), (q, name)
+        g.close()
+
+
+def test_fp64_batch_matches_fp64_oracle():
+    cs = _mix()[2:4]
+    runner = BatchRunner([spec_from_case(c) for c in cs], storage="f64")
+    n = 25
+    runner.run(n)
+    for q, c in enumerate(cs):
+        sol = oracle_solver(c, np.float64)
+        sol.run(n)
+        got = runner.fields(q)
+        for name in ("f", "f2"):
+            err = np.abs(got[name] - getattr(sol.lat, name)).max()
+            print(f"  lattice {q} {name}: |fp64 hip - fp64 oracle| = {err:.2e}")
+            assert err < 1e-13
+        assert np.abs(got["vel"] - sol.lat.vel).max() < 1e-7   # vel is written back as fp32
+
+
+@pytest.mark.parametrize("storage,bound", [("f16", 2e-4), ("bf16", 2e-3)])
+def test_shifted_storage_error_vs_fp64(storage, bound):
+    cs = _mix()[:3]
+    lo = BatchRunner([spec_from_case(c) for c in cs], storage=storage)
+    hi = BatchRunner([spec_from_case(c) for c in cs], storage="f64")
+    lo.run(200)
+    hi.run(200)
+    for q in range(len(cs)):
+        du = np.abs(lo.fields(q)["vel"] - hi.fields(q)["vel"]).max()
+        print(f"  {storage} lattice {q}: max |u - u_fp64| = {du:.2e}")
+        assert du < bound
+
+
+def test_deactivated_lattice_is_frozen_and_others_unchanged():
+    cs = _mix()
+    a = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    b = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    a.run(10)
+    b.run(10)
+    before = b.fields(1)
+    b.set_active(1, 0)
+    a.run(20)
+    b.run(20)
+    after = b.fields(1)
+    assert np.array_equal(before["f2"], after["f2"])
+    for q in (0, 2, 3, 4):
+        assert np.array_equal(a.fields(q)["f2"], b.fields(q)["f2"]), q
+
+
+def test_residuals_match_host():
+    cs = _mix()[:2]
+    r = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    r.run(5)
+    r.residuals()
+    u_prev = [r.fields(q)["vel"].astype(np.float64) for q in range(2)]
+    r.run(7)
+    res = r.residuals()
+    for q in range(2):
+        u = r.fields(q)["vel"].astype(np.float64)
+        act = (r.slots[q]["spec"].code.reshape(u.shape[:2]) & 0x100) != 0
+        d = ((u - u_prev[q]) ** 2).sum(-1)[act].sum()
+        assert np.isclose(res[q, 0], d, rtol=1e-6)
+
+
+def test_autotune_picks_a_candidate():
+    cs = _mix()
+    r = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    r.run(1)
+    best, timings = autotune_grid(r, candidates=(2, 8), steps=4)
+    assert best in timings and best % 8 == 0
