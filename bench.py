@@ -106,7 +106,7 @@ def cpu_baseline(case, steps):
 
 def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100):
     """512 independent cylinder channels (NX ~ U{128..1024}, NY ~ U{64..256},
-    r ~ U[4, NY/8] at (NX/5, NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
+    r ~ U[4, NY/8] at (max(NX/5, r + 4), NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
     BGK or TRT (magic 1/4) by a seeded coin), FP32: one persistent launch per
     step replayed from a graph, against the plain path (one lbm_run launch per
     lattice per step)."""
@@ -124,8 +124,8 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             NX, NY = int(rng.integers(128, 1025)), int(rng.integers(64, 257))
             r, u = float(rng.uniform(4, NY / 8)), float(rng.uniform(0.01, 0.08))
             tau, coll = float(rng.uniform(0.52, 0.8)), ("bgk" if rng.random() < 0.5 else "trt")
-            c = cases.cylinder(NX=NX, NY=NY, cx=NX / 5, cy=NY / 2, r=r, tau=tau, u_in=u, collision=coll,
-                               name=f"b{q}")
+            c = cases.cylinder(NX=NX, NY=NY, cx=max(NX / 5, r + 4), cy=NY / 2, r=r, tau=tau, u_in=u,
+                               collision=coll, name=f"b{q}")
             specs.append(spec_from_case(c))
     nodes = sum(s.N for s in specs)
     fluid = sum(int(((s.code & H.CODE_ACTIVE) != 0).sum()) for s in specs)
