@@ -178,6 +178,76 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps}
 
 
+def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16", "bf16"), n_fp64=3):
+    """Poisson stream of 4:1 contraction-flow power-law solves (width ~ U{256..2048},
+    height ~ U{64..512}, n ~ U[0.5, 1.5], TRT magic 1/4, u_in = 0.02, tau in
+    [0.51, 2.0]), each run until ||u - u_(-tick)||/||u|| < tol or max_steps.
+    Continuous batching (SolveServer) against the plain path (each solve alone,
+    one after another in arrival order); shifted FP16/BF16 storage against
+    FP32; errors against the FP64 path on the first n_fp64 requests."""
+    import numpy as np
+    import torch
+    from taichi_lbm import cases
+    from taichi_lbm.batch import spec_from_case
+    from taichi_lbm.serving import Request, SolveServer, solve_alone
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    specs = []
+    with contextlib.redirect_stdout(io.StringIO()):
+        for q in range(n_req):
+            W, Hh, n = int(rng.integers(256, 2049)), int(rng.integers(64, 513)), float(rng.uniform(0.5, 1.5))
+            specs.append(spec_from_case(cases.contraction(W=W, H=Hh, power_n=n, name=f"r{q}")))
+    nodes = [s.N for s in specs]
+    out = {"requests": n_req, "rate_per_s": rate, "tol": tol, "max_steps": max_steps, "tick_steps": tick,
+           "mean_nodes": int(np.mean(nodes))}
+    results = {}
+    for storage in storages:
+        reqs = [Request(q, float(arrivals[q]), specs[q], max_steps=max_steps, tol=tol) for q in range(n_req)]
+        srv = SolveServer(storage=storage, hbm_budget_bytes=24 << 30, max_slots=n_req, tick_steps=tick)
+        st = srv.serve(reqs)
+        srv.close()
+        summ = st.summary()
+        summ["mean_steps"] = float(np.mean([r.steps for r in st.completed]))
+        out[f"batched_{storage}"] = summ
+        results[storage] = {r.rid: (r.result, r.steps) for r in st.completed}
+        log(f"serving {storage}: {summ}")
+    # plain path: each request alone, one after another in arrival order (FP32)
+    svc, steps_alone, same = [], [], True
+    for q in range(n_req):
+        t0 = time.perf_counter()
+        u, nst, _ = solve_alone(specs[q], max_steps, tol, tick, "f32")
+        torch.cuda.synchronize()
+        svc.append(time.perf_counter() - t0)
+        steps_alone.append(nst)
+        ub, sb = results["f32"][q]
+        same &= (nst == sb) and np.array_equal(u, ub)
+    fin, lat = 0.0, []
+    for q in range(n_req):
+        start = max(float(arrivals[q]), fin)
+        fin = start + svc[q]
+        lat.append(fin - float(arrivals[q]))
+    out["plain_f32"] = {"solves_per_s": round(n_req / fin, 3), "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                        "p99_latency_s": round(float(np.percentile(lat, 99)), 4), "wall_s": round(fin, 3),
+                        "mean_service_s": round(float(np.mean(svc)), 4)}
+    out["alone_equals_mix_f32"] = bool(same)
+    b = out["batched_f32"]
+    out["speedup_solves_per_s"] = round(b["solves_per_s"] / out["plain_f32"]["solves_per_s"], 2)
+    # precision: max |u - u_fp64| at the same step count, on the first requests
+    err = {s: [] for s in storages}
+    for q in range(min(n_fp64, n_req)):
+        nst = results["f32"][q][1]
+        u64, _, _ = solve_alone(specs[q], nst, 0.0, tick, "f64")
+        for s in storages:
+            u, ns = results[s][q]
+            if ns == nst:
+                err[s].append(float(np.abs(u - u64).max()))
+            else:   # a different stopping step: compare at the FP32 step count
+                u2, _, _ = solve_alone(specs[q], nst, 0.0, tick, s)
+                err[s].append(float(np.abs(u2 - u64).max()))
+    out["max_abs_u_err_vs_fp64"] = {s: max(v) if v else None for s, v in err.items()}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +261,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-lattices", type=int, default=512)
     ap.add_argument("--only-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--only-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -202,6 +275,9 @@ def main():
         sys.exit(1)
     torch.cuda.set_device(0)
 
+    if args.only_serving:
+        print(json.dumps(serving_leg(args.serve_requests, 4.0, 5, 50_000, 1e-6, 200)), flush=True)
+        return
     if args.only_batched:
         print(json.dumps(batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)), flush=True)
         return
@@ -245,6 +321,12 @@ def main():
         batched = batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)
         log(f"batched: {batched}")
 
+    serving = None
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        serving = serving_leg(args.serve_requests, 4.0, 5, 50_000, 1e-6, 200)
+        log(f"serving: {serving}")
+
     try:
         extra["copy_ceiling_gbs"] = round(copy_ceiling_gbs(), 1)
     except Exception as e:   # the bandwidth probe is informative only
@@ -277,6 +359,7 @@ def main():
                      "bytes_per_fluid_node": BYTES_POWER_LAW},
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)

@@ -251,7 +251,9 @@ enum {
   LBM_BC_INSIDE = 2,         /* InsideBoundary.apply, :440-449 (-1 entries are skipped) */
   LBM_BC_BOUNCEBACK = 3,     /* BounceBackWall.apply, :244-259 */
   LBM_BC_VELOCITY_NEEM = 4,  /* VelocityNEEM.apply, :154-166 */
-  LBM_BC_PRESSURE_NEEM = 5   /* PressureNEEM.apply, :215-226 */
+  LBM_BC_PRESSURE_NEEM = 5,  /* PressureNEEM.apply, :215-226 */
+  LBM_BC_VELOCITY_BB = 6,    /* VelocityBB.apply, :85-102 (additive, rule.ux/uy, direction) */
+  LBM_BC_OPEN_NEUMANN = 7    /* OpenNeumann.apply, :321-334 (copy unknowns, then density and velocity) */
 };
 int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                    const int32_t* group, int32_t count, void* stream);

@@ -551,6 +551,39 @@ class BC:
                 full = neem(lat.vel[I, J], lat.rho[I, J, c], lat.vel[I2, J2], lat.rho[I2, J2, c],
                             lat.f[I2, J2, c, :], dt)
                 lat.f[I[:, None], J[:, None], c, unk[None, :]] = full[:, unk]
+        elif self.kind == "velocity_bb":     # :85-102 (additive on the unknowns)
+            d = self.direction
+            uw = np.asarray(self.value, dt)
+            w = weights(dt)
+            for c in range(lat.nc):
+                rho_w = lat.rho[I, J, c]
+                for k in UNKNOWN[d]:
+                    cv = dt.type(C[k, 0]) * uw[0] + dt.type(C[k, 1]) * uw[1]
+                    lat.f[I, J, c, k] = lat.f[I, J, c, k] + 6 * w[k] * rho_w * cv
+        elif self.kind == "open_neumann":    # :321-334, OpenBoundary.density/vel :274-298
+            d = self.direction
+            I2, J2 = I - C[d, 0], J - C[d, 1]
+            for c in range(lat.nc):
+                for k in UNKNOWN[d]:
+                    lat.f[I, J, c, k] = lat.f[I2, J2, c, k]
+            tot = np.zeros(len(I), dt)
+            for c in range(lat.nc):
+                r = np.zeros(len(I), dt)
+                for k in range(9):
+                    r = r + lat.f[I, J, c, k]
+                lat.rho[I, J, c] = r
+                tot = tot + r
+            lat.total_rho[I, J] = tot
+            cf = _cf(dt)
+            acc = np.zeros((len(I), 2), dt)
+            for c in range(lat.nc):
+                vx = np.zeros(len(I), dt)
+                vy = np.zeros(len(I), dt)
+                for k in range(9):
+                    vx = vx + lat.f[I, J, c, k] * cf[k, 0]
+                    vy = vy + lat.f[I, J, c, k] * cf[k, 1]
+                acc = acc + np.stack([vx, vy], 1) + 0.5 * lat.body_force[I, J, c, :]
+            lat.vel[I, J] = acc / tot[:, None]
         else:
             raise ValueError(self.kind)
 

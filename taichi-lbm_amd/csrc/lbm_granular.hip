@@ -280,6 +280,48 @@ __global__ void k_boundary(const G a) {
       // InsideBoundary with a -1 entry reads outside the reference's field
       // (no guard at :446-449); here the population is left untouched.
     }
+  } else if (a.kind == LBM_BC_VELOCITY_BB) {
+    // f_d += 6 w_d rho c_d . u_w on the unknown populations (BoundaryCondition.py:94-102)
+    const int d = a.rule.direction;
+    for (uint32_t c = 0; c < a.nc; ++c) {
+      const float rhow = a.g.rho[c * ps + n];
+      for (int k = 0; k < 9; ++k)
+        if (neem_unknown(d, k)) {
+          const float cv = (float)cx(k) * a.rule.ux + (float)cy(k) * a.rule.uy;
+          float* fp = P(a.g.f, c * 9 + k, a.ps) + n;
+          *fp = *fp + 6.0f * wgt(k) * rhow * cv;
+        }
+    }
+  } else if (a.kind == LBM_BC_OPEN_NEUMANN) {
+    // copy the unknowns from x - c_d, then OpenBoundary.density / vel at x (:274-298)
+    const int d = a.rule.direction;
+    const uint32_t n2 = (uint32_t)((int)i - cx(d)) * ny + (uint32_t)((int)j - cy(d));
+    for (uint32_t c = 0; c < a.nc; ++c)
+      for (int k = 0; k < 9; ++k)
+        if (neem_unknown(d, k)) P(a.g.f, c * 9 + k, a.ps)[n] = P(a.g.f, c * 9 + k, a.ps)[n2];
+    float tot = 0.0f;
+    for (uint32_t c = 0; c < a.nc; ++c) {
+      float r = 0.0f;
+      for (int k = 0; k < 9; ++k) r += P(a.g.f, c * 9 + k, a.ps)[n];
+      a.g.rho[c * ps + n] = r;
+      tot += r;
+    }
+    a.g.total_rho[n] = tot;
+    float vx = 0.0f, vy = 0.0f;
+    for (uint32_t c = 0; c < a.nc; ++c) {
+      float tx = 0.0f, ty = 0.0f;
+      for (int k = 0; k < 9; ++k) {
+        const float q = P(a.g.f, c * 9 + k, a.ps)[n];
+        tx += q * (float)cx(k);
+        ty += q * (float)cy(k);
+      }
+      tx = tx + 0.5f * a.g.body_force[(2 * c) * ps + n];
+      ty = ty + 0.5f * a.g.body_force[(2 * c + 1) * ps + n];
+      vx += tx;
+      vy += ty;
+    }
+    a.g.vel[n] = vx / tot;
+    a.g.vel[ps + n] = vy / tot;
   } else {
     const int d = a.rule.direction;
     const uint32_t i2 = (uint32_t)((int)i - cx(d)), j2 = (uint32_t)((int)j - cy(d));
@@ -501,11 +543,11 @@ extern "C" int lbm_g_collide(const lbm_lattice* lat, const lbm_gfields* g, const
 extern "C" int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                               const int32_t* group, int32_t count, void* stream) {
   GRANULAR_PROLOGUE
-  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_PRESSURE_NEEM) return fail("bad boundary kind %d", kind);
+  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_OPEN_NEUMANN) return fail("bad boundary kind %d", kind);
   if ((kind == LBM_BC_INSIDE || kind == LBM_BC_BOUNCEBACK) && !g->neighbor)
     return fail("boundary kind %d needs the neighbour table", kind);
   if (kind >= LBM_BC_VELOCITY_NEEM) {
-    if (!rule) return fail("NEEM boundary needs its rule");
+    if (!rule) return fail("boundary kind %d needs its rule (direction, value)", kind);
     if (rule->direction < 1 || rule->direction > 4) return fail("NEEM direction %d", rule->direction);
     a.rule = *rule;
   }

@@ -190,6 +190,44 @@ class PressureNEEM(PressureBoundary):
         self._granular(lb_field, H.BC_PRESSURE_NEEM, self.rule())
 
 
+class VelocityBB(VelocityBoundary):
+    """BoundaryCondition.py:80-102: adds 6 w_d rho c_d . u_w to the unknown
+    populations (after an earlier bounce-back has written them)."""
+
+    kind = "velocity_bb"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = 0, int(self.direction)
+        r.ux, r.uy = (float(np.float32(v)) for v in self.velocity_value)
+        return r
+
+    def fused_spec(self, name=""):
+        ux, uy = (float(np.float32(v)) for v in self.velocity_value)
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), ux, uy, 0.0, name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_VELOCITY_BB, self.rule())
+
+
+class OpenNeumann(PlaneBoundary):
+    """BoundaryCondition.py:315-334: zero-gradient outlet -- copies the unknown
+    populations from x - c_d, then recomputes density and velocity at x."""
+
+    kind = "open_neumann"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = 0, int(self.direction)
+        return r
+
+    def fused_spec(self, name=""):
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), name=name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_OPEN_NEUMANN, self.rule())
+
+
 class BounceBackWall(BoundaryCondition):
     """BoundaryCondition.py:231-259 (halfway bounce-back through the neighbour table)."""
 
@@ -231,14 +269,12 @@ def _out_of_scope(name, where):
     def init(self, *a, **k):
         raise NotImplementedError(
             f"{name} ({where}) is not part of the MI355X hot path; supported boundaries: PeriodicAllBoundary, "
-            "InsideBoundary, BounceBackWall, VelocityNEEM, PressureNEEM, FluidBoundary")
+            "InsideBoundary, BounceBackWall, VelocityNEEM, PressureNEEM, VelocityBB, OpenNeumann, FluidBoundary")
     return type(name, (BoundaryCondition,), {"__init__": init})
 
 
-VelocityBB = _out_of_scope("VelocityBB", "BoundaryCondition.py:80-102")
 VelocityEquilibrium = _out_of_scope("VelocityEquilibrium", "BoundaryCondition.py:132-147")
 PressureABC = _out_of_scope("PressureABC", "BoundaryCondition.py:177-205")
-OpenNeumann = _out_of_scope("OpenNeumann", "BoundaryCondition.py:315-334")
 OpenConvective1order = _out_of_scope("OpenConvective1order", "BoundaryCondition.py:337-359")
 OpenConvective2order = _out_of_scope("OpenConvective2order", "BoundaryCondition.py:362-390")
 OpenExtrapolation = _out_of_scope("OpenExtrapolation", "BoundaryCondition.py:394-415")

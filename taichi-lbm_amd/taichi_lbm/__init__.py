@@ -18,6 +18,7 @@ from .SCBoundaryCondition import (BoundaryEngineSC, OpenNeumannPsi, OpenConvecti
                                   FixedPsi)
 from .EOS import SC_psi, vdW_psi, RK_psi, PR_psi, CS_psi
 from .FluidModels import PowerLawFluid, NewtonianFluid
+from .PostProcessing import PostProcessingEngine
 from .stepper import Stepper
 from ._codes import UnsupportedBoundaryList
 from ._hip import LBMError

@@ -15,7 +15,8 @@ STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT, STORE_F64 = 0, 1, 2, 3
 SC_NONE, SC_C1, SC_C2 = 0, 1, 2
 TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
 NEEM_VELOCITY, NEEM_PRESSURE = 1, 2
-BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM = 1, 2, 3, 4, 5
+BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM, BC_VELOCITY_BB, BC_OPEN_NEUMANN = \
+    1, 2, 3, 4, 5, 6, 7
 
 CODE_BOUNCE_MASK = 0x00FF
 CODE_ACTIVE = 0x0100

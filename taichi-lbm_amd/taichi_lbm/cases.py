@@ -293,3 +293,45 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
                 spec=dict(kind="contraction", NX=NX, NY=NY, power_n=power_n, power_m=power_m, u_in=u_in,
                           nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max))
+
+
+def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt", tau_bulk=None,
+                           name="cylinder_ref"):
+    """The boundary list of cases/Circular cylinder.py:72-133 with bounds-checked
+    predicates: cylinder at (3/8 NX - 1/2, NY/2 - 1/2), radius NY/10 - 1/2;
+    periodic streaming; bounce-back on the first/last columns and around the
+    cylinder; VelocityBB inlet (d=3) on column 0; OpenNeumann outlet (d=1) on
+    the last column.  (The reference's wall predicate reads the mask out of
+    range at the top/bottom rows; here those reads count as fluid, i.e. the
+    channel is periodic in y as the streaming is.)"""
+    from .BoundaryCondition import OpenNeumann, VelocityBB
+    lbf = LBField(name, NX, NY, 1)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    bce.Mask_cricle_identify(lbf, 3 * NX / 8 - 0.5, NY / 2 - 0.5, NY / 10 - 0.5)
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    periodic.precompute(clf)
+    bce.add_boundary_condition("periodic", periodic)
+    wmap = solid_neighbour_map(m)
+    wmap[0, :] |= m[0, :] == 1
+    wmap[-1, :] |= m[-1, :] == 1
+    wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+    wall.precompute(clf)
+    bce.add_boundary_condition("wall", wall)
+    inlet = VelocityBB(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0)), (u_in, 0.0), 3)
+    inlet.precompute(clf)
+    bce.add_boundary_condition("inlet", inlet)
+    outlet = OpenNeumann(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1)), 1)
+    outlet.precompute(clf)
+    bce.add_boundary_condition("outlet", outlet)
+    lbf.neighbor_classify()
+    nu = (tau - 0.5) / 3.0
+    nu_b = None if tau_bulk is None else [(tau_bulk - 0.5) / 3.0]
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [nu], nu_bulk=nu_b)
+    macro, rho, vel = _finish(lbf, bce, fluid, coll, u0=(u_in, 0.0))
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
+                spec=dict(kind="cylinder_ref", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision))

@@ -27,7 +27,7 @@ def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None):
     for name, bc in case["bce"].boundary_conditions.items():
         k = bc.kind
         val = None
-        if k == "velocity_neem":
+        if k in ("velocity_neem", "velocity_bb"):
             val = bc.velocity_value
         elif k == "pressure_neem":
             val = bc.rho_value

@@ -1,0 +1,86 @@
+"""The benchmark configurations at (or near) full size, checked through the
+C++ restatement (a second CPU oracle, fast enough for long runs), the
+granular GPU path, the FP64 path and size-independent invariants."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+from taichi_lbm import cases
+from taichi_lbm.batch import BatchRunner, spec_from_case
+from taichi_lbm.stepper import Stepper
+from tolerances import check
+
+pytestmark = pytest.mark.gpu
+
+
+def _quiet(fn, *a, **k):
+    with contextlib.redirect_stdout(io.StringIO()):
+        return fn(*a, **k)
+
+
+def test_cylinder_400x100_10k_steps():
+    """Plain-method cylinder: BGK tau = 0.6, r = 10 at (80, 50), VelocityNEEM
+    inlet u = 0.05, PressureNEEM outlet rho = 1, f = w at start, 10k steps."""
+    n = 10_000
+    case = _quiet(cases.cylinder)
+    cl = cpu_ref.from_case(case, nthreads=16)
+    cl.run(n)
+    _quiet(cases.initialise, case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"])
+    st.run(n)
+    lbf = case["lbf"]
+    for name in ("f", "f2", "rho", "vel", "total_rho"):
+        check(name, getattr(lbf, name).to_numpy(), getattr(cl, name))
+    # error of the FP32 field against the FP64 path
+    r64 = BatchRunner([spec_from_case(_quiet(cases.cylinder))], storage="f64")
+    r64.run(n)
+    err = np.abs(lbf.vel.to_numpy() - r64.fields(0)["vel"]).max()
+    print(f"  max |u_fp32 - u_fp64| after 10k steps = {err:.2e}")
+    assert err < 1e-4
+    # steady (Re = 30): the field has stopped changing
+    u = lbf.vel.to_numpy()
+    st.run(100)
+    du = np.abs(lbf.vel.to_numpy() - u).max()
+    print(f"  max |du| over 100 more steps = {du:.2e}")
+    assert du < 1e-4
+
+
+def test_porous_512_fused_vs_granular_gpu():
+    case = _quiet(cases.porous, NX=512, NY=512, seed=11, collision="mrt", rmin=8, rmax=32)
+    _quiet(cases.initialise, case)
+    twin = _quiet(cases.porous, NX=512, NY=512, seed=11, collision="mrt", rmin=8, rmax=32)
+    _quiet(cases.initialise, twin)
+    n = 60
+    Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True).run(n)
+    lbf, macro, col, bce = twin["lbf"], twin["macro"], twin["coll"], twin["bce"]
+    for _ in range(n):
+        macro.density(lbf)
+        macro.pressure(lbf)
+        macro.force_density(lbf)
+        macro.velocity(lbf)
+        col.shear_rate(lbf)
+        col.relaxation_time(lbf)
+        col.apply(lbf)
+        bce.apply_boundary_conditions(lbf)
+    for name in ("f", "f2", "rho", "vel", "shear_rate"):
+        check(name, getattr(case["lbf"], name).to_numpy(), getattr(lbf, name).to_numpy())
+
+
+@pytest.mark.parametrize("N", [256, 2048])
+def test_two_component_shan_chen_mass_fp64(N):
+    """Two components, G_cross = 0.9, periodic, rho_c = 1 +- 1 %: the FP64 sums
+    of each component's mass stay at their initial values (FP32 storage)."""
+    case = _quiet(cases.shan_chen, NX=N, NY=N, nc=2, gcross=0.9, seed=3)
+    _quiet(cases.initialise, case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
+    m0 = case["rho"].astype(np.float64).sum(axis=(0, 1))
+    steps = 2000 if N == 256 else 400
+    st.run(steps)
+    red = st.reduce()
+    drift = np.abs(np.array(red["mass"]) - m0) / m0
+    print(f"  N={N}: FP64 mass drift per component after {steps} steps = {drift}")
+    assert red["nonfinite"] == 0
+    assert np.all(drift < 1e-5)

@@ -170,3 +170,22 @@ def test_mass_reduction_fp64():
     assert np.allclose(red["mass"], rho.sum(axis=(0, 1)), rtol=1e-12)
     assert np.allclose(red["mass"], m0, rtol=1e-5)     # fp32 storage, periodic: conserved to rounding
     assert red["nonfinite"] == 0
+
+
+@pytest.mark.parametrize("coll", ["bgk", "mrt"])
+def test_reference_cylinder_bcs_granular(coll):
+    """cases/Circular cylinder.py's boundary list (VelocityBB inlet, OpenNeumann
+    outlet) on the granular kernels against the oracle."""
+    case = cases.initialise(cases.cylinder_reference_bcs(NX=120, NY=32, collision=coll, tau_bulk=0.9))
+    lbf, macro, col, bce = case["lbf"], case["macro"], case["coll"], case["bce"]
+    n = 25
+    for _ in range(n):
+        macro.density(lbf)
+        macro.pressure(lbf)
+        macro.force_density(lbf)
+        macro.velocity(lbf)
+        col.apply(lbf)
+        bce.apply_boundary_conditions(lbf)
+    sol = oracle_solver(case, np.float32)
+    sol.run(n)
+    compare_state(lbf, sol.lat, ("f", "f2", "rho", "vel", "total_rho"))

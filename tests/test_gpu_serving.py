@@ -1,0 +1,67 @@
+"""Continuous batching: each result is the same alone and in the mix, and
+admission respects the HBM budget."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+
+from taichi_lbm import cases
+from taichi_lbm.batch import footprint_bytes, spec_from_case
+from taichi_lbm.serving import Request, SolveServer, solve_alone
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    out = []
+    with contextlib.redirect_stdout(io.StringIO()):
+        for q, (W, Hh, n) in enumerate([(96, 32, 0.7), (128, 40, 1.3), (80, 24, 1.0), (112, 48, 0.5)]):
+            out.append(spec_from_case(cases.contraction(W=W, H=Hh, power_n=n, u_in=0.03, name=f"s{q}")))
+    return out
+
+
+def test_alone_equals_in_the_mix():
+    specs = _specs()
+    arrivals = [0.0, 0.0, 0.02, 0.05]
+    reqs = [Request(q, arrivals[q], s, max_steps=1200, tol=1e-9) for q, s in enumerate(specs)]
+    srv = SolveServer(storage="f32", hbm_budget_bytes=1 << 28, max_slots=8, tick_steps=100)
+    stats = srv.serve(reqs)
+    srv.close()
+    assert len(stats.completed) == 4
+    for r in stats.completed:
+        u, steps, res = solve_alone(specs[r.rid], 1200, 1e-9, 100, "f32")
+        print(f"  request {r.rid}: steps {r.steps} vs {steps}, residual {r.residual:.2e}", flush=True)
+        assert steps == r.steps
+        assert np.array_equal(u, r.result)
+        assert np.all(np.isfinite(u))
+
+
+def test_admission_respects_budget():
+    specs = _specs()
+    budget = footprint_bytes(specs[3]) + footprint_bytes(specs[1]) + 1
+    reqs = [Request(q, 0.0, s, max_steps=400, tol=0.0) for q, s in enumerate(specs)]
+    srv = SolveServer(storage="f32", hbm_budget_bytes=budget, max_slots=8, tick_steps=100)
+    stats = srv.serve(reqs)
+    srv.close()
+    assert len(stats.completed) == 4
+    assert stats.peak_bytes <= budget
+    assert stats.peak_resident <= 3
+    assert all(r.steps == 400 for r in stats.completed)
+    # later requests waited for memory: their admission is after the first finish
+    first_finish = min(r.finished for r in stats.completed)
+    assert max(r.admitted for r in stats.completed) >= first_finish - 1e-9
+
+
+def test_shifted_storage_runs_in_the_server():
+    specs = _specs()[:2]
+    for storage in ("f16", "bf16"):
+        reqs = [Request(q, 0.0, s, max_steps=600, tol=0.0) for q, s in enumerate(specs)]
+        srv = SolveServer(storage=storage, hbm_budget_bytes=1 << 28, max_slots=4, tick_steps=100)
+        stats = srv.serve(reqs)
+        srv.close()
+        for r in stats.completed:
+            u64, _, _ = solve_alone(specs[r.rid], 600, 0.0, 100, "f64")
+            err = np.abs(r.result - u64).max()
+            print(f"  {storage} request {r.rid}: max |u - u_fp64| = {err:.2e}")
+            assert err < (3e-4 if storage == "f16" else 3e-3)

@@ -466,3 +466,44 @@ def test_shan_chen_flat_interface_coexistence():
         assert abs(gas / eg - 1) < 5e-3, (G, gas, eg)
         # mass is conserved exactly by the force (sum of F over a periodic strip is 0)
         assert abs(lat.rho.sum() - np.where((J >= 16) & (J < 48), rl, rg).sum()) < 1e-10
+
+
+def test_couette_moving_wall_velocity_bb():
+    # bottom wall at rest, top wall moving with U (bounce-back + VelocityBB,
+    # BoundaryCondition.py:85-102): the steady profile is linear, u = U (y + 1/2) / NY
+    NX, NY, U, tau = 2, 10, 1e-3, 0.8
+    lat, grp = _lat_all_fluid(NX, NY)
+    for i in range(NX):
+        for j in range(NY):
+            for k in range(9):
+                i2, j2 = (i - O.C[k, 0]) % NX, j - O.C[k, 1]
+                lat.neighbor[i, j, k] = (i2, j2) if 0 <= j2 < NY else (-1, -1)
+    walls = np.argwhere(np.isin(np.arange(NY)[None, :].repeat(NX, 0), [0, NY - 1]))
+    top = np.argwhere(np.arange(NY)[None, :].repeat(NX, 0) == NY - 1)
+    lat.rho[..., 0] = 1.0
+    O.init_lbm(lat, grp)
+    nu = (tau - 0.5) / 3
+    col = O.Collision("trt", grp, O.FluidModel("newtonian", nu=nu), [nu], magic=[3 / 16])
+    col.init_tau(lat)
+    s = O.Solver(lat, grp, col, [O.BC("periodic", grp), O.BC("bounceback", walls),
+                                 O.BC("velocity_bb", top, 2, (U, 0.0))])
+    s.run(3000)
+    O.density(lat, grp)
+    O.velocity(lat, grp)
+    u = lat.vel[0, :, 0]
+    exact = U * (np.arange(NY) + 0.5) / NY
+    assert np.allclose(u, exact, rtol=1e-6, atol=1e-12)
+
+
+def test_open_neumann_copies_unknowns_and_recomputes_moments():
+    lat, grp = _lat_all_fluid(5, 4)
+    u0 = np.array([0.03, -0.01])
+    lat.f[:, :, 0, :] = O.f_eq(np.tile(u0, (20, 1)), np.full(20, 1.02), f64).reshape(5, 4, 9)
+    rng = np.random.default_rng(4)
+    lat.f[4, :, 0, :] += 1e-3 * rng.standard_normal((4, 9))      # outlet column perturbed
+    line = np.array([[4, j] for j in range(4)])
+    O.BC("open_neumann", line, 1).apply(lat)
+    assert np.allclose(lat.f[4, :, 0, [3, 6, 7]], lat.f[3, :, 0, [3, 6, 7]])
+    assert np.allclose(lat.rho[4, :, 0], lat.f[4, :, 0, :].sum(-1))
+    m = lat.f[4, :, 0, :] @ O.C.astype(f64)
+    assert np.allclose(lat.vel[4], m / lat.rho[4, :, 0][:, None])
