@@ -104,12 +104,16 @@ def cpu_baseline(case, steps):
             "seconds": round(dt, 2)}
 
 
-def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100):
+def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100, n_fp64=16):
     """512 independent cylinder channels (NX ~ U{128..1024}, NY ~ U{64..256},
     r ~ U[4, NY/8] at (max(NX/5, r + 4), NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
-    BGK or TRT (magic 1/4) by a seeded coin), FP32: one persistent launch per
-    step replayed from a graph, against the plain path (one lbm_run launch per
-    lattice per step)."""
+    BGK or TRT (magic 1/4) by a seeded coin; bounce-back walls, VelocityBB
+    inlet, PressureNEEM outlet), FP32: one persistent launch per step
+    replayed from a graph, against the plain path (one lbm_run launch per
+    lattice per step).  Afterwards every lattice is run to solve_steps in the
+    batch and the non-finite ones are counted; a sample is rerun in FP64 and
+    the largest velocity difference reported (the plain and batched paths
+    give bit-identical fields, so one number describes both)."""
     import ctypes
     import numpy as np
     import torch
@@ -125,7 +129,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             r, u = float(rng.uniform(4, NY / 8)), float(rng.uniform(0.01, 0.08))
             tau, coll = float(rng.uniform(0.52, 0.8)), ("bgk" if rng.random() < 0.5 else "trt")
             c = cases.cylinder(NX=NX, NY=NY, cx=max(NX / 5, r + 4), cy=NY / 2, r=r, tau=tau, u_in=u,
-                               collision=coll, name=f"b{q}")
+                               collision=coll, name=f"b{q}", inlet="bb")
             specs.append(spec_from_case(c))
     nodes = sum(s.N for s in specs)
     fluid = sum(int(((s.code & H.CODE_ACTIVE) != 0).sum()) for s in specs)
@@ -167,6 +171,17 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         ms_p = e0.elapsed_time(e1) / plain_steps
         g.close()
         runner.close()
+        # accuracy at solve_steps: a fresh batch run to the end of the solve
+        runner = BatchRunner(specs, storage="f32", grid_blocks=grid)
+        runner.run(solve_steps)
+        u32 = [runner.velocity(l) for l in range(n_lattices)]
+        runner.close()
+        finite = [bool(np.isfinite(u).all()) for u in u32]
+        sample = [l for l in range(n_lattices) if finite[l]][:n_fp64]
+        r64 = BatchRunner([specs[l] for l in sample], storage="f64", grid_blocks=grid)
+        r64.run(solve_steps)
+        err64 = max(float(np.abs(u32[l] - r64.velocity(q)).max()) for q, l in enumerate(sample)) if sample else None
+        r64.close()
     sps_b = n_lattices / (ms_b * 1e-3 * solve_steps)
     sps_p = n_lattices / (ms_p * 1e-3 * solve_steps)
     return {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid, "steps_per_solve": solve_steps,
@@ -175,7 +190,9 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             "hbm_frac_batched": round(fluid * 72 / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "solves_per_s_batched": round(sps_b, 3), "solves_per_s_plain": round(sps_p, 3),
             "speedup": round(sps_b / sps_p, 2), "grid_blocks": grid, "graph_steps": graph_steps,
-            "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps}
+            "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps,
+            "finite_after_solve": int(sum(finite)), "fp64_sample": len(sample),
+            "max_abs_u_err_vs_fp64": err64}
 
 
 def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16", "bf16"), n_fp64=3):
@@ -255,7 +272,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-trt", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -264,7 +281,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--only-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 8192^2 MRT power-law timing (for profiling runs)")
     args = ap.parse_args()
+    if args.headline_only:
+        args.no_cpu = args.no_trt = args.no_batched = args.no_serving = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 /* status codes */
 #define LBM_OK 0
@@ -58,11 +58,18 @@ enum { LBM_SC_NONE = 0, LBM_SC_C1 = 1, LBM_SC_C2 = 2 };
  * boundary-condition list (BoundaryCondition.py:459-483).               */
 #define LBM_CODE_BOUNCE_MASK 0x00FFu /* bit k-1 set: population k takes f2(x, opp k)   */
 #define LBM_CODE_ACTIVE 0x0100u      /* node belongs to the fluid (collision) group     */
-#define LBM_CODE_KEEP_MACRO 0x0200u  /* rho/u of this node are read by a NEEM rule      */
-#define LBM_CODE_NEEM_SHIFT 10       /* bits 10..13: NEEM rule index + 1 (0 = none)     */
+#define LBM_CODE_KEEP_MACRO 0x0200u  /* rho/u of this node are read by a node rule      */
+#define LBM_CODE_NEEM_SHIFT 10       /* bits 10..13: node rule index + 1 (0 = none)     */
 #define LBM_CODE_NEEM_MASK 0x3C00u
 
-enum { LBM_NEEM_VELOCITY = 1, LBM_NEEM_PRESSURE = 2 };
+/* Node-rule kinds: the boundary conditions that act on a node after its
+ * populations were streamed or bounced.                                  */
+enum {
+  LBM_NEEM_VELOCITY = 1,     /* VelocityNEEM.apply, BoundaryCondition.py:149-166 */
+  LBM_NEEM_PRESSURE = 2,     /* PressureNEEM.apply, :209-226                     */
+  LBM_RULE_VELOCITY_BB = 3,  /* VelocityBB.apply, :85-102 (adds 6 w_k rho c_k.u) */
+  LBM_RULE_OPEN_NEUMANN = 4  /* OpenNeumann.apply, :321-334 (unknowns := f(x_nb)) */
+};
 
 /* Where the relaxation time comes from.  The per-node state (lbm_fields.tau,
  * lbm_gfields.tau) holds BGK tau, TRT tau_sym or MRT diag[7] (= 1/tau); TRT
@@ -77,12 +84,12 @@ enum {
 #define LBM_MAX_NEEM 15
 #define LBM_MAX_NC 2
 
-/* One non-equilibrium-extrapolation rule.  VelocityNEEM
- * (BoundaryCondition.py:149-166) or PressureNEEM (:209-226).             */
+/* One node rule: VelocityNEEM (BoundaryCondition.py:149-166), PressureNEEM
+ * (:209-226), VelocityBB (:85-102) or OpenNeumann (:321-334).            */
 typedef struct lbm_neem_rule {
-  int32_t kind;      /* LBM_NEEM_VELOCITY | LBM_NEEM_PRESSURE               */
+  int32_t kind;      /* LBM_NEEM_VELOCITY | LBM_NEEM_PRESSURE | LBM_RULE_*   */
   int32_t direction; /* outward population index d in 1..4; x_nb = x - c_d */
-  float ux, uy;      /* velocity_value (velocity rule)                      */
+  float ux, uy;      /* velocity_value (velocity NEEM, VelocityBB)          */
   float rho;         /* rho_value (pressure rule; written to total_rho only)*/
   int32_t pad;
 } lbm_neem_rule;

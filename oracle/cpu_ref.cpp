@@ -35,7 +35,8 @@ typedef struct cref_params {
 } cref_params;
 
 typedef struct cref_bc {
-  int32_t kind;  // 1 periodic, 2 inside, 3 bounceback, 4 velocity NEEM, 5 pressure NEEM
+  int32_t kind;  // 1 periodic, 2 inside, 3 bounceback, 4 velocity NEEM, 5 pressure NEEM,
+                 // 6 VelocityBB, 7 OpenNeumann
   int32_t direction;
   float ux, uy, rho;
   const int32_t* nodes;  // (count, 2)
@@ -49,6 +50,8 @@ namespace {
 const int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 const int CY[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
 const int OPPK[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+// unknown populations of a plane boundary with outward direction d (BoundaryCondition.py:63-72)
+const int UNK[5][3] = {{0, 0, 0}, {3, 6, 7}, {4, 7, 8}, {1, 5, 8}, {2, 5, 6}};
 const float W[9] = {4.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f, 1.0f / 9.0f,
                     0.25f / 9.0f, 0.25f / 9.0f, 0.25f / 9.0f, 0.25f / 9.0f};
 const float MM[9][9] = {
@@ -259,10 +262,40 @@ void step(Ctx& C, const cref_bc* bcs, int nbc) {
             else if (bc.kind == 3) C.F(i, j, c)[k] = C.F2(i, j, c)[OPPK[k]];
           }
         }
+      } else if (bc.kind == 6) {
+        // VelocityBB, BoundaryCondition.py:95-102
+        for (int c = 0; c < nc; ++c) {
+          const float rw = C.rho[n * nc + c];
+          for (int u = 0; u < 3; ++u) {
+            const int k = UNK[d][u];
+            const float cv = (float)CX[k] * bc.ux + (float)CY[k] * bc.uy;
+            C.F(i, j, c)[k] += 6 * W[k] * rw * cv;
+          }
+        }
       } else {
         const int i2 = i - CX[d], j2 = j - CY[d];
         const size_t n2 = C.node(i2, j2);
-        if (bc.kind == 4) {
+        if (bc.kind == 7) {
+          // OpenNeumann, :327-334, then OpenBoundary.density / vel, :274-298
+          float t = 0.0f, vx = 0.0f, vy = 0.0f;
+          for (int c = 0; c < nc; ++c) {
+            float* q = C.F(i, j, c);
+            for (int u = 0; u < 3; ++u) q[UNK[d][u]] = C.F(i2, j2, c)[UNK[d][u]];
+            float r = 0.0f, ax = 0.0f, ay = 0.0f;
+            for (int k = 0; k < 9; ++k) {
+              r += q[k];
+              ax += q[k] * CX[k];
+              ay += q[k] * CY[k];
+            }
+            C.rho[n * nc + c] = r;
+            t += r;
+            vx += ax + 0.5f * C.bf[(n * nc + c) * 2];
+            vy += ay + 0.5f * C.bf[(n * nc + c) * 2 + 1];
+          }
+          C.trho[n] = t;
+          C.vel[2 * n] = vx / t;
+          C.vel[2 * n + 1] = vy / t;
+        } else if (bc.kind == 4) {
           C.vel[2 * n] = bc.ux;
           C.vel[2 * n + 1] = bc.uy;
           for (int c = 0; c < nc; ++c) {
@@ -284,7 +317,6 @@ void step(Ctx& C, const cref_bc* bcs, int nbc) {
           C.vel[2 * n] = C.vel[2 * n2];
           C.vel[2 * n + 1] = C.vel[2 * n2 + 1];
           C.trho[n] = bc.rho;
-          static const int UNK[5][3] = {{0, 0, 0}, {3, 6, 7}, {4, 7, 8}, {1, 5, 8}, {2, 5, 6}};
           for (int c = 0; c < nc; ++c) {
             float a[9], b2[9];
             feq(C.vel[2 * n], C.vel[2 * n + 1], C.rho[n * nc + c], a);

@@ -24,7 +24,8 @@ class BC(C.Structure):
                 ("rho", C.c_float), ("nodes", C.c_void_p), ("count", C.c_int32)]
 
 
-KIND = {"periodic": 1, "inside": 2, "bounceback": 3, "velocity_neem": 4, "pressure_neem": 5}
+KIND = {"periodic": 1, "inside": 2, "bounceback": 3, "velocity_neem": 4, "pressure_neem": 5, "velocity_bb": 6,
+        "open_neumann": 7}
 
 
 def build():
@@ -103,7 +104,7 @@ def from_case(case, nthreads=0):
         if bc.kind == "fluid":
             continue
         d = {"kind": bc.kind, "nodes": bc.group.nodes, "direction": getattr(bc, "direction", 0)}
-        if bc.kind == "velocity_neem":
+        if bc.kind in ("velocity_neem", "velocity_bb"):
             d["u"] = bc.velocity_value
         if bc.kind == "pressure_neem":
             d["rho"] = bc.rho_value

@@ -4,8 +4,9 @@
 // of step n-1 and produces those of step n, doing in between, per node:
 //   S7(n-1)  the reference's ordered boundary pass (BoundaryCondition.py:
 //            481-483): periodic / table pull, halfway bounce-back (:244-259),
-//            VelocityNEEM (:154-166) and PressureNEEM (:215-226), compiled on
-//            the host into one uint16 code per node;
+//            VelocityNEEM (:154-166), PressureNEEM (:215-226), VelocityBB
+//            (:85-102) and OpenNeumann (:321-334), compiled on the host into
+//            one uint16 code per node;
 //   S1..S6(n) density (lbm.py:153-164), force_density (:215-220), velocity
 //            (:223-237), power-law shear_rate/relaxation_time
 //            (CollisionEngine.py:72-92, 157-178, 322-377) and the BGK / TRT /
@@ -103,16 +104,44 @@ __device__ __forceinline__ void gather(const typename Pop<S>::T* __restrict__ sr
   }
 }
 
-// Apply NEEM rule `r` at node (i, j) whose pre-rule populations are f.
-// Reads the interior neighbour's post-stream populations (re-gathered) and
-// its rho/u of the previous step (mac_in); on `finish` also writes the
-// reference's vel / rho / total_rho overrides into mac_override.
+// Post-stream population k of component c at node nb, following nb's own
+// code (pull, or halfway bounce-back where its bit is set).
+template <int S>
+__device__ __forceinline__ typename Pop<S>::R pop_at(const typename Pop<S>::T* __restrict__ src, uint32_t ps,
+                                                     const Nbr& nb, uint32_t bmask, int c, int k) {
+  const bool bounce = (k > 0) && ((bmask >> (k - 1)) & 1u);
+  const int kk = bounce ? opp(k) : k;
+  return Pop<S>::ld(src + (size_t)(c * 9 + kk) * ps, bounce ? nb.n : nb.src(k), kk);
+}
+
+// Apply node rule `r` at node (i, j) whose pre-rule populations are f.
+// NEEM rules read the interior neighbour's post-stream populations
+// (re-gathered) and its rho/u of the previous step (mac_in); VelocityBB reads
+// the node's own rho of the previous step; OpenNeumann copies the unknowns
+// from the neighbour.  On `finish` the reference's vel / rho / total_rho
+// overrides are written into mac_override.
 template <int NC, int S>
 __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const float* mi,
                            const lbm_neem_rule& r, uint32_t i, uint32_t j, uint32_t n,
                            typename Pop<S>::R (&f)[NC][9], float* mac_override) {
   using R = typename Pop<S>::R;
   const int d = r.direction;
+  const uint32_t ps = a.ps;
+  if (r.kind == LBM_RULE_VELOCITY_BB) {
+    // BoundaryCondition.py:95-102: f_k += 6 w_k rho_c(x) (c_k . u_w) on the unknowns
+    const R uxw = r.ux, uyw = r.uy;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const R rw = mi[(size_t)c * ps + n];
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (neem_unknown(d, k)) {
+          const R cv = (R)cx(k) * uxw + (R)cy(k) * uyw;
+          f[c][k] = f[c][k] + R(6) * wgt<R>(k) * rw * cv;
+        }
+    }
+    return;
+  }
   int di = 0, dj = 0;
   switch (d) {
     case 1: di = 1; break;
@@ -123,9 +152,41 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const fl
   const uint32_t inb = (uint32_t)((int)i - di), jnb = (uint32_t)((int)j - dj);
   const Nbr nb(inb, jnb, a.nx, a.ny);
   const uint32_t code_nb = a.code[nb.n];
+  if (r.kind == LBM_RULE_OPEN_NEUMANN) {
+    // BoundaryCondition.py:327-334: unknowns := f(x - c_d), then density and
+    // velocity of the node (OpenBoundary.density / vel, :274-298)
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (neem_unknown(d, k)) f[c][k] = pop_at<S>(src, ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, c, k);
+    if (mac_override) {
+      R rtot = R(0), vx = R(0), vy = R(0);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s += f[c][k];
+        mac_override[(size_t)c * ps + n] = (float)s;
+        rtot += s;
+        const R* g = f[c];
+        R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
+        R ty = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
+        if ((a.force || a.sc != LBM_SC_NONE) && a.body_force) {
+          tx = tx + R(0.5) * (R)a.body_force[(size_t)(2 * c) * ps + n];
+          ty = ty + R(0.5) * (R)a.body_force[(size_t)(2 * c + 1) * ps + n];
+        }
+        vx += tx;
+        vy += ty;
+      }
+      mac_override[(size_t)NC * ps + n] = (float)rtot;
+      mac_override[(size_t)(NC + 1) * ps + n] = (float)(vx / rtot);
+      mac_override[(size_t)(NC + 2) * ps + n] = (float)(vy / rtot);
+    }
+    return;
+  }
   R fn[NC][9];
-  gather<NC, S>(src, a.ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, fn);
-  const uint32_t ps = a.ps;
+  gather<NC, S>(src, ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, fn);
   const R ux_nb = mi[(size_t)(NC + 1) * ps + nb.n];
   const R uy_nb = mi[(size_t)(NC + 2) * ps + nb.n];
   const R uv_nb = ux_nb * ux_nb + uy_nb * uy_nb;
@@ -186,7 +247,9 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
   const T* __restrict__ src = (const T*)a.pop[p];
 
   R f[NC][9];
-  if (MODE == MODE_STEP && local) {
+  // The first launch of a run may read post-stream f in place (no boundary
+  // pass): the psi pass and the step of that launch must both do so.
+  if (MODE != MODE_FINISH && local) {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -487,8 +550,8 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
   if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
   for (int r = 0; r < lat->n_neem; ++r) {
     if (neem[r].direction < 1 || neem[r].direction > 4) return fail("NEEM rule %d: direction %d", r, neem[r].direction);
-    if (neem[r].kind != LBM_NEEM_VELOCITY && neem[r].kind != LBM_NEEM_PRESSURE)
-      return fail("NEEM rule %d: kind %d", r, neem[r].kind);
+    if (neem[r].kind < LBM_NEEM_VELOCITY || neem[r].kind > LBM_RULE_OPEN_NEUMANN)
+      return fail("node rule %d: kind %d", r, neem[r].kind);
   }
   if (!fld->f[0] || !fld->f[1] || !fld->macro[0] || !fld->macro[1] || !fld->code)
     return fail("missing population / macro / code buffer");

@@ -198,7 +198,7 @@ class VelocityBB(VelocityBoundary):
 
     def rule(self):
         r = H.NeemRule()
-        r.kind, r.direction = 0, int(self.direction)
+        r.kind, r.direction = H.RULE_VELOCITY_BB, int(self.direction)
         r.ux, r.uy = (float(np.float32(v)) for v in self.velocity_value)
         return r
 
@@ -218,7 +218,7 @@ class OpenNeumann(PlaneBoundary):
 
     def rule(self):
         r = H.NeemRule()
-        r.kind, r.direction = 0, int(self.direction)
+        r.kind, r.direction = H.RULE_OPEN_NEUMANN, int(self.direction)
         return r
 
     def fused_spec(self, name=""):

@@ -14,13 +14,17 @@ writer.  This module replays the list symbolically on the host:
                                   halfway bounce-back f2(x, opp k)
   VelocityNEEM        (:154-166)  all nine populations
   PressureNEEM        (:215-226)  the three unknown populations
+  VelocityBB          (:85-102)   adds 6 w_k rho(x) c_k.u_w to the three unknowns
+  OpenNeumann         (:321-334)  the three unknowns := f(x - c_d)
   FluidBoundary / base (:51-54, :452-456)  nothing
 
 A table pull with a valid neighbour reads the same node as a wrapped pull, so
-each population resolves to PULL or BOUNCE, plus an optional NEEM rule that
-re-gathers its interior neighbour.  Lists whose reference semantics the fused
-kernel cannot reproduce are rejected with an explanation (never silently
-approximated).
+each population resolves to PULL or BOUNCE, plus an optional node rule (the
+last four kinds) applied after it.  NEEM and OpenNeumann rules re-gather their
+interior neighbour x - c_d, whose populations must then be final (no later
+boundary rewrites them) and carry no rule of their own.  Lists whose reference
+semantics the fused kernel cannot reproduce are rejected with an explanation
+(never silently approximated).
 """
 from __future__ import annotations
 
@@ -46,6 +50,7 @@ class UnsupportedBoundaryList(ValueError):
 class BCSpec:
     """Host description of one boundary-list entry."""
     kind: str                 # fluid | periodic | inside | bounceback | velocity_neem | pressure_neem
+                              # | velocity_bb | open_neumann
     nodes: np.ndarray         # (m, 2) int (i, j)
     direction: int = 0
     ux: float = 0.0
@@ -97,11 +102,13 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
         a = np.asarray(active_nodes).reshape(-1, 2)
         active[a[:, 0] * NY + a[:, 1]] = True
     rule = np.full((N, 9), NONE, np.int8)
-    neem_of = np.zeros(N, np.int32)          # NEEM index + 1 at the node
-    pre_rule = {}                            # neem idx -> rules of its nodes before it ran
-    nb_rule_at = {}                          # neem idx -> rules of its interior neighbours then
+    neem_of = np.zeros(N, np.int32)          # rule index + 1 at the node
+    pre_rule = {}                            # rule idx -> rules of its nodes before it ran
+    nb_rule_at = {}                          # rule idx -> rules of its interior neighbours then
     neem_rules: List[Tuple] = []
-    neem_nodes: List[Tuple[np.ndarray, np.ndarray]] = []
+    neem_nodes: List[Tuple[np.ndarray, np.ndarray]] = []   # (nodes, interior neighbours or None)
+    rule_kinds = {"velocity_neem": H.NEEM_VELOCITY, "pressure_neem": H.NEEM_PRESSURE,
+                  "velocity_bb": H.RULE_VELOCITY_BB, "open_neumann": H.RULE_OPEN_NEUMANN}
     nb_ok = (neighbor.reshape(N, 9) if neighbor.ndim == 3 else neighbor.reshape(N, 9, 2)[:, :, 0] != -1)
 
     for bc in bcs:
@@ -116,32 +123,33 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
                 "path only streams and collides fluid-group nodes")
         if np.any(neem_of[flat] != 0):
             raise UnsupportedBoundaryList(
-                f"boundary '{bc.name}' overwrites a node already set by a NEEM rule; the fused path applies "
-                "NEEM rules last")
+                f"boundary '{bc.name}' overwrites a node already set by a NEEM / VelocityBB / OpenNeumann rule; "
+                "the fused path applies node rules last")
         if bc.kind == "periodic":
             rule[flat, :] = PULL
         elif bc.kind in ("inside", "bounceback"):
             valid = nb_ok[flat]
             rule[flat, :] = np.where(valid, PULL, INVALID if bc.kind == "inside" else BOUNCE)
-        elif bc.kind in ("velocity_neem", "pressure_neem"):
+        elif bc.kind in rule_kinds:
             d = int(bc.direction)
             if d not in UNKNOWN:
-                raise UnsupportedBoundaryList(f"boundary '{bc.name}': NEEM direction must be 1..4, got {d}")
+                raise UnsupportedBoundaryList(f"boundary '{bc.name}': direction must be 1..4, got {d}")
             if len(neem_rules) >= H.MAX_NEEM:
-                raise UnsupportedBoundaryList(f"more than {H.MAX_NEEM} NEEM rules")
+                raise UnsupportedBoundaryList(f"more than {H.MAX_NEEM} node rules")
             e = len(neem_rules)
-            I2, J2 = nodes[:, 0] - C[d, 0], nodes[:, 1] - C[d, 1]
-            if np.any((I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)):
-                raise UnsupportedBoundaryList(
-                    f"boundary '{bc.name}': the interior neighbour x - c_{d} of a node lies outside the lattice")
-            nbf = I2 * NY + J2
-            if not active[nbf].all():
-                raise UnsupportedBoundaryList(f"boundary '{bc.name}': interior neighbour outside the fluid group")
             pre_rule[e] = rule[flat].copy()
-            nb_rule_at[e] = rule[nbf].copy()
+            nbf = None
+            if bc.kind != "velocity_bb":
+                I2, J2 = nodes[:, 0] - C[d, 0], nodes[:, 1] - C[d, 1]
+                if np.any((I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)):
+                    raise UnsupportedBoundaryList(
+                        f"boundary '{bc.name}': the interior neighbour x - c_{d} of a node lies outside the lattice")
+                nbf = I2 * NY + J2
+                if not active[nbf].all():
+                    raise UnsupportedBoundaryList(f"boundary '{bc.name}': interior neighbour outside the fluid group")
+                nb_rule_at[e] = rule[nbf].copy()
             neem_of[flat] = e + 1
-            kind = H.NEEM_VELOCITY if bc.kind == "velocity_neem" else H.NEEM_PRESSURE
-            neem_rules.append((kind, d, float(bc.ux), float(bc.uy), float(bc.rho)))
+            neem_rules.append((rule_kinds[bc.kind], d, float(bc.ux), float(bc.uy), float(bc.rho)))
             neem_nodes.append((flat, nbf))
         else:
             raise UnsupportedBoundaryList(f"boundary kind '{bc.kind}' ('{bc.name}') has no fused form")
@@ -151,11 +159,13 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     final = rule.copy()
     for e, (flat, nbf) in enumerate(neem_nodes):
         final[flat] = pre_rule[e]
+        if nbf is None:
+            continue
         if np.any(neem_of[nbf] != 0):
-            raise UnsupportedBoundaryList("a NEEM node's interior neighbour is itself a NEEM node")
+            raise UnsupportedBoundaryList("a NEEM / OpenNeumann node's interior neighbour carries a node rule too")
         if not np.array_equal(rule[nbf], nb_rule_at[e]):
             raise UnsupportedBoundaryList(
-                "a boundary after a NEEM rule rewrites that rule's interior neighbour")
+                "a boundary after a NEEM / OpenNeumann rule rewrites that rule's interior neighbour")
     act = np.nonzero(active)[0]
     missing = final[act] == NONE
     if missing.any():
@@ -176,8 +186,10 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
         bits |= (final[:, k] == BOUNCE).astype(np.uint32) << (k - 1)
     code[active] = (bits[active] | H.CODE_ACTIVE).astype(np.uint16)
     for e, (flat, nbf) in enumerate(neem_nodes):
+        kind = neem_rules[e][0]
         code[flat] |= np.uint16((e + 1) << H.CODE_NEEM_SHIFT)
-        code[nbf] |= np.uint16(H.CODE_KEEP_MACRO)
-        if neem_rules[e][0] == H.NEEM_PRESSURE:
-            code[flat] |= np.uint16(H.CODE_KEEP_MACRO)
+        if kind in (H.NEEM_VELOCITY, H.NEEM_PRESSURE):
+            code[nbf] |= np.uint16(H.CODE_KEEP_MACRO)        # neighbour's rho/u of the previous step
+        if kind in (H.NEEM_PRESSURE, H.RULE_VELOCITY_BB):
+            code[flat] |= np.uint16(H.CODE_KEEP_MACRO)       # own rho of the previous step
     return code, neem_rules

@@ -14,7 +14,7 @@ LBM_BGK, LBM_TRT, LBM_MRT = 0, 1, 2
 STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT, STORE_F64 = 0, 1, 2, 3
 SC_NONE, SC_C1, SC_C2 = 0, 1, 2
 TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
-NEEM_VELOCITY, NEEM_PRESSURE = 1, 2
+NEEM_VELOCITY, NEEM_PRESSURE, RULE_VELOCITY_BB, RULE_OPEN_NEUMANN = 1, 2, 3, 4
 BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM, BC_VELOCITY_BB, BC_OPEN_NEUMANN = \
     1, 2, 3, 4, 5, 6, 7
 
@@ -25,7 +25,7 @@ CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class NeemRule(C.Structure):

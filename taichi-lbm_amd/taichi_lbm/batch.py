@@ -308,8 +308,12 @@ class BatchRunner:
             self.finished = True
 
     def residuals(self) -> np.ndarray:
-        """Per slot: (sum |u - u_old|^2, sum |u|^2) in FP64 over its active nodes; u_old <- u."""
-        self.commit()
+        """Per slot: (sum |u - u_old|^2, sum |u|^2) in FP64 over its active nodes; u_old <- u.
+
+        u is the velocity field as the reference reads it after a step (the
+        boundary pass's overrides included), as in the error() of
+        cases/Power Law Fluid.py:183-192."""
+        self.finish()
         out = (ctypes.c_double * (2 * self.max_slots))()
         H.check(H.lib().lbm_batch_residual(self.h, self.parity, out, stream()), "lbm_batch_residual")
         return np.array(out).reshape(-1, 2)

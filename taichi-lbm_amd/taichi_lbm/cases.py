@@ -56,9 +56,21 @@ def _finish(lbf, bce, fluid_bc, coll, rho0=1.0, u0=(0.0, 0.0)):
 
 
 def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0, collision="bgk", magic=0.25,
-             tau_bulk=None, name="cylinder"):
+             tau_bulk=None, name="cylinder", inlet="neem"):
     """Cylinder in a channel: bounce-back top/bottom rows and cylinder,
-    VelocityNEEM inlet (d=3) and PressureNEEM outlet (d=1), f = w at start."""
+    PressureNEEM outlet (d=1) on 0 < j < NY-1, f = w at start.  The inlet
+    (i = 0, 0 < j < NY-1, d = 3) is
+
+      "neem"  VelocityNEEM (BoundaryCondition.py:149-166), the plain-method
+              configuration as stated.  With the reference's NEEM semantics
+              (post-stream f of the neighbour against its rho/u of the step
+              before) this inlet next to the bounce-back rows diverges after a
+              few hundred steps (in FP64 as in FP32);
+      "bb"    VelocityBB over a bounce-back inlet column, the inlet every
+              reference case uses (cases/Circular cylinder.py:91-118,
+              cases/contraction flow.py:110-138); stable."""
+    if inlet not in ("neem", "bb"):
+        raise ValueError("inlet must be 'neem' or 'bb'")
     lbf = LBField(name, NX, NY, 1)
     bce = BoundaryEngine()
     clf = BoundaryClassifier((NX, NY))
@@ -71,13 +83,19 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
     periodic.precompute(clf)
     bce.add_boundary_condition("periodic", periodic)
     wmap = solid_neighbour_map(m, y_walls=True)
+    if inlet == "bb":
+        wmap[0, :] |= m[0, :] == 1
     wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
     wall.precompute(clf)
     bce.add_boundary_condition("wall", wall)
-    inlet = VelocityNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1)),
-                         (u_in, 0.0), 3)
-    inlet.precompute(clf)
-    bce.add_boundary_condition("inlet", inlet)
+    inlet_spec = BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1))
+    if inlet == "bb":
+        from .BoundaryCondition import VelocityBB
+        inlet_bc = VelocityBB(inlet_spec, (u_in, 0.0), 3)
+    else:
+        inlet_bc = VelocityNEEM(inlet_spec, (u_in, 0.0), 3)
+    inlet_bc.precompute(clf)
+    bce.add_boundary_condition("inlet", inlet_bc)
     outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
                           rho_out, 1)
     outlet.precompute(clf)
@@ -88,7 +106,8 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
     coll = make_collision(collision, fluid.group, NX, NY, 1, [nu], magic=magic, nu_bulk=nu_b)
     macro, rho, vel = _finish(lbf, bce, fluid, coll)
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
-                spec=dict(kind="cylinder", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision, magic=magic))
+                spec=dict(kind="cylinder", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision, magic=magic,
+                          inlet=inlet))
 
 
 def place_disks(NX, NY, seed, rmin=8.0, rmax=32.0, solid_fraction=0.3, margin=2.0, max_tries=10_000_000):
@@ -250,13 +269,18 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
 
 
 def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt", magic=0.25, tau_min=0.51,
-                tau_max=2.0, name="contraction"):
+                tau_max=2.0, name="contraction", inlet="bb"):
     """4:1 planar contraction scaled from cases/contraction flow.py:16-33,80-81
     (a 40 x 8 channel with solid blocks over x in (25, 35) leaving the gap
-    y in (3, 5)): VelocityNEEM inlet (d=3, u_in), PressureNEEM outlet (d=1,
-    rho=1), bounce-back walls, power-law fluid with m chosen so that the
+    y in (3, 5)): VelocityBB inlet over a bounce-back inlet column (d=3,
+    u_in; as the reference case, :110-138; inlet="neem" gives a VelocityNEEM
+    inlet instead, which diverges like the cylinder's), PressureNEEM outlet
+    (d=1, rho=1), bounce-back walls, power-law fluid with m chosen so that the
     apparent viscosity is nu_ref at the inlet shear rate u_in / (H/2), tau
-    clamped to [tau_min, tau_max]."""
+    clamped to [tau_min, tau_max].  The reference's table stream
+    (InsideBoundary) reads outside the field at the inlet column; the
+    equivalent periodic pull is used, the bounce-back column then replacing
+    what would come from outside."""
     NX, NY = int(W), int(H)
     lbf = LBField(name, NX, NY, 1)
     bce = BoundaryEngine()
@@ -272,13 +296,19 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     periodic.precompute(clf)
     bce.add_boundary_condition("periodic", periodic)
     wmap = solid_neighbour_map(m, y_walls=True)
+    if inlet == "bb":
+        wmap[0, :] |= m[0, :] == 1
     wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
     wall.precompute(clf)
     bce.add_boundary_condition("wall", wall)
-    inlet = VelocityNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1)),
-                         (u_in, 0.0), 3)
-    inlet.precompute(clf)
-    bce.add_boundary_condition("inlet", inlet)
+    inlet_spec = BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0) & (J > 0) & (J < NY - 1))
+    if inlet == "bb":
+        from .BoundaryCondition import VelocityBB
+        inlet_bc = VelocityBB(inlet_spec, (u_in, 0.0), 3)
+    else:
+        inlet_bc = VelocityNEEM(inlet_spec, (u_in, 0.0), 3)
+    inlet_bc.precompute(clf)
+    bce.add_boundary_condition("inlet", inlet_bc)
     outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
                           1.0, 1)
     outlet.precompute(clf)
@@ -292,7 +322,7 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     macro, rho, vel = _finish(lbf, bce, fluid, coll)
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
                 spec=dict(kind="contraction", NX=NX, NY=NY, power_n=power_n, power_m=power_m, u_in=u_in,
-                          nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max))
+                          nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max, inlet=inlet))
 
 
 def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt", tau_bulk=None,

@@ -7,28 +7,52 @@ import numpy as np
 from oracle import lbm_oracle as O
 
 
-def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None):
+def _r32(x):
+    return float(np.float32(x))
+
+
+def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None, params32=False):
+    """params32: give an FP64 oracle the FP32-rounded parameters the HIP library
+    receives (relaxation state as initTau computes it in FP32, boundary
+    velocities, gravity, power-law constants), so that it integrates the same
+    discrete problem as the HIP FP64 path."""
     lbf = case["lbf"]
     coll = case["coll"]
-    lat = O.Lattice(lbf.NX, lbf.NY, lbf.nc, dtype=dtype, gravity=lbf.gravity_force)
+    g = tuple(_r32(v) for v in lbf.gravity_force) if params32 else lbf.gravity_force
+    lat = O.Lattice(lbf.NX, lbf.NY, lbf.nc, dtype=dtype, gravity=g)
     lat.mask[...] = lbf.mask.arr
     lat.neighbor[...] = lbf.neighbor.arr
     group = case["fluid"].group.nodes.astype(np.int64)
     fm = coll.fluid_model
     if type(fm).__name__ == "PowerLawFluid":
-        model = O.FluidModel("power", m=fm.m, n=fm.n)
+        if params32:
+            model = O.FluidModel("power", m=_r32(fm.m), n=1.0 + _r32(fm.n - 1.0))
+        else:
+            model = O.FluidModel("power", m=fm.m, n=fm.n)
     else:
         model = O.FluidModel("newtonian", nu=fm.nu)
     kind = {0: "bgk", 1: "trt", 2: "mrt"}[coll.KIND]
-    oc = O.Collision(kind, group, model, list(coll.nu), magic=list(getattr(coll, "magic", [])) or None,
-                     nu_bulk=list(getattr(coll, "nu_bulk", [])) or None, tau_min=coll.tau_min, tau_max=coll.tau_max)
+    magic = list(getattr(coll, "magic", [])) or None
+    tmin, tmax = coll.tau_min, coll.tau_max
+    if params32:
+        magic = [_r32(v) for v in magic] if magic else None
+        tmin, tmax = _r32(tmin), _r32(tmax)
+    args = dict(magic=magic, nu_bulk=list(getattr(coll, "nu_bulk", [])) or None, tau_min=tmin, tau_max=tmax)
+    oc = O.Collision(kind, group, model, list(coll.nu), **args)
     oc.init_tau(lat)
+    if params32 and np.dtype(dtype) != np.float32:
+        lat32 = O.Lattice(lbf.NX, lbf.NY, lbf.nc, dtype=np.float32)
+        oc32 = O.Collision(kind, group, model, list(coll.nu), **args)
+        oc32.init_tau(lat32)
+        for attr in ("tau", "tau_antisym", "diag"):
+            if hasattr(oc32, attr):
+                setattr(oc, attr, getattr(oc32, attr).astype(dtype))
     bcs = []
     for name, bc in case["bce"].boundary_conditions.items():
         k = bc.kind
         val = None
         if k in ("velocity_neem", "velocity_bb"):
-            val = bc.velocity_value
+            val = tuple(_r32(v) for v in bc.velocity_value) if params32 else bc.velocity_value
         elif k == "pressure_neem":
             val = bc.rho_value
         bcs.append(O.BC(k, bc.group.nodes.astype(np.int64), getattr(bc, "direction", None), val))

@@ -174,3 +174,44 @@ def test_rejections():
                                       BCSpec("velocity_neem", line2, 3, 0.05)], nb)
     with pytest.raises(UnsupportedBoundaryList, match="outside the lattice"):
         compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("velocity_neem", line, 1, 0.05)], nb)
+
+
+def test_reference_cylinder_codes():
+    """cases/Circular cylinder.py's list: VelocityBB inlet adds to the bounced
+    unknowns of column 0 (own rho kept), OpenNeumann outlet re-gathers column
+    NX-2 (which is plain streaming and final)."""
+    NX, NY = 60, 20
+    case = cases.cylinder_reference_bcs(NX=NX, NY=NY)
+    lbf, bce = case["lbf"], case["bce"]
+    code, rules = compile_codes(NX, NY, case["fluid"].group.nodes, bce.fused_list(), lbf.neighbor.arr)
+    assert [r[:2] for r in rules] == [(H.RULE_VELOCITY_BB, 3), (H.RULE_OPEN_NEUMANN, 1)]
+    c2 = code.reshape(NX, NY)
+    e = (c2 & H.CODE_NEEM_MASK) >> H.CODE_NEEM_SHIFT
+    assert set(map(tuple, np.argwhere(e == 1))) == {(0, j) for j in range(NY)}
+    assert set(map(tuple, np.argwhere(e == 2))) == {(NX - 1, j) for j in range(NY)}
+    assert np.all(c2[0] & H.CODE_KEEP_MACRO)                      # VelocityBB reads its own rho
+    assert not np.any(c2[NX - 2] & H.CODE_KEEP_MACRO)
+    for k in (1, 5, 8):                                            # unknowns of the inlet bounce first
+        assert np.all(c2[0] & (1 << (k - 1)))
+    for k in (3, 6, 7):
+        assert np.all(c2[NX - 1] & (1 << (k - 1)))
+
+
+def test_rule_rejections():
+    NX, NY = 8, 6
+    mask = np.ones((NX, NY), np.int32)
+    nb = neighbor_table(mask)
+    fluid = np.argwhere(mask == 1)
+    col = lambda i: np.array([[i, j] for j in range(NY)])
+    # OpenNeumann's neighbour rewritten by a later boundary
+    with pytest.raises(UnsupportedBoundaryList, match="rewrites"):
+        compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("open_neumann", col(NX - 1), 1),
+                                      BCSpec("bounceback", col(NX - 2))], nb)
+    # OpenNeumann whose neighbour carries a VelocityBB rule
+    with pytest.raises(UnsupportedBoundaryList, match="node rule"):
+        compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("velocity_bb", col(NX - 2), 1, 0.01),
+                                      BCSpec("open_neumann", col(NX - 1), 1)], nb)
+    # a later boundary overwriting a VelocityBB node
+    with pytest.raises(UnsupportedBoundaryList, match="node rule"):
+        compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("velocity_bb", col(0), 3, 0.01),
+                                      BCSpec("bounceback", col(0))], nb)
