@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _hip as H
-from ._device import DeviceField, device, planes_view, stream, torch
+from ._device import DeviceField, device, plane_stride, planes_view, stream, torch
 from .FluidModels import NewtonianFluid
 from .lbm import gfields, group_device, lattice_struct
 
@@ -62,7 +62,7 @@ class Collision:
     def _setup(self, params):
         self.NX, self.NY, self.nc = int(params['NX']), int(params['NY']), int(params['num_components'])
         self.N = self.NX * self.NY
-        self.ps = (self.N + 63) // 64 * 64
+        self.ps = plane_stride(self.N)
         self.nu = np.array([f32(v) for v in params['shearviscosity']], f32)
 
     # ---- device state ---------------------------------------------------

@@ -45,7 +45,7 @@ def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None, 
         oc32 = O.Collision(kind, group, model, list(coll.nu), **args)
         oc32.init_tau(lat32)
         for attr in ("tau", "tau_antisym", "diag"):
-            if hasattr(oc32, attr):
+            if getattr(oc32, attr, None) is not None:
                 setattr(oc, attr, getattr(oc32, attr).astype(dtype))
     bcs = []
     for name, bc in case["bce"].boundary_conditions.items():
