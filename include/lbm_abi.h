@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 3
+#define LBM_ABI_VERSION 4
 
 /* status codes */
 #define LBM_OK 0
@@ -122,7 +122,8 @@ typedef struct lbm_lattice {
 /* Device buffers of one lattice (see the plane layout above). */
 typedef struct lbm_fields {
   void* f[2];           /* population ping-pong, nc*9 planes each           */
-  float* macro[2];      /* macro ping-pong, nc+3 planes each                */
+  void* macro[2];       /* macro ping-pong, nc+3 planes each: double with  */
+                        /* LBM_STORE_F64 storage, float otherwise           */
   float* tau;           /* nc planes, per-node relaxation state             */
   float* psi[2];        /* nc planes each, Shan-Chen psi ping-pong          */
   float* body_force;    /* nc*2 planes                                      */

@@ -42,7 +42,7 @@ namespace lbm {
 // value; batched launches read it from a device array.
 struct Lat {
   void* pop[2];          // population ping-pong (nc*9 planes each)
-  float* mac[2];         // macro ping-pong: rho_0..rho_{nc-1}, total_rho, ux, uy
+  void* mac[2];          // macro ping-pong: rho_0..rho_{nc-1}, total_rho, ux, uy (element type R)
   void* tau;             // per-node relaxation state (R), nc planes
   void* psi[2];          // Shan-Chen psi (R), nc planes each
   float* body_force;
@@ -121,9 +121,9 @@ __device__ __forceinline__ typename Pop<S>::R pop_at(const typename Pop<S>::T* _
 // from the neighbour.  On `finish` the reference's vel / rho / total_rho
 // overrides are written into mac_override.
 template <int NC, int S>
-__device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const float* mi,
+__device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const typename Pop<S>::R* mi,
                            const lbm_neem_rule& r, uint32_t i, uint32_t j, uint32_t n,
-                           typename Pop<S>::R (&f)[NC][9], float* mac_override) {
+                           typename Pop<S>::R (&f)[NC][9], typename Pop<S>::R* mac_override) {
   using R = typename Pop<S>::R;
   const int d = r.direction;
   const uint32_t ps = a.ps;
@@ -167,7 +167,7 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const fl
         R s = R(0);
 #pragma unroll
         for (int k = 0; k < 9; ++k) s += f[c][k];
-        mac_override[(size_t)c * ps + n] = (float)s;
+        mac_override[(size_t)c * ps + n] = s;
         rtot += s;
         const R* g = f[c];
         R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
@@ -179,9 +179,9 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const fl
         vx += tx;
         vy += ty;
       }
-      mac_override[(size_t)NC * ps + n] = (float)rtot;
-      mac_override[(size_t)(NC + 1) * ps + n] = (float)(vx / rtot);
-      mac_override[(size_t)(NC + 2) * ps + n] = (float)(vy / rtot);
+      mac_override[(size_t)NC * ps + n] = rtot;
+      mac_override[(size_t)(NC + 1) * ps + n] = vx / rtot;
+      mac_override[(size_t)(NC + 2) * ps + n] = vy / rtot;
     }
     return;
   }
@@ -205,9 +205,9 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const fl
     }
     if (mac_override) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) mac_override[(size_t)c * ps + n] = (float)rw[c];
-      mac_override[(size_t)(NC + 1) * ps + n] = (float)uxw;
-      mac_override[(size_t)(NC + 2) * ps + n] = (float)uyw;
+      for (int c = 0; c < NC; ++c) mac_override[(size_t)c * ps + n] = rw[c];
+      mac_override[(size_t)(NC + 1) * ps + n] = uxw;
+      mac_override[(size_t)(NC + 2) * ps + n] = uyw;
     }
   } else {
 #pragma unroll
@@ -220,9 +220,9 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const fl
           f[c][k] = feq_k<R>(k, ux_nb, uy_nb, rho_own, uv_nb) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
     }
     if (mac_override) {
-      mac_override[(size_t)NC * ps + n] = r.rho;
-      mac_override[(size_t)(NC + 1) * ps + n] = (float)ux_nb;
-      mac_override[(size_t)(NC + 2) * ps + n] = (float)uy_nb;
+      mac_override[(size_t)NC * ps + n] = (R)r.rho;
+      mac_override[(size_t)(NC + 1) * ps + n] = ux_nb;
+      mac_override[(size_t)(NC + 2) * ps + n] = uy_nb;
     }
   }
 }
@@ -236,9 +236,9 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
   const uint32_t code = a.code[n];
   if (!(code & LBM_CODE_ACTIVE)) {
     if (MODE == MODE_STEP && q.write_macro) {
-      float* mo = a.mac[p ^ 1u];
+      R* mo = (R*)a.mac[p ^ 1u];
 #pragma unroll
-      for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = 0.0f;
+      for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = R(0);
     }
     return;
   }
@@ -259,7 +259,8 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
     gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
     const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
     if (e)
-      apply_neem<NC, S>(a, src, a.mac[p], a.neem[e - 1], i, j, n, f, MODE == MODE_FINISH ? a.mac[p] : nullptr);
+      apply_neem<NC, S>(a, src, (const R*)a.mac[p], a.neem[e - 1], i, j, n, f,
+                        MODE == MODE_FINISH ? (R*)a.mac[p] : nullptr);
   }
 
   if constexpr (MODE == MODE_FINISH) {
@@ -396,12 +397,12 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
   }
 
   if (q.write_macro || (code & LBM_CODE_KEEP_MACRO)) {
-    float* mo = a.mac[p ^ 1u];
+    R* mo = (R*)a.mac[p ^ 1u];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = (float)rho[c];
-    mo[(size_t)NC * ps + n] = (float)rtot;
-    mo[(size_t)(NC + 1) * ps + n] = (float)ux;
-    mo[(size_t)(NC + 2) * ps + n] = (float)uy;
+    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = rho[c];
+    mo[(size_t)NC * ps + n] = rtot;
+    mo[(size_t)(NC + 1) * ps + n] = ux;
+    mo[(size_t)(NC + 2) * ps + n] = uy;
   }
   if (q.write_macro) {
 #pragma unroll
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const Lat* __restrict__ items,
 
 // Velocity residual of every lattice of a batch: acc[2l] += sum |u - u_old|^2,
 // acc[2l+1] += sum |u|^2 over its active nodes, then u_old = u.
-template <int NC>
+template <int NC, typename M>
 __global__ __launch_bounds__(kBlock) void k_batch_residual(const Lat* __restrict__ items,
                                                            const uint32_t* __restrict__ tile_item,
                                                            const uint32_t* __restrict__ tile_count,
@@ -471,13 +472,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(const Lat* __restrict
     const uint32_t n = (w - a.block0) * kBlock + threadIdx.x;
     double d2 = 0.0, u2 = 0.0;
     if (n < a.N && (a.code[n] & LBM_CODE_ACTIVE) && a.vel_old) {
-      const float* m = a.mac[parity];
-      const float ux = m[(size_t)(NC + 1) * a.ps + n], uy = m[(size_t)(NC + 2) * a.ps + n];
-      const float ox = a.vel_old[n], oy = a.vel_old[(size_t)a.ps + n];
-      d2 = (double)(ux - ox) * (ux - ox) + (double)(uy - oy) * (uy - oy);
-      u2 = (double)ux * ux + (double)uy * uy;
-      a.vel_old[n] = ux;
-      a.vel_old[(size_t)a.ps + n] = uy;
+      const M* m = (const M*)a.mac[parity];
+      const double ux = (double)m[(size_t)(NC + 1) * a.ps + n], uy = (double)m[(size_t)(NC + 2) * a.ps + n];
+      const double ox = (double)a.vel_old[n], oy = (double)a.vel_old[(size_t)a.ps + n];
+      d2 = (ux - ox) * (ux - ox) + (uy - oy) * (uy - oy);
+      u2 = ux * ux + uy * uy;
+      a.vel_old[n] = (float)ux;
+      a.vel_old[(size_t)a.ps + n] = (float)uy;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -918,12 +919,12 @@ extern "C" int lbm_batch_residual(lbm_batch* b, int32_t parity, double* out, voi
   int rc = hip_check(hipGetLastError(), "zero launch");
   if (rc) return rc;
   const uint32_t grid = b->tiles < 4096u ? (b->tiles ? b->tiles : 1u) : 4096u;
-  if (b->nc == 1)
-    hipLaunchKernelGGL(k_batch_residual<1>, dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count,
-                       (uint32_t)parity, b->acc);
-  else
-    hipLaunchKernelGGL(k_batch_residual<2>, dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count,
-                       (uint32_t)parity, b->acc);
+  using ResFn = void (*)(const Lat*, const uint32_t*, const uint32_t*, uint32_t, double*);
+  const bool f64 = b->storage == LBM_STORE_F64;
+  const ResFn fn = b->nc == 1 ? (f64 ? k_batch_residual<1, double> : k_batch_residual<1, float>)
+                              : (f64 ? k_batch_residual<2, double> : k_batch_residual<2, float>);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count, (uint32_t)parity,
+                     b->acc);
   if ((rc = hip_check(hipGetLastError(), "residual launch"))) return rc;
   if ((rc = hip_check(hipMemcpyAsync(out, b->acc, sizeof(double) * n, hipMemcpyDeviceToHost, s), "copy out")))
     return rc;

@@ -25,7 +25,7 @@ CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class NeemRule(C.Structure):

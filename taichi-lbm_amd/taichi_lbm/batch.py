@@ -74,8 +74,8 @@ def footprint_bytes(spec_or_n, storage="f32") -> int:
     """Device bytes one lattice occupies in a batch (its arena segment)."""
     n = spec_or_n if isinstance(spec_or_n, int) else spec_or_n.N
     ps = plane_stride(n)
-    r = 8 if storage == "f64" else 4
-    return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * 4 + 2 * 4 + 2 + r)
+    r = 8 if storage == "f64" else 4          # relaxation state and macro planes: fp64 with fp64 storage
+    return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + r)
 
 
 class _Arena:
@@ -129,9 +129,10 @@ class BatchRunner:
         pdt = _torch_dtype(storage)
         rdt = t.float64 if storage == "f64" else t.float32
         # arenas indexed by node offset: lattice l at offset o uses populations
-        # [9*o, 9*(o+ps)), macro [4*o, 4*(o+ps)), code / tau [o, o+ps), u_old [2*o, ...)
+        # [9*o, 9*(o+ps)), macro [4*o, 4*(o+ps)), code / tau [o, o+ps), u_old [2*o, ...);
+        # macro planes and relaxation state are fp64 with fp64 storage
         self.popseg = t.zeros((2, 9 * cap), dtype=pdt, device=dev)
-        self.macseg = t.zeros((2, 4 * cap), dtype=t.float32, device=dev)
+        self.macseg = t.zeros((2, 4 * cap), dtype=rdt, device=dev)
         self.velseg = t.zeros(2 * cap, dtype=t.float32, device=dev)
         self.code = t.zeros(cap, dtype=t.int16, device=dev)
         self.tau = t.zeros(cap, dtype=rdt, device=dev)
@@ -169,8 +170,9 @@ class BatchRunner:
         pe = self.popseg.element_size()
         F.f[0] = self.popseg[0].data_ptr() + 9 * o * pe
         F.f[1] = self.popseg[1].data_ptr() + 9 * o * pe
-        F.macro[0] = self.macseg[0].data_ptr() + 4 * o * 4
-        F.macro[1] = self.macseg[1].data_ptr() + 4 * o * 4
+        me = self.macseg.element_size()
+        F.macro[0] = self.macseg[0].data_ptr() + 4 * o * me
+        F.macro[1] = self.macseg[1].data_ptr() + 4 * o * me
         F.tau = self.tau.data_ptr() + o * self.tau.element_size()
         F.code = self.code.data_ptr() + o * 2
 
