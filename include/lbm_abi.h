@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 4
+#define LBM_ABI_VERSION 5
 
 /* status codes */
 #define LBM_OK 0
@@ -124,7 +124,8 @@ typedef struct lbm_fields {
   void* f[2];           /* population ping-pong, nc*9 planes each           */
   void* macro[2];       /* macro ping-pong, nc+3 planes each: double with  */
                         /* LBM_STORE_F64 storage, float otherwise           */
-  float* tau;           /* nc planes, per-node relaxation state             */
+  void* tau;            /* nc planes, per-node relaxation state (double    */
+                        /* with LBM_STORE_F64 storage, float otherwise)     */
   float* psi[2];        /* nc planes each, Shan-Chen psi ping-pong          */
   float* body_force;    /* nc*2 planes                                      */
   float* sc_force;      /* nc*2 planes                                      */
@@ -154,6 +155,21 @@ const char* lbm_last_error(void);
  * before the boundary pass.                                                */
 int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
             int32_t nsteps, int32_t parity, int32_t first_local, void* stream);
+
+/* Same steps as lbm_run, two per launch where the lattice allows it
+ * (temporal blocking: step n is kept in LDS and step n+1 computed from it,
+ * halving the HBM traffic per step).  Lattices with node rules (n_neem > 0)
+ * or Shan-Chen, and FP64 storage, run one step per launch.
+ * A two-step launch flips the population / macro buffers once, so the index
+ * of the buffer holding the newest f2 -- what lbm_finish takes -- is returned
+ * in *out_parity.  With LBM_TAU_POWER_LAW the relaxation state is
+ * double-buffered: tau_next is a second state buffer of the same size as
+ * fields->tau (required, else one step per launch); the final state is left
+ * in fields->tau.  Bit-identical to lbm_run.  No counterpart in the
+ * reference: its steps are the same lbm_solve() sequence. */
+int lbm_run_blocked(const lbm_lattice* lattice, const lbm_fields* fields, const lbm_neem_rule* neem,
+                    int32_t nsteps, int32_t parity, int32_t first_local, void* tau_next, int32_t* out_parity,
+                    void* stream);
 
 /* The reference's boundary pass alone (BoundaryEngine.apply_boundary_conditions,
  * BoundaryCondition.py:481-483) applied to f2 = f[parity]: writes the

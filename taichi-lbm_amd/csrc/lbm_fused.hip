@@ -227,6 +227,166 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
   }
 }
 
+// Everything one step computes at a node once its post-boundary populations
+// f are known (S1 density .. S6 collide), kept in registers.
+template <int NC, typename R>
+struct StepOut {
+  R f[NC][9];                 // post-collision populations
+  R rho[NC], rtot, ux, uy;    // macroscopic fields of the step
+  R Fx[NC], Fy[NC];           // body force (force_density)
+  R scx[NC], scy[NC];         // Shan-Chen force
+  R gam[NC];                  // shear rate (power law)
+  R state[NC];                // new relaxation state (power law)
+};
+
+// S1..S6 of one step at node n = (i, j).  state_in is the per-node relaxation
+// state (BGK tau, TRT tau_sym, MRT 1/tau) for LBM_TAU_FIELD / _POWER_LAW.
+// Shan-Chen reads psi of this step at x + c_k from psi plane set `psi_set`.
+template <int COLL, int NC, typename R>
+__device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32_t n, uint32_t i, uint32_t j,
+                                          const R (&f)[NC][9], const R (&state_in)[NC], StepOut<NC, R>& o) {
+  const uint32_t ps = a.ps;
+  // S1 density (lbm.py:153-164)
+  o.rtot = R(0);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    R s = R(0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s += f[c][k];
+    o.rho[c] = s;
+    o.rtot += s;
+  }
+
+  // Shan-Chen force (ExternalForce.py:78-91 / 124-145); psi read at x + c_k.
+#pragma unroll
+  for (int c = 0; c < NC; ++c) o.scx[c] = o.scy[c] = R(0);
+  if (a.sc != LBM_SC_NONE) {
+    const Nbr nb(i, j, a.nx, a.ny);
+    const R* psi = (const R*)a.psi[psi_set];
+    if (a.sc == LBM_SC_C1) {
+      R Fx = R(0), Fy = R(0);
+#pragma unroll
+      for (int k = 1; k < 9; ++k) {
+        const R v = psi[nb.src(opp(k))];  // x - c_opp(k) = x + c_k
+        Fx += (wgt<R>(k) * (R)cx(k)) * v;
+        Fy += (wgt<R>(k) * (R)cy(k)) * v;
+      }
+      const R s = -(R)a.g_coh[0] * psi[n];
+      o.scx[0] = Fx * s;
+      o.scy[0] = Fy * s;
+    } else {
+#pragma unroll
+      for (int c1 = 0; c1 < NC; ++c1) {
+        R Fx = R(0), Fy = R(0);
+#pragma unroll
+        for (int k = 1; k < 9; ++k) {
+          const uint32_t xk = nb.src(opp(k));
+#pragma unroll
+          for (int c2 = 0; c2 < NC; ++c2) {
+            if (c2 == c1) continue;
+            const R v = psi[(size_t)c2 * ps + xk];
+            const R G = a.g_coh[c1 * LBM_MAX_NC + c2];
+            Fx += (wgt<R>(k) * (R)cx(k)) * v * G;
+            Fy += (wgt<R>(k) * (R)cy(k)) * v * G;
+          }
+        }
+        const R s = -psi[(size_t)c1 * ps + n];
+        o.scx[c1] = s * Fx;
+        o.scy[c1] = s * Fy;
+      }
+    }
+  }
+
+  // S3 force_density (lbm.py:215-220)
+  const bool force = a.force || a.sc != LBM_SC_NONE;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const R frac = o.rho[c] / o.rtot;
+    o.Fx[c] = o.scx[c] + frac * (R)a.gx;
+    o.Fy[c] = o.scy[c] + frac * (R)a.gy;
+  }
+
+  // S4 velocity (lbm.py:223-237)
+  R vx = R(0), vy = R(0);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const R* g = f[c];
+    R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
+    R ty = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
+    if (force) {
+      tx = tx + R(0.5) * o.Fx[c];
+      ty = ty + R(0.5) * o.Fy[c];
+    }
+    vx += tx;
+    vy += ty;
+  }
+  o.ux = vx / o.rtot;
+  o.uy = vy / o.rtot;
+
+  // S5 power law + S6 collide, per component
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    R tau = a.tau0[c], tau_anti = a.tau_anti0[c], s_shear = a.s_shear[c];
+    const R s_bulk = a.s_bulk[c];
+    o.gam[c] = R(0);
+    o.state[c] = state_in[c];
+    if (a.tau_mode == LBM_TAU_FIELD) {
+      const R state = state_in[c];
+      if constexpr (COLL == LBM_MRT) {
+        s_shear = state;
+      } else {
+        tau = state;
+        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
+      }
+    } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
+      const R state = state_in[c];
+      const R g = shear_rate_of<COLL, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
+      o.gam[c] = g;
+      const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
+      const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
+      if constexpr (COLL == LBM_MRT) {
+        s_shear = R(1) / t;
+        o.state[c] = s_shear;
+      } else {
+        tau = t;
+        o.state[c] = t;
+        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
+      }
+    }
+    collide<COLL, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+  }
+}
+
+// Writes a step's macroscopic fields (macro plane set `mo`, and on the last
+// step of a run the body force, Shan-Chen force and shear rate).
+template <int NC, typename R>
+__device__ __forceinline__ void store_macro(const Lat& a, R* mo, uint32_t n, const StepOut<NC, R>& o,
+                                            bool keep, bool last) {
+  const uint32_t ps = a.ps;
+  if (last || keep) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = o.rho[c];
+    mo[(size_t)NC * ps + n] = o.rtot;
+    mo[(size_t)(NC + 1) * ps + n] = o.ux;
+    mo[(size_t)(NC + 2) * ps + n] = o.uy;
+  }
+  if (last) {
+    const bool force = a.force || a.sc != LBM_SC_NONE;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (force && a.body_force) {
+        a.body_force[(size_t)(2 * c) * ps + n] = (float)o.Fx[c];
+        a.body_force[(size_t)(2 * c + 1) * ps + n] = (float)o.Fy[c];
+      }
+      if (a.sc != LBM_SC_NONE) {
+        a.sc_force[(size_t)(2 * c) * ps + n] = (float)o.scx[c];
+        a.sc_force[(size_t)(2 * c + 1) * ps + n] = (float)o.scy[c];
+      }
+      if (a.tau_mode == LBM_TAU_POWER_LAW && a.shear_rate) a.shear_rate[(size_t)c * ps + n] = (float)o.gam[c];
+    }
+  }
+}
+
 template <int COLL, int NC, int S, int MODE>
 __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint32_t n, bool local) {
   using T = typename Pop<S>::T;
@@ -272,152 +432,31 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
     return;
   }
 
-  // S1 density (lbm.py:153-164)
-  R rho[NC];
-  R rtot = R(0);
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    R s = R(0);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) s += f[c][k];
-    rho[c] = s;
-    rtot += s;
-  }
-
   if constexpr (MODE == MODE_PSI) {
     R* psi_out = (R*)a.psi[p ^ 1u];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) psi_out[(size_t)c * ps + n] = sc_psi<R>(rho[c], (R)a.psi_rho0[c]);
+    for (int c = 0; c < NC; ++c) {
+      R s = R(0);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s += f[c][k];
+      psi_out[(size_t)c * ps + n] = sc_psi<R>(s, (R)a.psi_rho0[c]);
+    }
     return;
   }
 
-  // Shan-Chen force (ExternalForce.py:78-91 / 124-145); psi read at x + c_k.
-  R scx[NC], scy[NC];
+  R st_in[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) scx[c] = scy[c] = R(0);
-  if (a.sc != LBM_SC_NONE) {
-    const Nbr nb(i, j, a.nx, a.ny);
-    const R* psi = (const R*)a.psi[p ^ 1u];
-    if (a.sc == LBM_SC_C1) {
-      R Fx = R(0), Fy = R(0);
-#pragma unroll
-      for (int k = 1; k < 9; ++k) {
-        const R v = psi[nb.src(opp(k))];  // x - c_opp(k) = x + c_k
-        Fx += (wgt<R>(k) * (R)cx(k)) * v;
-        Fy += (wgt<R>(k) * (R)cy(k)) * v;
-      }
-      const R s = -(R)a.g_coh[0] * psi[n];
-      scx[0] = Fx * s;
-      scy[0] = Fy * s;
-    } else {
-#pragma unroll
-      for (int c1 = 0; c1 < NC; ++c1) {
-        R Fx = R(0), Fy = R(0);
-#pragma unroll
-        for (int k = 1; k < 9; ++k) {
-          const uint32_t xk = nb.src(opp(k));
-#pragma unroll
-          for (int c2 = 0; c2 < NC; ++c2) {
-            if (c2 == c1) continue;
-            const R v = psi[(size_t)c2 * ps + xk];
-            const R G = a.g_coh[c1 * LBM_MAX_NC + c2];
-            Fx += (wgt<R>(k) * (R)cx(k)) * v * G;
-            Fy += (wgt<R>(k) * (R)cy(k)) * v * G;
-          }
-        }
-        const R s = -psi[(size_t)c1 * ps + n];
-        scx[c1] = s * Fx;
-        scy[c1] = s * Fy;
-      }
-    }
-  }
-
-  // S3 force_density (lbm.py:215-220)
-  const bool force = a.force || a.sc != LBM_SC_NONE;
-  R Fx[NC], Fy[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const R frac = rho[c] / rtot;
-    Fx[c] = scx[c] + frac * (R)a.gx;
-    Fy[c] = scy[c] + frac * (R)a.gy;
-  }
-
-  // S4 velocity (lbm.py:223-237)
-  R vx = R(0), vy = R(0);
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const R* g = f[c];
-    R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
-    R ty = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
-    if (force) {
-      tx = tx + R(0.5) * Fx[c];
-      ty = ty + R(0.5) * Fy[c];
-    }
-    vx += tx;
-    vy += ty;
-  }
-  const R ux = vx / rtot, uy = vy / rtot;
-
-  // S5 power law + S6 collide, per component
+  for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
+  StepOut<NC, R> o;
+  step_math<COLL, NC, R>(a, p ^ 1u, n, i, j, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[p ^ 1u];
-  R gam[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    R tau = a.tau0[c], tau_anti = a.tau_anti0[c], s_shear = a.s_shear[c];
-    const R s_bulk = a.s_bulk[c];
-    gam[c] = R(0);
-    if (a.tau_mode == LBM_TAU_FIELD) {
-      const R state = ((const R*)a.tau)[(size_t)c * ps + n];
-      if constexpr (COLL == LBM_MRT) {
-        s_shear = state;
-      } else {
-        tau = state;
-        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
-      }
-    } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
-      R* tp = (R*)a.tau + (size_t)c * ps + n;
-      const R state = *tp;
-      const R g = shear_rate_of<COLL, R>(f[c], rho[c], ux, uy, state, s_bulk);
-      gam[c] = g;
-      const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
-      const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
-      if constexpr (COLL == LBM_MRT) {
-        s_shear = R(1) / t;
-        *tp = s_shear;
-      } else {
-        tau = t;
-        *tp = t;
-        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
-      }
-    }
-    R out[9];
-    collide<COLL, R>(f[c], out, rho[c], ux, uy, Fx[c], Fy[c], force, tau, tau_anti, s_shear, s_bulk);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, out[k]);
+    for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+    if (a.tau_mode == LBM_TAU_POWER_LAW) ((R*)a.tau)[(size_t)c * ps + n] = o.state[c];
   }
-
-  if (q.write_macro || (code & LBM_CODE_KEEP_MACRO)) {
-    R* mo = (R*)a.mac[p ^ 1u];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) mo[(size_t)c * ps + n] = rho[c];
-    mo[(size_t)NC * ps + n] = rtot;
-    mo[(size_t)(NC + 1) * ps + n] = ux;
-    mo[(size_t)(NC + 2) * ps + n] = uy;
-  }
-  if (q.write_macro) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      if (force && a.body_force) {
-        a.body_force[(size_t)(2 * c) * ps + n] = (float)Fx[c];
-        a.body_force[(size_t)(2 * c + 1) * ps + n] = (float)Fy[c];
-      }
-      if (a.sc != LBM_SC_NONE) {
-        a.sc_force[(size_t)(2 * c) * ps + n] = (float)scx[c];
-        a.sc_force[(size_t)(2 * c + 1) * ps + n] = (float)scy[c];
-      }
-      if (a.tau_mode == LBM_TAU_POWER_LAW && a.shear_rate) a.shear_rate[(size_t)c * ps + n] = (float)gam[c];
-    }
-  }
+  store_macro<NC, R>(a, (R*)a.mac[p ^ 1u], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
 template <int COLL, int NC, int S, int MODE>
@@ -425,6 +464,140 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
   const uint32_t n = logical_block(q) * kBlock + threadIdx.x;
   if (n >= a.N) return;
   node_update<COLL, NC, S, MODE>(a, q, n, q.local_read != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Two steps per launch (temporal blocking through LDS).
+//
+// A block owns a tile of kTI rows x kTJ columns.  Phase A runs step n on the
+// tile and a one-node ring around it (pull gathers from HBM, exactly as the
+// one-step kernel) and keeps the post-collision populations and relaxation
+// state in LDS; phase B runs step n+1 on the tile, gathering from LDS, and
+// writes only its result to HBM.  Per pair of steps HBM sees the ring's
+// re-reads (~1.2x of one read) plus one write, instead of two reads and two
+// writes.  The per-node arithmetic is step_math, shared with the one-step
+// kernel, so both give bit-identical fields.
+//
+// Used for boundary lists without node rules (NEEM / VelocityBB /
+// OpenNeumann read a neighbour's post-stream populations, two nodes away)
+// and without Shan-Chen (psi needs its own pass); the host falls back to
+// one-step launches otherwise.  The result of a launch goes to pop[parity^1]
+// like a one-step launch, so a two-step launch flips the buffers once.
+// ---------------------------------------------------------------------------
+constexpr int kTJ = 64;       // tile columns: one wavefront per tile row
+constexpr int kTI = 16;       // tile rows
+constexpr int kBlock2 = 1024;
+
+struct Launch2 {
+  uint32_t parity;            // pop[parity]: f2 of step n-1 (post-stream f if local_read)
+  int32_t local_read;
+  int32_t write_macro;        // step n+1 ends the run
+  int32_t xcd_remap;
+  uint32_t tiles_i, tiles_j;
+  const void* tau_in;         // relaxation state before step n (power law)
+  void* tau_out;              // relaxation state after step n+1 (power law)
+};
+
+template <int COLL, int NC, int S>
+__global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
+  using T = typename Pop<S>::T;
+  using R = typename Pop<S>::R;
+  constexpr int RJ = kTJ + 2, RN = (kTI + 2) * RJ;
+  __shared__ R sf[NC * 9][RN];
+  __shared__ R sst[NC][RN];
+  __shared__ uint16_t scode[RN];
+
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  const uint32_t i0 = ti * kTI, j0 = tj * kTJ;
+  const uint32_t ps = a.ps, p = q.parity, nx = a.nx, ny = a.ny;
+  const bool pl = a.tau_mode == LBM_TAU_POWER_LAW;
+  const R* tau_a = (const R*)(pl ? q.tau_in : a.tau);
+
+  // phase A: step n on the tile and its ring (wrapped periodically)
+  const T* __restrict__ src = (const T*)a.pop[p];
+  for (uint32_t r1 = threadIdx.x; r1 < (uint32_t)RN; r1 += kBlock2) {
+    const uint32_t rr = r1 / RJ, cc = r1 - rr * RJ;
+    const uint32_t gi = (i0 + rr + nx - 1) % nx, gj = (j0 + cc + ny - 1) % ny;
+    const uint32_t n = gi * ny + gj;
+    const uint32_t code = a.code[n];
+    scode[r1] = (uint16_t)code;
+    if (!(code & LBM_CODE_ACTIVE)) continue;
+    R f[NC][9];
+    if (q.local_read) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
+    } else {
+      const Nbr nb(gi, gj, nx, ny);
+      gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
+    }
+    R st_in[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? tau_a[(size_t)c * ps + n] : R(0);
+    StepOut<NC, R> o;
+    step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) sf[c * 9 + k][r1] = o.f[c][k];
+      sst[c][r1] = o.state[c];
+    }
+  }
+  __syncthreads();
+
+  // phase B: step n+1 on the tile from LDS
+  T* __restrict__ dst = (T*)a.pop[p ^ 1u];
+  R* mo = (R*)a.mac[p ^ 1u];
+  for (uint32_t t = threadIdx.x; t < (uint32_t)(kTI * kTJ); t += kBlock2) {
+    const uint32_t rr = t / kTJ, cc = t - rr * kTJ;
+    const uint32_t gi = i0 + rr, gj = j0 + cc;
+    if (gi >= nx || gj >= ny) continue;
+    const uint32_t n = gi * ny + gj;
+    const uint32_t r1 = (rr + 1) * RJ + cc + 1;
+    const uint32_t code = scode[r1];
+    if (!(code & LBM_CODE_ACTIVE)) {
+      if (q.write_macro) {
+#pragma unroll
+        for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = R(0);
+      }
+      continue;
+    }
+    R f[NC][9];
+    const uint32_t bm = code & LBM_CODE_BOUNCE_MASK;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const bool bounce = (k > 0) && ((bm >> (k - 1)) & 1u);
+      const uint32_t s1 = bounce ? r1 : (uint32_t)((int)r1 - cx(k) * RJ - cy(k));
+      const int kk = bounce ? opp(k) : k;
+      if (bounce || (scode[s1] & LBM_CODE_ACTIVE)) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c][k] = sf[c * 9 + kk][s1];
+      } else {
+        // pull from a node outside the fluid group: its value in the buffer
+        // step n would have written (never written there: the stored one)
+        const Nbr nb(gi, gj, nx, ny);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) f[c][k] = Pop<S>::ld(dst + (size_t)(c * 9 + k) * ps, nb.src(k), k);
+      }
+    }
+    R st_in[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      st_in[c] = pl ? sst[c][r1] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
+    StepOut<NC, R> o;
+    step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+      if (pl) ((R*)q.tau_out)[(size_t)c * ps + n] = o.state[c];
+    }
+    store_macro<NC, R>(a, mo, n, o, false, q.write_macro != 0);
+  }
 }
 
 // Persistent batched step over the 256-node tiles of many lattices.  Each XCD
@@ -710,6 +883,98 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
     if ((rc = launch(step, a, q, s))) return rc;
     p ^= 1u;
   }
+  return LBM_OK;
+}
+
+namespace lbm {
+using Kernel2Fn = void (*)(const Lat, const Launch2);
+
+template <int NC, int S>
+Kernel2Fn pick2_coll(int coll) {
+  switch (coll) {
+    case LBM_BGK: return k_fused2<LBM_BGK, NC, S>;
+    case LBM_TRT: return k_fused2<LBM_TRT, NC, S>;
+    default: return k_fused2<LBM_MRT, NC, S>;
+  }
+}
+// nullptr where the two-step kernel is not built: FP64 storage (the
+// high-precision reference path; in a 1024-thread block its registers would
+// spill) runs one step per launch.
+Kernel2Fn pick2(int nc, int store, int coll) {
+  switch (store) {
+    case LBM_STORE_F32: return nc == 1 ? pick2_coll<1, LBM_STORE_F32>(coll) : pick2_coll<2, LBM_STORE_F32>(coll);
+    case LBM_STORE_F16_SHIFT:
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT>(coll) : pick2_coll<2, LBM_STORE_F16_SHIFT>(coll);
+    case LBM_STORE_BF16_SHIFT:
+      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT>(coll) : pick2_coll<2, LBM_STORE_BF16_SHIFT>(coll);
+    default: return nullptr;
+  }
+}
+}  // namespace lbm
+
+extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
+                               int32_t nsteps, int32_t parity, int32_t first_local, void* tau_next,
+                               int32_t* out_parity, void* stream) {
+  using namespace lbm;
+  int rc = validate(lat, fld, neem);
+  if (rc) return rc;
+  if (nsteps < 0) return fail("nsteps %d < 0", nsteps);
+  if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
+  if (!out_parity) return fail("null out_parity");
+  hipStream_t s = (hipStream_t)stream;
+  const bool pl = lat->tau_mode == LBM_TAU_POWER_LAW;
+  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision);
+  const bool ok2 = two && lat->n_neem == 0 && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
+  if (!ok2) {
+    rc = lbm_run(lat, fld, neem, nsteps, parity, first_local, stream);
+    if (!rc) *out_parity = parity ^ (nsteps & 1);
+    return rc;
+  }
+  Lat a = make_lat(lat, fld, neem);
+  const size_t rbytes = lat->storage == LBM_STORE_F64 ? 8 : 4;
+  const void* tcur = fld->tau;
+  void* tnext = tau_next;
+  Launch2 q2;
+  memset(&q2, 0, sizeof q2);
+  q2.tiles_i = (a.nx + kTI - 1) / kTI;
+  q2.tiles_j = (a.ny + kTJ - 1) / kTJ;
+  const uint64_t nt = (uint64_t)q2.tiles_i * q2.tiles_j;
+  if (nt >= (1ull << 31)) return fail("lattice too large for the two-step tiling");
+  q2.xcd_remap = (nt % 8 == 0 && nt >= 64) ? 1 : 0;
+  uint32_t p = (uint32_t)parity;
+  const int pairs = nsteps / 2;
+  for (int t = 0; t < pairs; ++t) {
+    q2.parity = p;
+    q2.local_read = (t == 0 && first_local) ? 1 : 0;
+    q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
+    q2.tau_in = tcur;
+    q2.tau_out = tnext;
+    hipLaunchKernelGGL(two, dim3((unsigned)nt), dim3(kBlock2), 0, s, a, q2);
+    if ((rc = hip_check(hipGetLastError(), "two-step launch"))) return rc;
+    if (pl) {
+      void* w = tnext;
+      tnext = const_cast<void*>(tcur);
+      tcur = w;
+    }
+    p ^= 1u;
+  }
+  if (nsteps & 1) {
+    // the odd step: a one-step launch on the current relaxation-state buffer
+    a.tau = const_cast<void*>(tcur);
+    Launch q = make_launch(a.N);
+    q.parity = p;
+    q.step = 0;
+    q.local_read = (pairs == 0 && first_local) ? 1 : 0;
+    q.write_macro = 1;
+    if ((rc = launch(pick(lat->nc, lat->storage, lat->collision, MODE_STEP), a, q, s))) return rc;
+    p ^= 1u;
+  }
+  if (pl && tcur != fld->tau) {
+    const size_t bytes = (size_t)lat->nc * (size_t)lat->plane_stride * rbytes;
+    if ((rc = hip_check(hipMemcpyAsync(fld->tau, tcur, bytes, hipMemcpyDeviceToDevice, s), "copy tau state")))
+      return rc;
+  }
+  *out_parity = (int32_t)p;
   return LBM_OK;
 }
 

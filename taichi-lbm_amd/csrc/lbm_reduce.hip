@@ -83,9 +83,12 @@ __global__ void k_zero(double* p, int n) {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// One 16-byte element per thread, streamed (non-temporal) in and out: the
+// fastest plain copy measured on MI355X (a grid-stride loop over a capped
+// grid reaches about 80 % of its bandwidth).
 __global__ __launch_bounds__(256) void k_copy(const v4f* __restrict__ src, v4f* __restrict__ dst, size_t n4) {
-  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n4; q += (size_t)gridDim.x * 256)
-    __builtin_nontemporal_store(src[q], dst + q);
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(src + q), dst + q);
 }
 
 __global__ void k_copy_tail(const float* __restrict__ src, float* __restrict__ dst, size_t start, size_t n) {
@@ -170,8 +173,8 @@ extern "C" int lbm_copy_f32(const float* src, float* dst, int64_t n, void* strea
   hipStream_t s = (hipStream_t)stream;
   const size_t n4 = (size_t)n / 4;
   if (n4) {
-    size_t blocks = (n4 + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
+    const size_t blocks = (n4 + 255) / 256;
+    if (blocks >= (1ull << 31)) return fail("lbm_copy_f32: copy too large for one launch");
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const v4f*)src, (v4f*)dst, n4);
     int rc = hip_check(hipGetLastError(), "copy launch");
     if (rc) return rc;

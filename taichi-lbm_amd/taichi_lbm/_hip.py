@@ -25,7 +25,7 @@ CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 
 class NeemRule(C.Structure):
@@ -93,6 +93,7 @@ def lib():
         "lbm_abi_version": ([], C.c_int),
         "lbm_last_error": ([], C.c_char_p),
         "lbm_run": ([LP, FP, NP, I32, I32, I32, P], C.c_int),
+        "lbm_run_blocked": ([LP, FP, NP, I32, I32, I32, P, C.POINTER(C.c_int32), P], C.c_int),
         "lbm_finish": ([LP, FP, NP, I32, P], C.c_int),
         "lbm_reduce_macro": ([LP, P, P, D, P], C.c_int),
         "lbm_residual": ([I64, I64, P, P, D, P], C.c_int),
@@ -133,8 +134,8 @@ def lib():
 
 def exported_symbols():
     """Names of every entry point the header declares (checked by the tests)."""
-    return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_finish", "lbm_reduce_macro", "lbm_residual",
-            "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
+    return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_run_blocked", "lbm_finish", "lbm_reduce_macro",
+            "lbm_residual", "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",

@@ -40,8 +40,12 @@ def _same_nodes(g1, g2, NY) -> bool:
 
 class Stepper:
     def __init__(self, lb_field, macroscopic_engine, collision_engine, boundary_engine, power_law=False,
-                 boundary_engine_sc=None, advance_time=False, storage="f32"):
+                 boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True):
+        """blocked: run two steps per launch where the lattice allows it
+        (lbm_run_blocked; bit-identical to one step per launch)."""
         self.lbf = lb_field
+        self.blocked = bool(blocked)
+        self._tau_next = None
         self.macro = macroscopic_engine
         self.coll = collision_engine
         self.bce = boundary_engine
@@ -114,9 +118,21 @@ class Stepper:
             parity, first_local = 1 - st["f2_idx"], 1
         else:
             parity, first_local = st["f2_idx"], 0
-        H.check(H.lib().lbm_run(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
-                                stream()), "lbm_run")
-        st["f2_idx"] = (parity + nsteps) % 2
+        if self.blocked:
+            tau_next = None
+            if tau_mode == H.TAU_POWER_LAW:
+                state = self.coll.state_tensor()
+                if self._tau_next is None or self._tau_next.shape != state.shape:
+                    self._tau_next = torch().empty_like(state)
+                tau_next = H.ptr(self._tau_next)
+            out = ctypes.c_int32(-1)
+            H.check(H.lib().lbm_run_blocked(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
+                                            tau_next, ctypes.byref(out), stream()), "lbm_run_blocked")
+            st["f2_idx"] = int(out.value)
+        else:
+            H.check(H.lib().lbm_run(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
+                                    stream()), "lbm_run")
+            st["f2_idx"] = (parity + nsteps) % 2
         st["f_valid"] = False
         if tau_mode == H.TAU_POWER_LAW:
             self.coll._uniform = False
