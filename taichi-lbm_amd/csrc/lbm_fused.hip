@@ -485,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // like a one-step launch, so a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
 constexpr int kTJ = 64;       // tile columns: one wavefront per tile row
-constexpr int kTI = 16;       // tile rows
+constexpr int kTI = 13;       // tile rows: the ring-extended tile (15 x 66) fits one 1024-thread round
 constexpr int kBlock2 = 1024;
 
 struct Launch2 {

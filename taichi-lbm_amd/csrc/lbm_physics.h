@@ -147,30 +147,46 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
 // Power-law shear rate of one component (CollisionEngine.py:80-92, 166-178,
 // 332-377).  `tau_state` is the stored relaxation state of the previous step:
 // BGK tau, TRT tau_sym, MRT diag[7] (= 1/tau).
+//
+// The reference sums the non-equilibrium part over the populations:
+//   Sxx = cof sum_k cx^2 q_k,  Syy = cof sum_k cy^2 q_k,  Sxy = cof sum_k cx cy q_k
+// with q = f - feq (BGK/TRT, cof = -1.5/(rho tau)) or q = M^-1 S M (f - feq)
+// (MRT, cof = -1.5/rho).  These are evaluated here in moment form:
+//   sum_k cx^2 feq = rho (1/3 + ux^2),  sum_k cy^2 feq = rho (1/3 + uy^2),
+//   sum_k cx cy feq = rho ux uy;
+// and, for MRT, sum_k {cx^2, cy^2, cx cy} (M^-1 v)_k = {v0 2/3 + v1/6 + v7/2,
+// v0 2/3 + v1/6 - v7/2, v8} (the column sums of M^-1 = M^T diag(1/9, 1/36,
+// 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4)), where v = S M (f - feq) has
+// v0 = 0, v1 = s_bulk (e - e_eq), v7 = s (p_xx - p_xx,eq), v8 = s (p_xy - p_xy,eq).
+// The same quantities with about a tenth of the arithmetic of the matrix form.
 template <int COLL, typename R>
 __device__ __forceinline__ R shear_rate_of(const R (&f)[9], R rho, R ux, R uy, R tau_state, R s_bulk) {
-  const R uv = ux * ux + uy * uy;
-  R q[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) q[k] = f[k] - feq_k(k, ux, uy, rho, uv);
+  const R s14 = f[1] + f[2] + f[3] + f[4];
+  const R s58 = f[5] + f[6] + f[7] + f[8];
+  const R pxy = f[5] - f[6] + f[7] - f[8];
+  R Sxx, Syy, Sxy;
   if constexpr (COLL == LBM_MRT) {
-    R m[9], sm[9];
-    mrt_m(q, m);
-    sm[0] = R(0.0) * m[0];
-    sm[1] = s_bulk * m[1];
-    sm[2] = m[2];
-    sm[3] = R(0.0) * m[3];
-    sm[4] = m[4];
-    sm[5] = R(0.0) * m[5];
-    sm[6] = m[6];
-    sm[7] = tau_state * m[7];
-    sm[8] = tau_state * m[8];
-    R qq[9];
-    mrt_minv(sm, qq);
-    return shear_from(qq, -R(1.5) / rho);
+    const R uv = ux * ux + uy * uy;
+    const R e = R(-4) * f[0] - s14 + R(2) * s58;
+    const R pxx = f[1] - f[2] + f[3] - f[4];
+    const R m1 = e - rho * (-R(2) + R(3) * uv);
+    const R m7 = pxx - rho * (ux * ux - uy * uy);
+    const R m8 = pxy - rho * (ux * uy);
+    const R cof = -R(1.5) / rho;
+    const R a = s_bulk * m1 * (R(1) / R(6));
+    const R b = tau_state * m7 * R(0.5);
+    Sxx = (a + b) * cof;
+    Syy = (a - b) * cof;
+    Sxy = tau_state * m8 * cof;
   } else {
-    return shear_from(q, -R(1.5) / rho / tau_state);
+    const R cof = -R(1.5) / rho / tau_state;
+    const R pxx = f[1] + f[3] + s58;     // sum_k cx^2 f_k
+    const R pyy = f[2] + f[4] + s58;     // sum_k cy^2 f_k
+    Sxx = (pxx - rho * (R(1) / R(3) + ux * ux)) * cof;
+    Syy = (pyy - rho * (R(1) / R(3) + uy * uy)) * cof;
+    Sxy = (pxy - rho * (ux * uy)) * cof;
   }
+  return sqrt(R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy));
 }
 
 // SC_psi.get_psi, EOS.py:40-47.
