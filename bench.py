@@ -353,11 +353,13 @@ def main():
     except Exception as e:   # the bandwidth probe is informative only
         log("copy ceiling probe failed:", e)
 
-    traffic = None
+    traffic, steps_per_launch = None, 2
     pmc = ROOT / "profiles" / "pmc_traffic.json"
     if pmc.exists():
         try:
-            traffic = json.loads(pmc.read_text()).get("mrt_step_bytes_per_launch")
+            rec = json.loads(pmc.read_text())
+            traffic = rec.get("mrt_step_bytes_per_launch")
+            steps_per_launch = rec.get("steps_per_launch", steps_per_launch)
         except Exception:
             traffic = None
 
@@ -377,7 +379,8 @@ def main():
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_fluid_node": BYTES_POWER_LAW},
+                     "bytes_per_fluid_node": BYTES_POWER_LAW, "steps_per_launch": steps_per_launch,
+                     "kernel": "k_fused2 (two steps per launch, LDS-staged)"},
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

@@ -104,25 +104,45 @@ __device__ __forceinline__ void gather(const typename Pop<S>::T* __restrict__ sr
   }
 }
 
-// Post-stream population k of component c at node nb, following nb's own
-// code (pull, or halfway bounce-back where its bit is set).
+// Where a node rule reads the previous step's state: post-collision
+// populations f2 and macroscopic planes in HBM.  (The two-step kernel has an
+// LDS counterpart with the same interface.)
 template <int S>
-__device__ __forceinline__ typename Pop<S>::R pop_at(const typename Pop<S>::T* __restrict__ src, uint32_t ps,
-                                                     const Nbr& nb, uint32_t bmask, int c, int k) {
-  const bool bounce = (k > 0) && ((bmask >> (k - 1)) & 1u);
-  const int kk = bounce ? opp(k) : k;
-  return Pop<S>::ld(src + (size_t)(c * 9 + kk) * ps, bounce ? nb.n : nb.src(k), kk);
-}
+struct GlobalView {
+  using T = typename Pop<S>::T;
+  using R = typename Pop<S>::R;
+  struct Node {
+    uint32_t i, j;
+    Nbr nb;
+  };
+  const T* src;
+  const R* mac;
+  const uint16_t* code;
+  uint32_t ps, nx, ny;
+  __device__ __forceinline__ Node node(uint32_t i, uint32_t j) const { return Node{i, j, Nbr(i, j, nx, ny)}; }
+  // x - (di, dj): rule neighbours lie inside the lattice (checked on the host)
+  __device__ __forceinline__ Node back(const Node& x, int di, int dj) const {
+    return node((uint32_t)((int)x.i - di), (uint32_t)((int)x.j - dj));
+  }
+  __device__ __forceinline__ uint32_t code_of(const Node& x) const { return code[x.nb.n]; }
+  __device__ __forceinline__ R macro(const Node& x, int plane) const { return mac[(size_t)plane * ps + x.nb.n]; }
+  // post-stream population k of component c at x: pull f2(x - c_k, k), or
+  // f2(x, opp k) where x's code has the bounce bit of k
+  __device__ __forceinline__ R pop(const Node& x, uint32_t bm, int c, int k) const {
+    const bool bounce = (k > 0) && ((bm >> (k - 1)) & 1u);
+    const int kk = bounce ? opp(k) : k;
+    return Pop<S>::ld(src + (size_t)(c * 9 + kk) * ps, bounce ? x.nb.n : x.nb.src(k), kk);
+  }
+};
 
-// Apply node rule `r` at node (i, j) whose pre-rule populations are f.
-// NEEM rules read the interior neighbour's post-stream populations
-// (re-gathered) and its rho/u of the previous step (mac_in); VelocityBB reads
-// the node's own rho of the previous step; OpenNeumann copies the unknowns
-// from the neighbour.  On `finish` the reference's vel / rho / total_rho
-// overrides are written into mac_override.
-template <int NC, int S>
-__device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const typename Pop<S>::R* mi,
-                           const lbm_neem_rule& r, uint32_t i, uint32_t j, uint32_t n,
+// Apply node rule `r` at node x (global index n) whose pre-rule populations
+// are f.  NEEM rules read the interior neighbour's post-stream populations
+// and its rho/u of the previous step; VelocityBB reads the node's own rho of
+// the previous step; OpenNeumann copies the unknowns from the neighbour.  On
+// `finish` the reference's vel / rho / total_rho overrides are written into
+// mac_override.
+template <int NC, int S, class V>
+__device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, const typename V::Node& x, uint32_t n,
                            typename Pop<S>::R (&f)[NC][9], typename Pop<S>::R* mac_override) {
   using R = typename Pop<S>::R;
   const int d = r.direction;
@@ -132,7 +152,7 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
     const R uxw = r.ux, uyw = r.uy;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const R rw = mi[(size_t)c * ps + n];
+      const R rw = v.macro(x, c);
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         if (neem_unknown(d, k)) {
@@ -149,9 +169,8 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
     case 3: di = -1; break;
     case 4: dj = -1; break;
   }
-  const uint32_t inb = (uint32_t)((int)i - di), jnb = (uint32_t)((int)j - dj);
-  const Nbr nb(inb, jnb, a.nx, a.ny);
-  const uint32_t code_nb = a.code[nb.n];
+  const typename V::Node y = v.back(x, di, dj);
+  const uint32_t bm_y = v.code_of(y) & LBM_CODE_BOUNCE_MASK;
   if (r.kind == LBM_RULE_OPEN_NEUMANN) {
     // BoundaryCondition.py:327-334: unknowns := f(x - c_d), then density and
     // velocity of the node (OpenBoundary.density / vel, :274-298)
@@ -159,7 +178,7 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int k = 0; k < 9; ++k)
-        if (neem_unknown(d, k)) f[c][k] = pop_at<S>(src, ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, c, k);
+        if (neem_unknown(d, k)) f[c][k] = v.pop(y, bm_y, c, k);
     if (mac_override) {
       R rtot = R(0), vx = R(0), vy = R(0);
 #pragma unroll
@@ -186,9 +205,12 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
     return;
   }
   R fn[NC][9];
-  gather<NC, S>(src, ps, nb, code_nb & LBM_CODE_BOUNCE_MASK, fn);
-  const R ux_nb = mi[(size_t)(NC + 1) * ps + nb.n];
-  const R uy_nb = mi[(size_t)(NC + 2) * ps + nb.n];
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) fn[c][k] = v.pop(y, bm_y, c, k);
+  const R ux_nb = v.macro(y, NC + 1);
+  const R uy_nb = v.macro(y, NC + 2);
   const R uv_nb = ux_nb * ux_nb + uy_nb * uy_nb;
   if (r.kind == LBM_NEEM_VELOCITY) {
     const R uxw = r.ux, uyw = r.uy;
@@ -198,7 +220,7 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
     for (int c = 0; c < NC; ++c) rw[c] = neem_rhow<R>(d, uxw, uyw, f[c]);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const R rho_nb = mi[(size_t)c * ps + nb.n];
+      const R rho_nb = v.macro(y, c);
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         f[c][k] = feq_k<R>(k, uxw, uyw, rw[c], uvw) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
@@ -212,8 +234,8 @@ __device__ void apply_neem(const Lat& a, const typename Pop<S>::T* src, const ty
   } else {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const R rho_own = mi[(size_t)c * ps + n];
-      const R rho_nb = mi[(size_t)c * ps + nb.n];
+      const R rho_own = v.macro(x, c);
+      const R rho_nb = v.macro(y, c);
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         if (neem_unknown(d, k))
@@ -418,9 +440,10 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
     const Nbr nb(i, j, a.nx, a.ny);
     gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
     const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
-    if (e)
-      apply_neem<NC, S>(a, src, (const R*)a.mac[p], a.neem[e - 1], i, j, n, f,
-                        MODE == MODE_FINISH ? (R*)a.mac[p] : nullptr);
+    if (e) {
+      const GlobalView<S> v{src, (const R*)a.mac[p], a.code, ps, a.nx, a.ny};
+      apply_rule<NC, S>(a, v, a.neem[e - 1], v.node(i, j), n, f, MODE == MODE_FINISH ? (R*)a.mac[p] : nullptr);
+    }
   }
 
   if constexpr (MODE == MODE_FINISH) {

@@ -11,6 +11,10 @@
   f - feq, a cancellation of O(0.1) numbers.
 * Relaxation state tau = clamp(3 m g^(n-1) + 0.5): the shear-rate bound
   propagated through d tau / d g = 3 m (n-1) g^(n-2), plus 1e-5 relative.
+* Long runs (10^4 steps) compare two FP32 restatements whose per-step
+  rounding differences accumulate: `growth` widens every bound (10 there,
+  against sqrt(10^4) = 100 for an unbiased random walk); the tests also state
+  each run's distance to the FP64 path.
 """
 import numpy as np
 
@@ -22,7 +26,7 @@ SCALE_FIELDS = {"f", "f2", "rho", "total_rho", "pressure", "total_pressure", "ps
 MOMENT_FIELDS = {"vel", "body_force", "SCforce"}
 
 
-def check(name, got, ref, log=True):
+def check(name, got, ref, log=True, growth=1.0):
     got = np.asarray(got, np.float64)
     ref = np.asarray(ref, np.float64)
     assert got.shape == ref.shape, (name, got.shape, ref.shape)
@@ -37,6 +41,7 @@ def check(name, got, ref, log=True):
         bound = SHEAR_ATOL + 1e-4 * np.abs(ref)
     else:
         bound = POP_RTOL * scale
+    bound = bound * growth
     ok = np.all(err <= bound)
     if log:
         print(f"  {name}: max abs err {err.max():.3e} (scale {scale:.3e})", flush=True)
