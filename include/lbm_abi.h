@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 5
+#define LBM_ABI_VERSION 6
 
 /* status codes */
 #define LBM_OK 0
@@ -202,8 +202,11 @@ int lbm_batch_set(lbm_batch* b, int32_t index, int32_t active, int32_t start_ste
 int lbm_batch_commit(lbm_batch* b, void* stream);
 int lbm_batch_tiles(const lbm_batch* b);
 int lbm_batch_capacity(const lbm_batch* b);
-/* Reserve room for `tiles` tiles in the tile table (before capturing a graph). */
+/* Reserve room for `tiles` tiles in the one-step tile table, or `tiles` in
+ * the two-step table (reserve2), before capturing a graph: after a capture a
+ * commit that needs more room fails instead of moving the tables.          */
 int lbm_batch_reserve(lbm_batch* b, int32_t tiles);
+int lbm_batch_reserve2(lbm_batch* b, int32_t tiles);
 /* nsteps launches (launch t reads parity (parity + t) % 2; the last writes
  * every node's macro fields).  grid_blocks: persistent grid, multiple of 8
  * (0 = 2048).                                                              */
@@ -214,6 +217,22 @@ int lbm_batch_finish(lbm_batch* b, int32_t parity, int32_t grid_blocks, void* st
  * graph (stream must not be NULL); replay with lbm_graph_launch.          */
 int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream,
                     void** graph_exec);
+/* Two steps per launch for the whole batch where it allows it (as
+ * lbm_run_blocked: every active lattice without power law, storage other
+ * than FP64; ring width 2 if any lattice has node rules).  lbm_batch_mode
+ * reports what the last commit allows: 0 = one step per launch, 1 or 2 =
+ * two-step launches with that ring width.  A two-step launch flips the
+ * buffers once; the parity of the buffer with the newest f2 is returned in
+ * *out_parity.  grid_blocks applies to the one-step launches only; a
+ * two-step launch has one block per tile (per entry of the table's capacity
+ * in a graph).  The graph variant captures the same launch sequence; it
+ * stays valid while the mode does not change.  No counterpart in the
+ * reference. */
+int lbm_batch_mode(const lbm_batch* b);
+int lbm_batch_run_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, int32_t* out_parity,
+                          void* stream);
+int lbm_batch_graph_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream,
+                            void** graph_exec, int32_t* out_parity);
 int lbm_graph_launch(void* graph_exec, void* stream);
 int lbm_graph_destroy(void* graph_exec);
 /* out[2l] = sum |u - u_old|^2, out[2l+1] = sum |u|^2 over lattice l's active

@@ -154,11 +154,14 @@ __device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, con
     for (int c = 0; c < NC; ++c) {
       const R rw = v.macro(x, c);
 #pragma unroll
-      for (int k = 0; k < 9; ++k)
-        if (neem_unknown(d, k)) {
-          const R cv = (R)cx(k) * uxw + (R)cy(k) * uyw;
-          f[c][k] = f[c][k] + R(6) * wgt<R>(k) * rw * cv;
-        }
+      for (int k = 0; k < 9; ++k) {
+        // a select per population, not a branch: branches on d let the
+        // compiler merge the updates of different k into one store through
+        // a pointer, which moves f to scratch memory
+        const R cv = (R)cx(k) * uxw + (R)cy(k) * uyw;
+        const R upd = f[c][k] + R(6) * wgt<R>(k) * rw * cv;
+        f[c][k] = neem_unknown(d, k) ? upd : f[c][k];
+      }
     }
     return;
   }
@@ -177,8 +180,10 @@ __device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, con
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int k = 0; k < 9; ++k)
-        if (neem_unknown(d, k)) f[c][k] = v.pop(y, bm_y, c, k);
+      for (int k = 0; k < 9; ++k) {
+        const R g = neem_unknown(d, k) ? v.pop(y, bm_y, c, k) : f[c][k];
+        f[c][k] = g;
+      }
     if (mac_override) {
       R rtot = R(0), vx = R(0), vy = R(0);
 #pragma unroll
@@ -237,9 +242,10 @@ __device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, con
       const R rho_own = v.macro(x, c);
       const R rho_nb = v.macro(y, c);
 #pragma unroll
-      for (int k = 0; k < 9; ++k)
-        if (neem_unknown(d, k))
-          f[c][k] = feq_k<R>(k, ux_nb, uy_nb, rho_own, uv_nb) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
+      for (int k = 0; k < 9; ++k) {
+        const R upd = feq_k<R>(k, ux_nb, uy_nb, rho_own, uv_nb) + fn[c][k] - feq_k<R>(k, ux_nb, uy_nb, rho_nb, uv_nb);
+        f[c][k] = neem_unknown(d, k) ? upd : f[c][k];
+      }
     }
     if (mac_override) {
       mac_override[(size_t)NC * ps + n] = (R)r.rho;
@@ -345,7 +351,11 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
   o.ux = vx / o.rtot;
   o.uy = vy / o.rtot;
 
-  // S5 power law + S6 collide, per component
+  // S5 power law + S6 collide, per component.  COLL < 0: the collision model
+  // is read from the descriptor (batches mixing models share one kernel; the
+  // branches are wave-uniform).
+  const int coll = COLL >= 0 ? COLL : a.coll;
+  const bool is_mrt = coll == LBM_MRT, is_trt = coll == LBM_TRT;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     R tau = a.tau0[c], tau_anti = a.tau_anti0[c], s_shear = a.s_shear[c];
@@ -354,28 +364,37 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
     o.state[c] = state_in[c];
     if (a.tau_mode == LBM_TAU_FIELD) {
       const R state = state_in[c];
-      if constexpr (COLL == LBM_MRT) {
+      if (is_mrt) {
         s_shear = state;
       } else {
         tau = state;
-        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
+        if (is_trt) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
       }
     } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
       const R state = state_in[c];
-      const R g = shear_rate_of<COLL, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
+      const R g = is_mrt ? shear_rate_of<LBM_MRT, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk)
+                         : shear_rate_of<LBM_BGK, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
       o.gam[c] = g;
       const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
       const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
-      if constexpr (COLL == LBM_MRT) {
+      if (is_mrt) {
         s_shear = R(1) / t;
         o.state[c] = s_shear;
       } else {
         tau = t;
         o.state[c] = t;
-        if constexpr (COLL == LBM_TRT) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
+        if (is_trt) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
       }
     }
-    collide<COLL, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+    if constexpr (COLL >= 0) {
+      collide<COLL, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+    } else if (is_mrt) {
+      collide<LBM_MRT, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+    } else if (is_trt) {
+      collide<LBM_TRT, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+    } else {
+      collide<LBM_BGK, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
+    }
   }
 }
 
@@ -492,64 +511,108 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // ---------------------------------------------------------------------------
 // Two steps per launch (temporal blocking through LDS).
 //
-// A block owns a tile of kTI rows x kTJ columns.  Phase A runs step n on the
-// tile and a one-node ring around it (pull gathers from HBM, exactly as the
-// one-step kernel) and keeps the post-collision populations and relaxation
-// state in LDS; phase B runs step n+1 on the tile, gathering from LDS, and
-// writes only its result to HBM.  Per pair of steps HBM sees the ring's
-// re-reads (~1.2x of one read) plus one write, instead of two reads and two
-// writes.  The per-node arithmetic is step_math, shared with the one-step
-// kernel, so both give bit-identical fields.
+// A block owns a tile of TI rows x kTJ columns.  Phase A runs step n on the
+// tile and a ring of H nodes around it (pull gathers and node rules read HBM,
+// exactly as the one-step kernel) and keeps the post-collision populations,
+// relaxation state and -- with node rules -- the macroscopic fields in LDS;
+// phase B runs step n+1 on the tile, reading step n from LDS, and writes only
+// its result to HBM.  Per pair of steps HBM sees the ring's re-reads plus one
+// write, instead of two reads and two writes.  The per-node arithmetic
+// (step_math, apply_rule) is shared with the one-step kernel, so both give
+// bit-identical fields.
 //
-// Used for boundary lists without node rules (NEEM / VelocityBB /
-// OpenNeumann read a neighbour's post-stream populations, two nodes away)
-// and without Shan-Chen (psi needs its own pass); the host falls back to
-// one-step launches otherwise.  The result of a launch goes to pop[parity^1]
-// like a one-step launch, so a two-step launch flips the buffers once.
+// Ring width: H = 1 streams; H = 2 when the lattice has node rules (a NEEM or
+// OpenNeumann node reads its interior neighbour's post-stream populations,
+// two nodes away).  Shan-Chen (psi needs its own pass) and FP64 storage run
+// one step per launch.  The result of a launch goes to pop[parity^1] like a
+// one-step launch: a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
 constexpr int kTJ = 64;       // tile columns: one wavefront per tile row
-constexpr int kTI = 13;       // tile rows: the ring-extended tile (15 x 66) fits one 1024-thread round
 constexpr int kBlock2 = 1024;
+
+template <int H>
+struct TileShape {
+  // tile rows chosen so the ring-extended tile fits one round of the block:
+  // (13 + 2) x 66 = 990 and (11 + 4) x 68 = 1020 nodes
+  static constexpr int TI = H == 1 ? 13 : 11;
+  static constexpr int RJ = kTJ + 2 * H;
+  static constexpr int RN = (TI + 2 * H) * RJ;
+  static_assert(RN <= kBlock2, "ring-extended tile larger than the block");
+};
+
+template <int NC, int S, int H>
+struct TileLds {
+  using R = typename Pop<S>::R;
+  static constexpr int RN = TileShape<H>::RN;
+  R f[NC * 9][RN];                      // post-collision populations of step n
+  R mac[H == 2 ? NC + 3 : 1][RN];       // rho_c, total_rho, ux, uy of step n (node rules)
+  R st[NC][RN];                         // relaxation state after step n
+  uint16_t code[RN];
+};
+
+// Step n as phase B reads it: the LDS tile (nodes addressed by their index r
+// in the ring-extended tile, plus global coordinates for the rare pull from a
+// node outside the fluid group, whose value is the stored one in HBM).
+template <int NC, int S, int H>
+struct LdsView {
+  using T = typename Pop<S>::T;
+  using R = typename Pop<S>::R;
+  static constexpr int RJ = TileShape<H>::RJ;
+  struct Node {
+    uint32_t r, gi, gj;
+  };
+  const TileLds<NC, S, H>* L;
+  const T* stale;
+  uint32_t ps, nx, ny;
+  __device__ __forceinline__ Node back(const Node& x, int di, int dj) const {
+    return Node{(uint32_t)((int)x.r - di * RJ - dj), (uint32_t)((int)(x.gi + nx) - di) % nx,
+                (uint32_t)((int)(x.gj + ny) - dj) % ny};
+  }
+  __device__ __forceinline__ uint32_t code_of(const Node& x) const { return L->code[x.r]; }
+  __device__ __forceinline__ R macro(const Node& x, int plane) const { return L->mac[plane][x.r]; }
+  __device__ __forceinline__ R pop(const Node& x, uint32_t bm, int c, int k) const {
+    const bool bounce = (k > 0) && ((bm >> (k - 1)) & 1u);
+    if (bounce) return L->f[c * 9 + opp(k)][x.r];
+    const uint32_t s1 = (uint32_t)((int)x.r - cx(k) * RJ - cy(k));
+    if (L->code[s1] & LBM_CODE_ACTIVE) return L->f[c * 9 + k][s1];
+    const Nbr nb(x.gi, x.gj, nx, ny);
+    return Pop<S>::ld(stale + (size_t)(c * 9 + k) * ps, nb.src(k), k);
+  }
+};
 
 struct Launch2 {
   uint32_t parity;            // pop[parity]: f2 of step n-1 (post-stream f if local_read)
-  int32_t local_read;
+  int32_t local_read;         // single lattice: the first launch reads f in place
+  int32_t step;               // batches: launch index (in-place read where == start_step)
   int32_t write_macro;        // step n+1 ends the run
   int32_t xcd_remap;
-  uint32_t tiles_i, tiles_j;
-  const void* tau_in;         // relaxation state before step n (power law)
-  void* tau_out;              // relaxation state after step n+1 (power law)
+  uint32_t tiles_i, tiles_j;  // single lattice
+  const void* tau_in;         // single lattice, power law: state before step n
+  void* tau_out;              // single lattice, power law: state after step n+1
 };
 
-template <int COLL, int NC, int S>
-__global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
+template <int COLL, int NC, int S, int H>
+__device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
+                                               const void* tau_in, void* tau_out, TileLds<NC, S, H>& L) {
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
-  constexpr int RJ = kTJ + 2, RN = (kTI + 2) * RJ;
-  __shared__ R sf[NC * 9][RN];
-  __shared__ R sst[NC][RN];
-  __shared__ uint16_t scode[RN];
-
-  const uint32_t nt = q.tiles_i * q.tiles_j;
-  uint32_t b = blockIdx.x;
-  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
-  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  const uint32_t i0 = ti * kTI, j0 = tj * kTJ;
+  using Shape = TileShape<H>;
+  constexpr int RJ = Shape::RJ, RN = Shape::RN, TI = Shape::TI;
   const uint32_t ps = a.ps, p = q.parity, nx = a.nx, ny = a.ny;
   const bool pl = a.tau_mode == LBM_TAU_POWER_LAW;
-  const R* tau_a = (const R*)(pl ? q.tau_in : a.tau);
+  const R* tau_a = (const R*)(pl ? tau_in : a.tau);
 
   // phase A: step n on the tile and its ring (wrapped periodically)
   const T* __restrict__ src = (const T*)a.pop[p];
   for (uint32_t r1 = threadIdx.x; r1 < (uint32_t)RN; r1 += kBlock2) {
     const uint32_t rr = r1 / RJ, cc = r1 - rr * RJ;
-    const uint32_t gi = (i0 + rr + nx - 1) % nx, gj = (j0 + cc + ny - 1) % ny;
+    const uint32_t gi = (i0 + rr + H * nx - H) % nx, gj = (j0 + cc + H * ny - H) % ny;
     const uint32_t n = gi * ny + gj;
     const uint32_t code = a.code[n];
-    scode[r1] = (uint16_t)code;
+    L.code[r1] = (uint16_t)code;
     if (!(code & LBM_CODE_ACTIVE)) continue;
     R f[NC][9];
-    if (q.local_read) {
+    if (local) {
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -557,6 +620,13 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
     } else {
       const Nbr nb(gi, gj, nx, ny);
       gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
+      if constexpr (H == 2) {
+        const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
+        if (e) {
+          const GlobalView<S> v{src, (const R*)a.mac[p], a.code, ps, nx, ny};
+          apply_rule<NC, S>(a, v, a.neem[e - 1], v.node(gi, gj), n, f, nullptr);
+        }
+      }
     }
     R st_in[NC];
 #pragma unroll
@@ -566,8 +636,15 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) sf[c * 9 + k][r1] = o.f[c][k];
-      sst[c][r1] = o.state[c];
+      for (int k = 0; k < 9; ++k) L.f[c * 9 + k][r1] = o.f[c][k];
+      L.st[c][r1] = o.state[c];
+    }
+    if constexpr (H == 2) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) L.mac[c][r1] = o.rho[c];
+      L.mac[NC][r1] = o.rtot;
+      L.mac[NC + 1][r1] = o.ux;
+      L.mac[NC + 2][r1] = o.uy;
     }
   }
   __syncthreads();
@@ -575,13 +652,14 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
   // phase B: step n+1 on the tile from LDS
   T* __restrict__ dst = (T*)a.pop[p ^ 1u];
   R* mo = (R*)a.mac[p ^ 1u];
-  for (uint32_t t = threadIdx.x; t < (uint32_t)(kTI * kTJ); t += kBlock2) {
+  const LdsView<NC, S, H> v{&L, dst, ps, nx, ny};
+  for (uint32_t t = threadIdx.x; t < (uint32_t)(TI * kTJ); t += kBlock2) {
     const uint32_t rr = t / kTJ, cc = t - rr * kTJ;
     const uint32_t gi = i0 + rr, gj = j0 + cc;
     if (gi >= nx || gj >= ny) continue;
     const uint32_t n = gi * ny + gj;
-    const uint32_t r1 = (rr + 1) * RJ + cc + 1;
-    const uint32_t code = scode[r1];
+    const uint32_t r1 = (rr + H) * RJ + cc + H;
+    const uint32_t code = L.code[r1];
     if (!(code & LBM_CODE_ACTIVE)) {
       if (q.write_macro) {
 #pragma unroll
@@ -589,38 +667,64 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
       }
       continue;
     }
-    R f[NC][9];
+    const typename LdsView<NC, S, H>::Node x{r1, gi, gj};
     const uint32_t bm = code & LBM_CODE_BOUNCE_MASK;
+    R f[NC][9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const bool bounce = (k > 0) && ((bm >> (k - 1)) & 1u);
-      const uint32_t s1 = bounce ? r1 : (uint32_t)((int)r1 - cx(k) * RJ - cy(k));
-      const int kk = bounce ? opp(k) : k;
-      if (bounce || (scode[s1] & LBM_CODE_ACTIVE)) {
+    for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int c = 0; c < NC; ++c) f[c][k] = sf[c * 9 + kk][s1];
-      } else {
-        // pull from a node outside the fluid group: its value in the buffer
-        // step n would have written (never written there: the stored one)
-        const Nbr nb(gi, gj, nx, ny);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) f[c][k] = Pop<S>::ld(dst + (size_t)(c * 9 + k) * ps, nb.src(k), k);
-      }
+      for (int k = 0; k < 9; ++k) f[c][k] = v.pop(x, bm, c, k);
+    if constexpr (H == 2) {
+      const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
+      if (e) apply_rule<NC, S>(a, v, a.neem[e - 1], x, n, f, nullptr);
     }
     R st_in[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      st_in[c] = pl ? sst[c][r1] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
+      st_in[c] = pl ? L.st[c][r1] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
     StepOut<NC, R> o;
     step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
-      if (pl) ((R*)q.tau_out)[(size_t)c * ps + n] = o.state[c];
+      if (pl) ((R*)tau_out)[(size_t)c * ps + n] = o.state[c];
     }
-    store_macro<NC, R>(a, mo, n, o, false, q.write_macro != 0);
+    store_macro<NC, R>(a, mo, n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
   }
+}
+
+template <int COLL, int NC, int S, int H>
+__global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
+  __shared__ TileLds<NC, S, H> L;
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  tile_two_steps<COLL, NC, S, H>(a, q, q.local_read != 0, ti * TileShape<H>::TI, tj * kTJ, q.tau_in, q.tau_out, L);
+}
+
+// Batched two-step launch: one block per tile of many lattices (tile ->
+// lattice and tile -> index tables and the tile count in device memory; a
+// captured graph launches one block per entry of the table's capacity and
+// blocks past the current count exit, so the graph follows admissions and
+// retirements).  Blocks are remapped so that each XCD takes one contiguous
+// range of the tile list.  Power-law lattices are not batched this way
+// (their state would need a second buffer per lattice).
+template <int NC, int S, int H>
+__global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
+    const Lat* __restrict__ items, const uint32_t* __restrict__ tile_lat, const uint32_t* __restrict__ tile_idx,
+    const uint32_t* __restrict__ tile_count, const Launch2 q) {
+  __shared__ TileLds<NC, S, H> L;
+  const uint32_t per = gridDim.x >> 3;   // the host launches a multiple of 8 blocks
+  const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (w >= *tile_count) return;
+  const uint32_t l = __builtin_amdgcn_readfirstlane(tile_lat[w]);
+  const uint32_t t = __builtin_amdgcn_readfirstlane(tile_idx[w]);
+  const Lat& a = items[l];
+  const uint32_t tiles_j = (a.ny + kTJ - 1) / kTJ;
+  const uint32_t ti = t / tiles_j, tj = t - ti * tiles_j;
+  tile_two_steps<-1, NC, S, H>(a, q, a.start_step == q.step, ti * TileShape<H>::TI, tj * kTJ, a.tau, nullptr, L);
 }
 
 // Persistent batched step over the 256-node tiles of many lattices.  Each XCD
@@ -882,6 +986,15 @@ struct lbm_batch {
   size_t tile_cap = 0;
   uint32_t tiles = 0;
   std::vector<uint32_t> host_tiles;
+  // two-step launches: (lattice, tile) tables over the ring-extended tiling
+  uint32_t* tile2_lat = nullptr;
+  uint32_t* tile2_idx = nullptr;
+  uint32_t* tile2_count = nullptr;
+  size_t tile2_cap = 0;
+  uint32_t tiles2 = 0;
+  int mode2 = 0;                     // 0: one step per launch, else the ring width H
+  std::vector<uint32_t> host_t2lat, host_t2idx;
+  bool captured = false;             // a graph holds the table addresses: no reallocation
 };
 
 extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
@@ -912,27 +1025,33 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
 namespace lbm {
 using Kernel2Fn = void (*)(const Lat, const Launch2);
 
-template <int NC, int S>
+template <int NC, int S, int H>
 Kernel2Fn pick2_coll(int coll) {
   switch (coll) {
-    case LBM_BGK: return k_fused2<LBM_BGK, NC, S>;
-    case LBM_TRT: return k_fused2<LBM_TRT, NC, S>;
-    default: return k_fused2<LBM_MRT, NC, S>;
+    case LBM_BGK: return k_fused2<LBM_BGK, NC, S, H>;
+    case LBM_TRT: return k_fused2<LBM_TRT, NC, S, H>;
+    default: return k_fused2<LBM_MRT, NC, S, H>;
+  }
+}
+template <int H>
+Kernel2Fn pick2_h(int nc, int store, int coll) {
+  switch (store) {
+    case LBM_STORE_F32: return nc == 1 ? pick2_coll<1, LBM_STORE_F32, H>(coll) : pick2_coll<2, LBM_STORE_F32, H>(coll);
+    case LBM_STORE_F16_SHIFT:
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT, H>(coll) : pick2_coll<2, LBM_STORE_F16_SHIFT, H>(coll);
+    case LBM_STORE_BF16_SHIFT:
+      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT, H>(coll) : pick2_coll<2, LBM_STORE_BF16_SHIFT, H>(coll);
+    default: return nullptr;
   }
 }
 // nullptr where the two-step kernel is not built: FP64 storage (the
 // high-precision reference path; in a 1024-thread block its registers would
 // spill) runs one step per launch.
-Kernel2Fn pick2(int nc, int store, int coll) {
-  switch (store) {
-    case LBM_STORE_F32: return nc == 1 ? pick2_coll<1, LBM_STORE_F32>(coll) : pick2_coll<2, LBM_STORE_F32>(coll);
-    case LBM_STORE_F16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT>(coll) : pick2_coll<2, LBM_STORE_F16_SHIFT>(coll);
-    case LBM_STORE_BF16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT>(coll) : pick2_coll<2, LBM_STORE_BF16_SHIFT>(coll);
-    default: return nullptr;
-  }
+Kernel2Fn pick2(int nc, int store, int coll, int h) {
+  return h == 1 ? pick2_h<1>(nc, store, coll) : pick2_h<2>(nc, store, coll);
 }
+int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
+uint32_t tile_rows(int h) { return h == 1 ? (uint32_t)TileShape<1>::TI : (uint32_t)TileShape<2>::TI; }
 }  // namespace lbm
 
 extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
@@ -946,8 +1065,9 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   if (!out_parity) return fail("null out_parity");
   hipStream_t s = (hipStream_t)stream;
   const bool pl = lat->tau_mode == LBM_TAU_POWER_LAW;
-  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision);
-  const bool ok2 = two && lat->n_neem == 0 && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
+  const int h = ring_width(lat);
+  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h);
+  const bool ok2 = two && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
   if (!ok2) {
     rc = lbm_run(lat, fld, neem, nsteps, parity, first_local, stream);
     if (!rc) *out_parity = parity ^ (nsteps & 1);
@@ -959,7 +1079,7 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   void* tnext = tau_next;
   Launch2 q2;
   memset(&q2, 0, sizeof q2);
-  q2.tiles_i = (a.nx + kTI - 1) / kTI;
+  q2.tiles_i = (a.nx + tile_rows(h) - 1) / tile_rows(h);
   q2.tiles_j = (a.ny + kTJ - 1) / kTJ;
   const uint64_t nt = (uint64_t)q2.tiles_i * q2.tiles_j;
   if (nt >= (1ull << 31)) return fail("lattice too large for the two-step tiling");
@@ -1034,10 +1154,12 @@ extern "C" int lbm_batch_create(const lbm_lattice* lats, const lbm_fields* flds,
   int rc = hip_check(hipMalloc((void**)&b->items, sizeof(Lat) * count), "hipMalloc(batch items)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->tile_count, sizeof(uint32_t)), "hipMalloc(tile count)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->acc, sizeof(double) * 2 * count), "hipMalloc(residuals)");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->tile2_count, sizeof(uint32_t)), "hipMalloc(tile2 count)");
   if (rc) {
     (void)hipFree(b->items);
     (void)hipFree(b->tile_count);
     (void)hipFree(b->acc);
+    (void)hipFree(b->tile2_count);
     delete b;
     return rc;
   }
@@ -1083,8 +1205,51 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
     for (uint32_t t = 0; t < nt; ++t) b->host_tiles.push_back((uint32_t)l);
   }
   b->tiles = (uint32_t)b->host_tiles.size();
+  // two-step tiling: every active lattice without power law or fp64 storage
+  bool ok2 = b->storage != LBM_STORE_F64;
+  int h = 1;
+  for (int l = 0; l < b->count; ++l) {
+    const Lat& a = b->host[l];
+    if (!a.active) continue;
+    if (a.tau_mode == LBM_TAU_POWER_LAW) ok2 = false;
+    if (a.n_neem > 0) h = 2;
+  }
+  b->host_t2lat.clear();
+  b->host_t2idx.clear();
+  if (ok2) {
+    for (int l = 0; l < b->count; ++l) {
+      const Lat& a = b->host[l];
+      if (!a.active) continue;
+      const uint32_t nt = ((a.nx + tile_rows(h) - 1) / tile_rows(h)) * ((a.ny + kTJ - 1) / kTJ);
+      for (uint32_t t = 0; t < nt; ++t) {
+        b->host_t2lat.push_back((uint32_t)l);
+        b->host_t2idx.push_back(t);
+      }
+    }
+  }
+  b->mode2 = ok2 ? h : 0;
+  b->tiles2 = (uint32_t)b->host_t2lat.size();
+  if (b->tiles2 > b->tile2_cap) {
+    if (b->captured)
+      return fail("two-step tile table full (%u > %zu) after a graph capture: reserve before capturing",
+                  b->tiles2, b->tile2_cap);
+    (void)hipFree(b->tile2_lat);
+    (void)hipFree(b->tile2_idx);
+    b->tile2_cap = b->tiles2 + b->tiles2 / 16 + 64;
+    int rc = hip_check(hipMalloc((void**)&b->tile2_lat, sizeof(uint32_t) * b->tile2_cap), "hipMalloc(tile2 table)");
+    if (!rc) rc = hip_check(hipMalloc((void**)&b->tile2_idx, sizeof(uint32_t) * b->tile2_cap), "hipMalloc(tile2 table)");
+    if (rc) {
+      b->tile2_lat = b->tile2_idx = nullptr;
+      b->tile2_cap = 0;
+      b->mode2 = 0;
+      return rc;
+    }
+  }
   if (b->tiles > b->tile_cap) {
     // a captured graph holds the old table's address: grow only before capture
+    if (b->captured)
+      return fail("tile table full (%u > %zu) after a graph capture: reserve before capturing", b->tiles,
+                  b->tile_cap);
     if (b->tile_item) (void)hipFree(b->tile_item);
     b->tile_cap = b->tiles + b->tiles / 4 + 1024;
     int rc = hip_check(hipMalloc((void**)&b->tile_item, sizeof(uint32_t) * b->tile_cap), "hipMalloc(tile table)");
@@ -1102,6 +1267,16 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
   if (!rc)
     rc = hip_check(hipMemcpyAsync(b->tile_count, &b->tiles, sizeof(uint32_t), hipMemcpyHostToDevice, s),
                    "upload tile count");
+  if (!rc && b->tiles2) {
+    rc = hip_check(hipMemcpyAsync(b->tile2_lat, b->host_t2lat.data(), sizeof(uint32_t) * b->tiles2,
+                                  hipMemcpyHostToDevice, s), "upload tile2 table");
+    if (!rc)
+      rc = hip_check(hipMemcpyAsync(b->tile2_idx, b->host_t2idx.data(), sizeof(uint32_t) * b->tiles2,
+                                    hipMemcpyHostToDevice, s), "upload tile2 table");
+  }
+  if (!rc)
+    rc = hip_check(hipMemcpyAsync(b->tile2_count, &b->tiles2, sizeof(uint32_t), hipMemcpyHostToDevice, s),
+                   "upload tile2 count");
   // the host vectors are read by the copies above: wait before they can change
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   return rc;
@@ -1126,6 +1301,31 @@ extern "C" int lbm_batch_reserve(lbm_batch* b, int32_t tiles) {
   return rc;
 }
 
+extern "C" int lbm_batch_reserve2(lbm_batch* b, int32_t tiles) {
+  using namespace lbm;
+  if (!b || tiles < 0) return fail("bad lbm_batch_reserve2 arguments");
+  if ((size_t)tiles <= b->tile2_cap) return LBM_OK;
+  if (b->captured) return fail("cannot grow the two-step tile table after a graph capture");
+  uint32_t *t2l = nullptr, *t2i = nullptr;
+  int rc = hip_check(hipMalloc((void**)&t2l, sizeof(uint32_t) * (size_t)tiles), "hipMalloc(tile2 table)");
+  if (!rc) rc = hip_check(hipMalloc((void**)&t2i, sizeof(uint32_t) * (size_t)tiles), "hipMalloc(tile2 table)");
+  if (!rc && b->tiles2) {
+    rc = hip_check(hipMemcpy(t2l, b->tile2_lat, sizeof(uint32_t) * b->tiles2, hipMemcpyDeviceToDevice), "copy");
+    if (!rc) rc = hip_check(hipMemcpy(t2i, b->tile2_idx, sizeof(uint32_t) * b->tiles2, hipMemcpyDeviceToDevice), "copy");
+  }
+  if (rc) {
+    (void)hipFree(t2l);
+    (void)hipFree(t2i);
+    return rc;
+  }
+  (void)hipFree(b->tile2_lat);
+  (void)hipFree(b->tile2_idx);
+  b->tile2_lat = t2l;
+  b->tile2_idx = t2i;
+  b->tile2_cap = (size_t)tiles;
+  return LBM_OK;
+}
+
 namespace {
 int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t write_macro, int32_t grid,
                  hipStream_t s) {
@@ -1143,6 +1343,96 @@ int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t 
   return hip_check(hipGetLastError(), "batch launch");
 }
 }  // namespace
+
+namespace {
+using Batch2Fn = void (*)(const lbm::Lat*, const uint32_t*, const uint32_t*, const uint32_t*, const lbm::Launch2);
+template <int NC, int S>
+Batch2Fn pick_batch2_s(int h) {
+  return h == 1 ? lbm::k_batch2<NC, S, 1> : lbm::k_batch2<NC, S, 2>;
+}
+Batch2Fn pick_batch2(int nc, int store, int h) {
+  switch (store) {
+    case LBM_STORE_F32: return nc == 1 ? pick_batch2_s<1, LBM_STORE_F32>(h) : pick_batch2_s<2, LBM_STORE_F32>(h);
+    case LBM_STORE_F16_SHIFT:
+      return nc == 1 ? pick_batch2_s<1, LBM_STORE_F16_SHIFT>(h) : pick_batch2_s<2, LBM_STORE_F16_SHIFT>(h);
+    case LBM_STORE_BF16_SHIFT:
+      return nc == 1 ? pick_batch2_s<1, LBM_STORE_BF16_SHIFT>(h) : pick_batch2_s<2, LBM_STORE_BF16_SHIFT>(h);
+    default: return nullptr;
+  }
+}
+
+// The launch sequence of nsteps batched steps, two per launch where the
+// batch allows it; returns the parity after it.
+int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, hipStream_t s, bool capture,
+                   int32_t* out) {
+  using namespace lbm;
+  if (!b->tile_item) return fail("commit the batch before stepping it");
+  Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2) : nullptr;
+  uint32_t p = (uint32_t)parity;
+  int t0 = 0;
+  if (two) {
+    // one block per tile: the current count, or the capacity for a graph
+    const size_t nb = capture ? b->tile2_cap : b->tiles2;
+    const int g2 = nb ? (int)((nb + 7) / 8 * 8) : 8;   // an empty batch: 8 blocks that exit
+    const int pairs = nsteps / 2;
+    for (int t = 0; t < pairs; ++t) {
+      Launch2 q2;
+      memset(&q2, 0, sizeof q2);
+      q2.parity = p;
+      q2.step = t;
+      q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
+      hipLaunchKernelGGL(two, dim3(g2), dim3(kBlock2), 0, s, b->items, b->tile2_lat, b->tile2_idx, b->tile2_count, q2);
+      int rc = hip_check(hipGetLastError(), "batched two-step launch");
+      if (rc) return rc;
+      p ^= 1u;
+    }
+    t0 = pairs;
+    nsteps -= 2 * pairs;
+  }
+  for (int t = 0; t < nsteps; ++t) {
+    int rc = batch_launch(b, MODE_STEP, p, t0 + t, t == nsteps - 1, grid, s);
+    if (rc) return rc;
+    p ^= 1u;
+  }
+  *out = (int32_t)p;
+  return LBM_OK;
+}
+}  // namespace
+
+extern "C" int lbm_batch_mode(const lbm_batch* b) { return b ? b->mode2 : -1; }
+
+extern "C" int lbm_batch_run_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks,
+                                     int32_t* out_parity, void* stream) {
+  using namespace lbm;
+  if (!b || nsteps < 0 || (parity != 0 && parity != 1) || !out_parity)
+    return fail("bad lbm_batch_run_blocked arguments");
+  return batch_sequence(b, nsteps, parity, grid_blocks, (hipStream_t)stream, false, out_parity);
+}
+
+extern "C" int lbm_batch_graph_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks,
+                                       void* stream, void** graph_exec, int32_t* out_parity) {
+  using namespace lbm;
+  if (!b || nsteps < 1 || !graph_exec || !out_parity || (parity != 0 && parity != 1))
+    return fail("bad lbm_batch_graph_blocked arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) return fail("graph capture needs a non-null stream");
+  int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (rc) return rc;
+  b->captured = true;
+  rc = batch_sequence(b, nsteps, parity, grid_blocks, s, true, out_parity);
+  hipGraph_t g = nullptr;
+  const int rc2 = hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+  if (rc || rc2) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : rc2;
+  }
+  hipGraphExec_t ge = nullptr;
+  rc = hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipGraphDestroy(g);
+  if (rc) return rc;
+  *graph_exec = (void*)ge;
+  return LBM_OK;
+}
 
 extern "C" int lbm_batch_run(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, void* stream) {
   using namespace lbm;
@@ -1169,6 +1459,7 @@ extern "C" int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int
   if (!s) return fail("graph capture needs a non-null stream");
   int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (rc) return rc;
+  b->captured = true;
   for (int t = 0; t < nsteps && !rc; ++t)
     rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s);
   hipGraph_t g = nullptr;
@@ -1225,6 +1516,9 @@ extern "C" int lbm_batch_destroy(lbm_batch* b) {
   (void)hipFree(b->tile_item);
   (void)hipFree(b->tile_count);
   (void)hipFree(b->acc);
+  (void)hipFree(b->tile2_lat);
+  (void)hipFree(b->tile2_idx);
+  (void)hipFree(b->tile2_count);
   delete b;
   return LBM_OK;
 }

@@ -10,12 +10,16 @@ namespace lbm {
 
 // NEEMMethod.rhow, BoundaryCondition.py:118-129.
 template <typename R>
-__device__ __forceinline__ R neem_rhow(int d, R ux, R uy, const R* f) {
+__device__ __forceinline__ R neem_rhow(int d, R ux, R uy, const R (&f)[9]) {
+  const R sx = f[0] + f[2] + f[4];   // populations with c_x = 0
+  const R sy = f[0] + f[1] + f[3];   // populations with c_y = 0
+  const R out1 = f[1] + f[5] + f[8], out3 = f[3] + f[6] + f[7];
+  const R out2 = f[2] + f[5] + f[6], out4 = f[4] + f[7] + f[8];
   switch (d) {
-    case 2: return R(1) / (R(1) + uy) * (f[0] + f[1] + f[3] + R(2) * (f[2] + f[5] + f[6]));
-    case 4: return R(1) / (R(1) - uy) * (f[0] + f[1] + f[3] + R(2) * (f[4] + f[7] + f[8]));
-    case 1: return R(1) / (R(1) + ux) * (f[0] + f[2] + f[4] + R(2) * (f[1] + f[5] + f[8]));
-    case 3: return R(1) / (R(1) - ux) * (f[0] + f[2] + f[4] + R(2) * (f[3] + f[6] + f[7]));
+    case 2: return R(1) / (R(1) + uy) * (sy + R(2) * out2);
+    case 4: return R(1) / (R(1) - uy) * (sy + R(2) * out4);
+    case 1: return R(1) / (R(1) + ux) * (sx + R(2) * out1);
+    case 3: return R(1) / (R(1) - ux) * (sx + R(2) * out3);
   }
   return R(0);
 }

@@ -25,7 +25,7 @@ CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 
 class NeemRule(C.Structure):
@@ -117,6 +117,10 @@ def lib():
         "lbm_batch_run": ([BP, I32, I32, I32, P], C.c_int),
         "lbm_batch_finish": ([BP, I32, I32, P], C.c_int),
         "lbm_batch_graph": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p)], C.c_int),
+        "lbm_batch_mode": ([BP], C.c_int),
+        "lbm_batch_reserve2": ([BP, I32], C.c_int),
+        "lbm_batch_run_blocked": ([BP, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
+        "lbm_batch_graph_blocked": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)], C.c_int),
         "lbm_graph_launch": ([P, P], C.c_int),
         "lbm_graph_destroy": ([P], C.c_int),
         "lbm_batch_residual": ([BP, I32, D, P], C.c_int),
@@ -139,7 +143,9 @@ def exported_symbols():
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
-            "lbm_batch_reserve", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph", "lbm_graph_launch",
+            "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
+            "lbm_batch_mode",
+            "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
 
 

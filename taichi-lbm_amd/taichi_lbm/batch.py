@@ -78,6 +78,13 @@ def footprint_bytes(spec_or_n, storage="f32") -> int:
     return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + r)
 
 
+def two_step_tiles(specs) -> int:
+    """Tiles of the two-step launch for these lattices (13 x 64 tiles, or
+    11 x 64 when any lattice has node rules)."""
+    ti = 11 if any(len(s.neem) for s in specs) else 13
+    return sum(-(-s.NX // ti) * -(-s.NY // 64) for s in specs)
+
+
 class _Arena:
     """First-fit allocator over node offsets (units of 64 nodes)."""
 
@@ -114,7 +121,7 @@ class BatchRunner:
     dynamic use (``add`` / ``remove``)."""
 
     def __init__(self, specs: Sequence[LatticeSpec] = (), storage: str = "f32", grid_blocks: int = 0,
-                 capacity_nodes: int = 0, max_slots: int = 0):
+                 capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         t = torch()
@@ -122,6 +129,7 @@ class BatchRunner:
         specs = list(specs)
         self.storage = storage
         self.grid = int(grid_blocks)
+        self.blocked = bool(blocked)    # two steps per launch where the batch allows it
         cap = int(capacity_nodes) or sum(plane_stride(s.N) for s in specs)
         cap = max(64, (cap + 63) // 64 * 64)
         self.capacity = cap
@@ -154,6 +162,8 @@ class BatchRunner:
         for l in range(n):
             H.check(H.lib().lbm_batch_set(h, l, 0, -1, None), "lbm_batch_set")
         H.check(H.lib().lbm_batch_reserve(h, cap // 256 + n + 8), "lbm_batch_reserve")
+        if self.blocked and specs:
+            H.check(H.lib().lbm_batch_reserve2(h, two_step_tiles(specs) + 64), "lbm_batch_reserve2")
         self.slots: Dict[int, dict] = {}
         self.parity = 0
         self.finished = False
@@ -278,29 +288,48 @@ class BatchRunner:
             self._pending_local.clear()
             self._dirty = True
 
+    @property
+    def mode(self) -> int:
+        """0: one step per launch; 1 / 2: two-step launches with that ring width."""
+        self.commit()
+        return H.lib().lbm_batch_mode(self.h) if self.blocked else 0
+
     def run(self, nsteps: int):
         if nsteps <= 0:
             return
         self.commit()
-        H.check(H.lib().lbm_batch_run(self.h, int(nsteps), self.parity, self.grid, stream()), "lbm_batch_run")
-        self.parity ^= (int(nsteps) & 1)
+        if self.blocked:
+            out = ctypes.c_int32(-1)
+            H.check(H.lib().lbm_batch_run_blocked(self.h, int(nsteps), self.parity, self.grid, ctypes.byref(out),
+                                                  stream()), "lbm_batch_run_blocked")
+            self.parity = int(out.value)
+        else:
+            H.check(H.lib().lbm_batch_run(self.h, int(nsteps), self.parity, self.grid, stream()), "lbm_batch_run")
+            self.parity ^= (int(nsteps) & 1)
         self.finished = False
         self._after_launches()
 
     def capture(self, nsteps: int) -> "BatchGraph":
-        """A graph of `nsteps` batched steps from the current parity.  nsteps
-        must be even so every replay starts on the same parity; descriptor and
+        """A graph of `nsteps` batched steps from the current parity.  Every
+        replay must start on the same parity: nsteps even with one step per
+        launch, a multiple of 4 with two-step launches.  Descriptor and
         tile-list changes committed between replays are picked up."""
-        if nsteps % 2:
-            raise ValueError("capture an even number of steps")
         s = stream()
         if s == 0:
             raise RuntimeError("capture needs a non-default stream (torch.cuda.stream(...))")
-        self.commit()
+        mode = self.mode
+        if nsteps % (4 if mode else 2):
+            raise ValueError(f"capture a multiple of {4 if mode else 2} steps")
         g = ctypes.c_void_p()
-        H.check(H.lib().lbm_batch_graph(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g)),
-                "lbm_batch_graph")
-        return BatchGraph(self, g, int(nsteps))
+        if self.blocked:
+            out = ctypes.c_int32(-1)
+            H.check(H.lib().lbm_batch_graph_blocked(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g),
+                                                    ctypes.byref(out)), "lbm_batch_graph_blocked")
+            assert int(out.value) == self.parity
+        else:
+            H.check(H.lib().lbm_batch_graph(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g)),
+                    "lbm_batch_graph")
+        return BatchGraph(self, g, int(nsteps), mode)
 
     def finish(self):
         """Boundary pass for every active lattice (f, rho, vel as the reference reads them)."""
@@ -361,14 +390,17 @@ class BatchRunner:
 class BatchGraph:
     """An instantiated graph of `nsteps` batched steps."""
 
-    def __init__(self, runner: BatchRunner, g, nsteps: int):
-        self.runner, self.g, self.nsteps = runner, g, nsteps
+    def __init__(self, runner: BatchRunner, g, nsteps: int, mode: int = 0):
+        self.runner, self.g, self.nsteps, self.mode = runner, g, nsteps, mode
         self.parity = runner.parity
 
     def replay(self):
         r = self.runner
         if r.parity != self.parity:
             raise RuntimeError("the batch parity moved since capture (an odd eager run in between)")
+        if r.mode != self.mode:
+            raise RuntimeError("the batch's launch mode changed since capture (power law or node rules "
+                               "admitted); capture a new graph")
         r.commit()
         H.check(H.lib().lbm_graph_launch(self.g, stream()), "lbm_graph_launch")
         r.finished = False

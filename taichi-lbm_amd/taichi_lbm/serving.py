@@ -82,8 +82,10 @@ class SolveServer:
         # arena sized from the budget: bytes per node of a resident lattice
         per_node = footprint_bytes(64, storage) // 64
         self.capacity_nodes = self.budget // per_node // 64 * 64
+        # one step per launch: requests come and go after the graph is
+        # captured, and the two-step tiling of future lattices is unknown
         self.runner = BatchRunner([], storage=storage, grid_blocks=grid_blocks,
-                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots)
+                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=False)
         self.tick_steps = int(tick_steps)
         self.keep_results = keep_results
         self.graph = None

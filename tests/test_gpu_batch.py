@@ -66,12 +66,12 @@ def test_graph_replay_equals_run():
     with torch.cuda.stream(s):
         a.run(2)
         b.run(2)
-        g = b.capture(6)
+        g = b.capture(8)
         for rep in range(4):
             g.replay()
             torch.cuda.synchronize()          # eager work between replays
             _ = b.residuals()
-        a.run(24)
+        a.run(32)
         torch.cuda.synchronize()
         for q in range(len(cs)):
             fa, fb = a.fields(q), b.fields(q)
@@ -146,3 +146,48 @@ def test_autotune_picks_a_candidate():
     r.run(1)
     best, timings = autotune_grid(r, candidates=(2, 8), steps=4)
     assert best in timings and best % 8 == 0
+
+
+def test_blocked_batch_equals_one_step_batch():
+    """Two-step batched launches (ring width 2: the channels have node rules)
+    give the fields of one-step batched launches, bit for bit."""
+    cs = _mix()
+    a = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=True)
+    b = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=False)
+    assert a.mode == 2 and b.mode == 0
+    for n in (7, 10):
+        a.run(n)
+        b.run(n)
+    for q in range(len(cs)):
+        fa, fb = a.fields(q), b.fields(q)
+        for name in ("f", "f2", "rho", "vel", "total_rho"):
+            assert np.array_equal(fa[name], fb[name]), (q, name)
+
+
+def test_blocked_batch_without_rules_and_graph():
+    """Lattices without node rules (ring width 1), run eagerly and replayed
+    from a graph, against one-step launches."""
+    import torch
+    import contextlib, io
+    with contextlib.redirect_stdout(io.StringIO()):
+        cs = [cases.porous(NX=90 + 13 * q, NY=70 + 20 * q, seed=q, collision=("bgk", "trt", "mrt")[q % 3],
+                           rmin=3.0, rmax=6.0, g=(1e-5, 0.0)) for q in range(4)]
+    specs = [spec_from_case(c, power_law=False) for c in cs]
+    a = BatchRunner(specs, storage="f32", blocked=True)
+    b = BatchRunner(specs, storage="f32", blocked=False)
+    assert a.mode == 1
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        a.run(2)
+        b.run(2)
+        g = a.capture(12)
+        for _ in range(3):
+            g.replay()
+            torch.cuda.synchronize()
+        b.run(36)
+        torch.cuda.synchronize()
+        for q in range(len(cs)):
+            fa, fb = a.fields(q), b.fields(q)
+            for name in ("f", "f2", "rho", "vel"):
+                assert np.array_equal(fa[name], fb[name]), (q, name)
+        g.close()

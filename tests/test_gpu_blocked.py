@@ -1,8 +1,8 @@
 """Two steps per launch (lbm_run_blocked) against one step per launch
 (lbm_run): bit-identical populations, macroscopic fields and relaxation state,
 over odd and even step counts, restarts after a readback, tiles cut by the
-lattice edges, lattices smaller than one tile, and the fallback for lattices
-with node rules."""
+lattice edges, lattices smaller than one tile, and node rules (two-node
+ring)."""
 import contextlib
 import io
 
@@ -67,9 +67,22 @@ def test_lattice_smaller_than_a_tile():
     _same(a, b, ("f", "f2", "rho", "vel"))
 
 
-def test_node_rules_fall_back_to_one_step():
-    build = lambda: cases.cylinder(NX=64, NY=32, cx=16, cy=15, r=5, collision="trt", inlet="bb")
+@pytest.mark.parametrize("inlet", ["bb", "neem"])
+def test_node_rules_two_steps(inlet):
+    """Node rules (VelocityBB / VelocityNEEM inlet, PressureNEEM outlet) in
+    the two-step kernel with a two-node ring; 23 rows so the outlet row is
+    the first row of an 11-row tile and its interior neighbour lies in the
+    ring."""
+    build = lambda: cases.cylinder(NX=23, NY=70, cx=9, cy=35, r=4, collision="trt", inlet=inlet)
     a, b, sa, sb = _pair(build, False)
-    sa.run(20)
-    sb.run(20)
+    sa.run(21)
+    sb.run(21)
+    _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
+
+
+def test_reference_cylinder_rules_two_steps():
+    build = lambda: cases.cylinder_reference_bcs(NX=58, NY=40, collision="mrt", tau_bulk=0.9)
+    a, b, sa, sb = _pair(build, False)
+    sa.run(16)
+    sb.run(16)
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
