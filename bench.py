@@ -108,9 +108,10 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     """512 independent cylinder channels (NX ~ U{128..1024}, NY ~ U{64..256},
     r ~ U[4, NY/8] at (max(NX/5, r + 4), NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
     BGK or TRT (magic 1/4) by a seeded coin; bounce-back walls, VelocityBB
-    inlet, PressureNEEM outlet), FP32: one persistent launch per step
-    replayed from a graph, against the plain path (one lbm_run launch per
-    lattice per step).  Afterwards every lattice is run to solve_steps in the
+    inlet, PressureNEEM outlet), FP32: batched launches (two steps per launch
+    over the tiles of every lattice) replayed from a graph, against the plain
+    path (one lbm_run launch per lattice per step) and, for reference, the
+    plain path with two steps per launch (lbm_run_blocked per lattice).  Afterwards every lattice is run to solve_steps in the
     batch and the non-finite ones are counted; a sample is rerun in FP64 and
     the largest velocity difference reported (the plain and batched paths
     give bit-identical fields, so one number describes both)."""
@@ -138,8 +139,13 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     with torch.cuda.stream(stream):
         runner = BatchRunner(specs, storage="f32")
         runner.run(2)
-        grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
-        log(f"autotuned grid {grid}: {timings}")
+        mode = runner.mode
+        if mode == 0:          # the persistent one-step grid is tuned; two-step launches have a block per tile
+            grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
+            log(f"autotuned grid {grid}: {timings}")
+        else:
+            grid = 0
+            runner.run(10)
         g = runner.capture(graph_steps)
         g.replay()
         torch.cuda.synchronize()
@@ -169,6 +175,17 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         e1.record(stream)
         torch.cuda.synchronize()
         ms_p = e0.elapsed_time(e1) / plain_steps
+        # the same, each lattice with two steps per launch (lbm_run_blocked)
+        out = ctypes.c_int32()
+        e0.record(stream)
+        for l in slots:
+            H.check(lib.lbm_run_blocked(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
+                                        ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
+                                                    ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, None,
+                                        ctypes.byref(out), s))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_p2 = e0.elapsed_time(e1) / plain_steps
         g.close()
         runner.close()
         # accuracy at solve_steps: a fresh batch run to the end of the solve
@@ -189,7 +206,9 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             "mlups_batched": round(fluid / (ms_b * 1e-3) / 1e6, 1),
             "hbm_frac_batched": round(fluid * 72 / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "solves_per_s_batched": round(sps_b, 3), "solves_per_s_plain": round(sps_p, 3),
-            "speedup": round(sps_b / sps_p, 2), "grid_blocks": grid, "graph_steps": graph_steps,
+            "speedup": round(sps_b / sps_p, 2), "launch_mode": mode,
+            "ms_per_step_plain_two_step": round(ms_p2, 4),
+            "speedup_vs_plain_two_step": round(ms_p2 / ms_b, 2), "grid_blocks": grid, "graph_steps": graph_steps,
             "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps,
             "finite_after_solve": int(sum(finite)), "fp64_sample": len(sample),
             "max_abs_u_err_vs_fp64": err64}
