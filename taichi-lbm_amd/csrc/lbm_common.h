@@ -49,6 +49,19 @@ __device__ __forceinline__ R fforce_k(int k, R Fx, R Fy, R ux, R uy, R uF) {
   return wgt<R>(k) * (R(3) * cF + R(9) * cF * cu - R(3) * uF);
 }
 
+// 1/x: the hardware reciprocal (1 ulp) in FP32, exact division in FP64 (the
+// high-precision path).  The hot path divides by a handful of per-node
+// quantities (rho, tau, tau - 1/2); an IEEE-exact FP32 division costs about
+// ten instructions, the reciprocal one.
+template <typename R>
+__device__ __forceinline__ R rcp(R x) {
+  if constexpr (sizeof(R) == 4) {
+    return __builtin_amdgcn_rcpf(x);
+  } else {
+    return R(1) / x;
+  }
+}
+
 // Collision.tau_max_min, CollisionEngine.py:28-34.
 template <typename R>
 __device__ __forceinline__ R clamp_tau(R t, R tmin, R tmax) {

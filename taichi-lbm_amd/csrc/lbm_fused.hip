@@ -327,9 +327,10 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
 
   // S3 force_density (lbm.py:215-220)
   const bool force = a.force || a.sc != LBM_SC_NONE;
+  const R inv_rtot = rcp(o.rtot);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const R frac = o.rho[c] / o.rtot;
+    const R frac = o.rho[c] * inv_rtot;
     o.Fx[c] = o.scx[c] + frac * (R)a.gx;
     o.Fy[c] = o.scy[c] + frac * (R)a.gy;
   }
@@ -348,8 +349,8 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
     vx += tx;
     vy += ty;
   }
-  o.ux = vx / o.rtot;
-  o.uy = vy / o.rtot;
+  o.ux = vx * inv_rtot;
+  o.uy = vy * inv_rtot;
 
   // S5 power law + S6 collide, per component.  COLL < 0: the collision model
   // is read from the descriptor (batches mixing models share one kernel; the
@@ -368,7 +369,7 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
         s_shear = state;
       } else {
         tau = state;
-        if (is_trt) tau_anti = (R)a.magic[c] / (state - R(0.5)) + R(0.5);
+        if (is_trt) tau_anti = (R)a.magic[c] * rcp(state - R(0.5)) + R(0.5);
       }
     } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
       const R state = state_in[c];
@@ -378,12 +379,12 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
       const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
       const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
       if (is_mrt) {
-        s_shear = R(1) / t;
+        s_shear = rcp(t);
         o.state[c] = s_shear;
       } else {
         tau = t;
         o.state[c] = t;
-        if (is_trt) tau_anti = (R)a.magic[c] / (t - R(0.5)) + R(0.5);
+        if (is_trt) tau_anti = (R)a.magic[c] * rcp(t - R(0.5)) + R(0.5);
       }
     }
     if constexpr (COLL >= 0) {

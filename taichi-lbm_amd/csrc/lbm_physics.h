@@ -87,7 +87,7 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
                                         R tau, R tau_anti, R s_shear, R s_bulk) {
   const R uv = ux * ux + uy * uy;
   if constexpr (COLL == LBM_BGK) {
-    const R omega = R(1.0) / tau;
+    const R omega = rcp(tau);
     const R uF = ux * Fx + uy * Fy;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
       out[k] = v;
     }
   } else if constexpr (COLL == LBM_TRT) {
-    const R ws = R(1.0) / tau, wa = R(1.0) / tau_anti;
+    const R ws = rcp(tau), wa = rcp(tau_anti);
     const R uF = ux * Fx + uy * Fy;
     R fe[9], ff[9];
 #pragma unroll
@@ -176,14 +176,14 @@ __device__ __forceinline__ R shear_rate_of(const R (&f)[9], R rho, R ux, R uy, R
     const R m1 = e - rho * (-R(2) + R(3) * uv);
     const R m7 = pxx - rho * (ux * ux - uy * uy);
     const R m8 = pxy - rho * (ux * uy);
-    const R cof = -R(1.5) / rho;
+    const R cof = -R(1.5) * rcp(rho);
     const R a = s_bulk * m1 * (R(1) / R(6));
     const R b = tau_state * m7 * R(0.5);
     Sxx = (a + b) * cof;
     Syy = (a - b) * cof;
     Sxy = tau_state * m8 * cof;
   } else {
-    const R cof = -R(1.5) / rho / tau_state;
+    const R cof = -R(1.5) * rcp(rho * tau_state);
     const R pxx = f[1] + f[3] + s58;     // sum_k cx^2 f_k
     const R pyy = f[2] + f[4] + s58;     // sum_k cy^2 f_k
     Sxx = (pxx - rho * (R(1) / R(3) + ux * ux)) * cof;

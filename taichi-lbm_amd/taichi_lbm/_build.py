@@ -25,7 +25,10 @@ ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-    "-ffp-contract=fast",
+    # multiply-adds fuse within one source expression only: the per-node
+    # arithmetic shared by the one-step, two-step and batched kernels then
+    # rounds the same wherever it is inlined (their fields are bit-identical)
+    "-ffp-contract=on",
     # packed fp32 ops off: hipcc must not emit v_pk_{add,mul,fma}_f32
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
