@@ -376,7 +376,7 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
       const R g = is_mrt ? shear_rate_of<LBM_MRT, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk)
                          : shear_rate_of<LBM_BGK, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
       o.gam[c] = g;
-      const R nu = (R)a.visc_m * pow(g, (R)a.visc_nm1);
+      const R nu = (R)a.visc_m * pow_shear(g, (R)a.visc_nm1);
       const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
       if (is_mrt) {
         s_shear = rcp(t);
