@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 6
+#define LBM_ABI_VERSION 7
 
 /* status codes */
 #define LBM_OK 0
@@ -61,6 +61,9 @@ enum { LBM_SC_NONE = 0, LBM_SC_C1 = 1, LBM_SC_C2 = 2 };
 #define LBM_CODE_KEEP_MACRO 0x0200u  /* rho/u of this node are read by a node rule      */
 #define LBM_CODE_NEEM_SHIFT 10       /* bits 10..13: node rule index + 1 (0 = none)     */
 #define LBM_CODE_NEEM_MASK 0x3C00u
+#define LBM_CODE_PULLED 0x4000u      /* node outside the fluid group whose populations  *
+                                      * an active node pulls (a plain, not bounced,    *
+                                      * population reads its unchanged f2)             */
 
 /* Node-rule kinds: the boundary conditions that act on a node after its
  * populations were streamed or bounced.                                  */

@@ -84,6 +84,17 @@ __device__ __forceinline__ R clamp_tau(R t, R tmin, R tmax) {
   return t;
 }
 
+// ---- buffer addressing ------------------------------------------------------
+// A raw buffer resource over a range of fewer than 2^31 bytes: loads and
+// stores then take a 32-bit per-lane offset plus a wave-uniform (scalar)
+// offset instead of a 64-bit address per lane, and accesses past the range
+// read 0 / are dropped by the hardware.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kNonTemporal = 2;   // cache-policy bits of a streaming store
+
 // ---- population storage ---------------------------------------------------
 // Pop<S>::T is the stored type, Pop<S>::R the arithmetic type.  Shifted
 // formats store g = f - w_k so that the stored magnitude is the deviation
@@ -95,6 +106,13 @@ template <> struct Pop<LBM_STORE_F32> {
   static __device__ __forceinline__ R ld(const T* p, uint32_t i, int) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, int, R v) { __builtin_nontemporal_store(v, p + i); }
   static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int, R v) { p[i] = v; }
+  // byte offsets: voff per lane, soff wave-uniform
+  static __device__ __forceinline__ R ldb(Rsrc r, uint32_t voff, uint32_t soff, int) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+  }
+  static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int, R v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, kNonTemporal);
+  }
 };
 template <> struct Pop<LBM_STORE_F64> {
   using T = double;
@@ -111,6 +129,14 @@ template <> struct Pop<LBM_STORE_F16_SHIFT> {
     __builtin_nontemporal_store((T)(v - wgt<float>(k)), p + i);
   }
   static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, R v) { p[i] = (T)(v - wgt<float>(k)); }
+  static __device__ __forceinline__ R ldb(Rsrc r, uint32_t voff, uint32_t soff, int k) {
+    const uint16_t b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+    return (float)__builtin_bit_cast(_Float16, b) + wgt<float>(k);
+  }
+  static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int k, R v) {
+    const uint16_t b = __builtin_bit_cast(uint16_t, (T)(v - wgt<float>(k)));
+    __builtin_amdgcn_raw_buffer_store_b16(b, r, (int)voff, (int)soff, kNonTemporal);
+  }
 };
 template <> struct Pop<LBM_STORE_BF16_SHIFT> {
   using T = uint16_t;  // raw bfloat16 bits
@@ -129,6 +155,13 @@ template <> struct Pop<LBM_STORE_BF16_SHIFT> {
     __builtin_nontemporal_store(enc(v - wgt<float>(k)), p + i);
   }
   static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int k, R v) { p[i] = enc(v - wgt<float>(k)); }
+  static __device__ __forceinline__ R ldb(Rsrc r, uint32_t voff, uint32_t soff, int k) {
+    const uint16_t b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+    return __uint_as_float(((uint32_t)b) << 16) + wgt<float>(k);
+  }
+  static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int k, R v) {
+    __builtin_amdgcn_raw_buffer_store_b16(enc(v - wgt<float>(k)), r, (int)voff, (int)soff, kNonTemporal);
+  }
 };
 
 // n / ny via a 64-bit multiply-high: mdiv = floor((2^64-1)/ny) + 1 is exact for

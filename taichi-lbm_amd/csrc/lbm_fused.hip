@@ -522,6 +522,18 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // (step_math, apply_rule) is shared with the one-step kernel, so both give
 // bit-identical fields.
 //
+// Phase B's pulls are plain LDS reads at fixed offsets.  Population k of node
+// x streams from slot x - c_k of plane k, and only x reads that slot of that
+// plane; so phase A stores there whatever x must receive:
+//   * the post-collision value of node x - c_k (its own write);
+//   * for a node outside the fluid group that an active node pulls
+//     (LBM_CODE_PULLED), its unchanged f2, read from HBM;
+//   * where x bounces population k, x's own post-collision population opp(k)
+//     (a "reflection", written after a barrier so that it wins over the
+//     first two).
+// The LDS tile carries one pad node around the ring, where the reflections
+// of ring-edge nodes land (no node of the tile reads them).
+//
 // Ring width: H = 1 streams; H = 2 when the lattice has node rules (a NEEM or
 // OpenNeumann node reads its interior neighbour's post-stream populations,
 // two nodes away).  Shan-Chen (psi needs its own pass) and FP64 storage run
@@ -536,48 +548,44 @@ struct TileShape {
   // tile rows chosen so the ring-extended tile fits one round of the block:
   // (13 + 2) x 66 = 990 and (11 + 4) x 68 = 1020 nodes
   static constexpr int TI = H == 1 ? 13 : 11;
-  static constexpr int RJ = kTJ + 2 * H;
-  static constexpr int RN = (TI + 2 * H) * RJ;
+  static constexpr int RI = TI + 2 * H;     // ring-extended rows
+  static constexpr int RJ = kTJ + 2 * H;    // ring-extended columns
+  static constexpr int RN = RI * RJ;
+  static constexpr int PJ = RJ + 2;         // LDS row pitch (one pad column each side)
+  static constexpr int PN = (RI + 2) * PJ;  // LDS slots (one pad row above and below)
   static_assert(RN <= kBlock2, "ring-extended tile larger than the block");
+  // LDS slot of ring node (rr, cc), rr < RI, cc < RJ
+  static __device__ __forceinline__ uint32_t slot(uint32_t rr, uint32_t cc) { return (rr + 1) * PJ + cc + 1; }
+  // slot offset of x - c_k from x
+  static constexpr int back(int k) { return cx(k) * PJ + cy(k); }
 };
 
 template <int NC, int S, int H>
 struct TileLds {
   using R = typename Pop<S>::R;
-  static constexpr int RN = TileShape<H>::RN;
-  R f[NC * 9][RN];                      // post-collision populations of step n
-  R mac[H == 2 ? NC + 3 : 1][RN];       // rho_c, total_rho, ux, uy of step n (node rules)
-  R st[NC][RN];                         // relaxation state after step n
-  uint16_t code[RN];
+  static constexpr int PN = TileShape<H>::PN;
+  R f[NC * 9][PN];                      // what each node's pull of step n+1 receives (see above)
+  R mac[H == 2 ? NC + 3 : 1][H == 2 ? PN : 1];   // rho_c, total_rho, ux, uy of step n (node rules)
+  R st[NC][PN];                         // relaxation state after step n
+  uint16_t code[PN];
 };
 
-// Step n as phase B reads it: the LDS tile (nodes addressed by their index r
-// in the ring-extended tile, plus global coordinates for the rare pull from a
-// node outside the fluid group, whose value is the stored one in HBM).
+// Step n as phase B reads it: the LDS tile, nodes addressed by their slot.
 template <int NC, int S, int H>
 struct LdsView {
-  using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
-  static constexpr int RJ = TileShape<H>::RJ;
   struct Node {
-    uint32_t r, gi, gj;
+    uint32_t q;
   };
   const TileLds<NC, S, H>* L;
-  const T* stale;
-  uint32_t ps, nx, ny;
   __device__ __forceinline__ Node back(const Node& x, int di, int dj) const {
-    return Node{(uint32_t)((int)x.r - di * RJ - dj), (uint32_t)((int)(x.gi + nx) - di) % nx,
-                (uint32_t)((int)(x.gj + ny) - dj) % ny};
+    return Node{(uint32_t)((int)x.q - di * TileShape<H>::PJ - dj)};
   }
-  __device__ __forceinline__ uint32_t code_of(const Node& x) const { return L->code[x.r]; }
-  __device__ __forceinline__ R macro(const Node& x, int plane) const { return L->mac[plane][x.r]; }
-  __device__ __forceinline__ R pop(const Node& x, uint32_t bm, int c, int k) const {
-    const bool bounce = (k > 0) && ((bm >> (k - 1)) & 1u);
-    if (bounce) return L->f[c * 9 + opp(k)][x.r];
-    const uint32_t s1 = (uint32_t)((int)x.r - cx(k) * RJ - cy(k));
-    if (L->code[s1] & LBM_CODE_ACTIVE) return L->f[c * 9 + k][s1];
-    const Nbr nb(x.gi, x.gj, nx, ny);
-    return Pop<S>::ld(stale + (size_t)(c * 9 + k) * ps, nb.src(k), k);
+  __device__ __forceinline__ uint32_t code_of(const Node& x) const { return L->code[x.q]; }
+  __device__ __forceinline__ R macro(const Node& x, int plane) const { return L->mac[plane][x.q]; }
+  // post-stream population k (bounces were resolved by phase A's reflections)
+  __device__ __forceinline__ R pop(const Node& x, uint32_t, int c, int k) const {
+    return L->f[c * 9 + k][x.q - TileShape<H>::back(k)];
   }
 };
 
@@ -592,75 +600,169 @@ struct Launch2 {
   void* tau_out;              // single lattice, power law: state after step n+1
 };
 
+// Phase A inputs of one ring node: gathered populations and relaxation state.
+template <int NC, int S>
+struct RingIn {
+  using R = typename Pop<S>::R;
+  R f[NC][9];
+  R st[NC];
+};
+
+// Ring node (rr, cc) of the tile whose first row / column is (i0, j0): global
+// coordinates, wrapped periodically.
+template <int H>
+__device__ __forceinline__ void ring_coords(uint32_t rr, uint32_t cc, uint32_t i0, uint32_t j0, uint32_t nx,
+                                            uint32_t ny, uint32_t& gi, uint32_t& gj) {
+  gi = i0 + rr + nx - H;
+  gj = j0 + cc + ny - H;
+  if (gi >= nx) gi -= nx;
+  if (gj >= ny) gj -= ny;
+  // lattices smaller than a tile wrap more than once
+  if (gi >= nx) gi %= nx;
+  if (gj >= ny) gj %= ny;
+}
+
+// Pull gather of one node with buffer addressing (the population buffer spans
+// fewer than 2^31 bytes; nx, ny < 2^24).  Byte offsets: plane p at node m is
+// p ps B + m B.  Population k pulls node x - c_k of plane k, or, bounced,
+// node x of plane opp(k); both are addressed from the lower of the two planes
+// (a scalar offset), the per-lane part selected by the bounce bit.
+template <int NC, int S>
+__device__ __forceinline__ void gather_buf(const Rsrc& rs, uint32_t ps, uint32_t nx, uint32_t ny, uint32_t gi,
+                                           uint32_t gj, uint32_t code, typename Pop<S>::R (&f)[NC][9]) {
+  constexpr uint32_t B = sizeof(typename Pop<S>::T);
+  const uint32_t two = 2u * ps * B;   // planes k and opp(k) are two apart
+  const uint32_t rc = __umul24(gi, ny);
+  const uint32_t rm = gi == 0 ? rc + (nx - 1u) * ny : rc - ny;
+  const uint32_t rp = gi + 1u == nx ? 0u : rc + ny;
+  const uint32_t jm = gj == 0 ? ny - 1u : gj - 1u;
+  const uint32_t jp = gj + 1u == ny ? 0u : gj + 1u;
+  const uint32_t nB = (rc + gj) * B, nB2 = nB + two;
+  const uint32_t row[3] = {rc * B, rm * B, rp * B};   // source row of cx = 0, +1, -1
+  const uint32_t col[3] = {gj * B, jm * B, jp * B};   // source column of cy = 0, +1, -1
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    f[c][0] = Pop<S>::ldb(rs, nB, (uint32_t)(c * 9) * ps * B, 0);
+#pragma unroll
+    for (int k = 1; k < 9; ++k) {
+      const bool low = opp(k) > k;   // k = 1, 2, 5, 6: plane k is the lower one
+      const uint32_t src = row[cx(k) == 0 ? 0 : (cx(k) == 1 ? 1 : 2)] + col[cy(k) == 0 ? 0 : (cy(k) == 1 ? 1 : 2)] +
+                           (low ? 0u : two);
+      const uint32_t own = low ? nB2 : nB;
+      const uint32_t m = (uint32_t)((int32_t)(code << (32 - k)) >> 31);   // all ones where bounced
+      const uint32_t off = (m & own) | (~m & src);
+      f[c][k] = Pop<S>::ldb(rs, off, (uint32_t)(c * 9 + (low ? k : opp(k))) * ps * B, k);
+    }
+  }
+}
+
+// Phase A loads of one active ring node.  BUF: buffer addressing (32-bit
+// offsets; see gather_buf); otherwise 64-bit addresses.
+template <int NC, int S, bool BUF>
+__device__ __forceinline__ void ring_load(const Lat& a, bool local, const typename Pop<S>::T* __restrict__ src,
+                                          const Rsrc& rsrc, const typename Pop<S>::R* tau_a, uint32_t gi,
+                                          uint32_t gj, uint32_t code, RingIn<NC, S>& in) {
+  using R = typename Pop<S>::R;
+  const uint32_t ps = a.ps, n = gi * a.ny + gj;
+  if (local) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int k = 0; k < 9; ++k) in.f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
+  } else if constexpr (BUF) {
+    gather_buf<NC, S>(rsrc, ps, a.nx, a.ny, gi, gj, code, in.f);
+  } else {
+    gather<NC, S>(src, ps, Nbr(gi, gj, a.nx, a.ny), code & LBM_CODE_BOUNCE_MASK, in.f);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) in.st[c] = a.tau_mode != LBM_TAU_CONST ? tau_a[(size_t)c * ps + n] : R(0);
+}
+
+// Phase A arithmetic of one active ring node: node rules (reading step n-1's
+// state in HBM), step n, results into LDS slot q.  The populations that the
+// node bounces are returned in `refl` for the reflection writes.
 template <int COLL, int NC, int S, int H>
-__device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
-                                               const void* tau_in, void* tau_out, TileLds<NC, S, H>& L) {
+__device__ __forceinline__ void ring_step(const Lat& a, bool local, const typename Pop<S>::T* __restrict__ src,
+                                          uint32_t parity, uint32_t q, uint32_t gi, uint32_t gj, uint32_t code,
+                                          RingIn<NC, S>& in, TileLds<NC, S, H>& L,
+                                          typename Pop<S>::R (&refl)[NC][9]) {
+  using R = typename Pop<S>::R;
+  const uint32_t n = gi * a.ny + gj;
+  if constexpr (H == 2) {
+    const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
+    if (e && !local) {
+      const GlobalView<S> v{src, (const R*)a.mac[parity], a.code, a.ps, a.nx, a.ny};
+      apply_rule<NC, S>(a, v, a.neem[e - 1], v.node(gi, gj), n, in.f, nullptr);
+    }
+  }
+  StepOut<NC, R> o;
+  step_math<COLL, NC, R>(a, 0, n, gi, gj, in.f, in.st, o);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      L.f[c * 9 + k][q] = o.f[c][k];
+      refl[c][k] = o.f[c][opp(k)];
+    }
+    L.st[c][q] = o.state[c];
+  }
+  if constexpr (H == 2) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) L.mac[c][q] = o.rho[c];
+    L.mac[NC][q] = o.rtot;
+    L.mac[NC + 1][q] = o.ux;
+    L.mac[NC + 2][q] = o.uy;
+  }
+}
+
+// Reflections of one active ring node (after the barrier that follows the
+// own writes): population k that the node bounces goes to slot q - c_k of
+// plane k, the slot its pull reads.
+template <int NC, int S, int H>
+__device__ __forceinline__ void ring_reflect(uint32_t q, uint32_t code, const typename Pop<S>::R (&refl)[NC][9],
+                                             TileLds<NC, S, H>& L) {
+#pragma unroll
+  for (int k = 1; k < 9; ++k) {
+    if (code & (1u << (k - 1))) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) L.f[c * 9 + k][q - TileShape<H>::back(k)] = refl[c][k];
+    }
+  }
+}
+
+// A node outside the fluid group that an active node pulls: its unchanged f2
+// (the stored value in HBM, as the one-step kernel reads it) into its slot.
+template <int NC, int S, int H>
+__device__ __forceinline__ void ring_pulled(const Lat& a, const typename Pop<S>::T* __restrict__ stale, uint32_t q,
+                                            uint32_t n, TileLds<NC, S, H>& L) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) L.f[c * 9 + k][q] = Pop<S>::ld(stale + (size_t)(c * 9 + k) * a.ps, n, k);
+}
+
+// Phase B: step n+1 on the tile from LDS; results to pop[parity^1] (and the
+// relaxation state to tau_out under the power law, macroscopic fields when
+// the run ends).
+template <int COLL, int NC, int S, int H, bool BUF>
+__device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q, uint32_t i0, uint32_t j0,
+                                                 void* tau_out, const TileLds<NC, S, H>& L) {
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
   using Shape = TileShape<H>;
-  constexpr int RJ = Shape::RJ, RN = Shape::RN, TI = Shape::TI;
-  const uint32_t ps = a.ps, p = q.parity, nx = a.nx, ny = a.ny;
+  const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
   const bool pl = a.tau_mode == LBM_TAU_POWER_LAW;
-  const R* tau_a = (const R*)(pl ? tau_in : a.tau);
-
-  // phase A: step n on the tile and its ring (wrapped periodically)
-  const T* __restrict__ src = (const T*)a.pop[p];
-  for (uint32_t r1 = threadIdx.x; r1 < (uint32_t)RN; r1 += kBlock2) {
-    const uint32_t rr = r1 / RJ, cc = r1 - rr * RJ;
-    const uint32_t gi = (i0 + rr + H * nx - H) % nx, gj = (j0 + cc + H * ny - H) % ny;
-    const uint32_t n = gi * ny + gj;
-    const uint32_t code = a.code[n];
-    L.code[r1] = (uint16_t)code;
-    if (!(code & LBM_CODE_ACTIVE)) continue;
-    R f[NC][9];
-    if (local) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
-    } else {
-      const Nbr nb(gi, gj, nx, ny);
-      gather<NC, S>(src, ps, nb, code & LBM_CODE_BOUNCE_MASK, f);
-      if constexpr (H == 2) {
-        const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
-        if (e) {
-          const GlobalView<S> v{src, (const R*)a.mac[p], a.code, ps, nx, ny};
-          apply_rule<NC, S>(a, v, a.neem[e - 1], v.node(gi, gj), n, f, nullptr);
-        }
-      }
-    }
-    R st_in[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? tau_a[(size_t)c * ps + n] : R(0);
-    StepOut<NC, R> o;
-    step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) L.f[c * 9 + k][r1] = o.f[c][k];
-      L.st[c][r1] = o.state[c];
-    }
-    if constexpr (H == 2) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) L.mac[c][r1] = o.rho[c];
-      L.mac[NC][r1] = o.rtot;
-      L.mac[NC + 1][r1] = o.ux;
-      L.mac[NC + 2][r1] = o.uy;
-    }
-  }
-  __syncthreads();
-
-  // phase B: step n+1 on the tile from LDS
-  T* __restrict__ dst = (T*)a.pop[p ^ 1u];
-  R* mo = (R*)a.mac[p ^ 1u];
-  const LdsView<NC, S, H> v{&L, dst, ps, nx, ny};
-  for (uint32_t t = threadIdx.x; t < (uint32_t)(TI * kTJ); t += kBlock2) {
+  T* __restrict__ dst = (T*)a.pop[q.parity ^ 1u];
+  const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+  R* mo = (R*)a.mac[q.parity ^ 1u];
+  const LdsView<NC, S, H> v{&L};
+  for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * kTJ); t += kBlock2) {
     const uint32_t rr = t / kTJ, cc = t - rr * kTJ;
     const uint32_t gi = i0 + rr, gj = j0 + cc;
     if (gi >= nx || gj >= ny) continue;
     const uint32_t n = gi * ny + gj;
-    const uint32_t r1 = (rr + H) * RJ + cc + H;
-    const uint32_t code = L.code[r1];
+    const typename LdsView<NC, S, H>::Node x{Shape::slot(rr + H, cc + H)};
+    const uint32_t code = L.code[x.q];
     if (!(code & LBM_CODE_ACTIVE)) {
       if (q.write_macro) {
 #pragma unroll
@@ -668,13 +770,11 @@ __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, b
       }
       continue;
     }
-    const typename LdsView<NC, S, H>::Node x{r1, gi, gj};
-    const uint32_t bm = code & LBM_CODE_BOUNCE_MASK;
     R f[NC][9];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int k = 0; k < 9; ++k) f[c][k] = v.pop(x, bm, c, k);
+      for (int k = 0; k < 9; ++k) f[c][k] = v.pop(x, 0u, c, k);
     if constexpr (H == 2) {
       const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
       if (e) apply_rule<NC, S>(a, v, a.neem[e - 1], x, n, f, nullptr);
@@ -682,27 +782,66 @@ __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, b
     R st_in[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      st_in[c] = pl ? L.st[c][r1] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
+      st_in[c] = pl ? L.st[c][x.q] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
     StepOut<NC, R> o;
     step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+      for (int k = 0; k < 9; ++k) {
+        if constexpr (BUF)
+          Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
+        else
+          Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+      }
       if (pl) ((R*)tau_out)[(size_t)c * ps + n] = o.state[c];
     }
     store_macro<NC, R>(a, mo, n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
   }
 }
 
-template <int COLL, int NC, int S, int H>
+template <int COLL, int NC, int S, int H, bool BUF>
+__device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
+                                               const void* tau_in, void* tau_out, TileLds<NC, S, H>& L) {
+  using T = typename Pop<S>::T;
+  using R = typename Pop<S>::R;
+  using Shape = TileShape<H>;
+  const T* __restrict__ src = (const T*)a.pop[q.parity];
+  const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * a.ps * (uint32_t)sizeof(T) : 0u);
+  const R* tau_a = (const R*)(a.tau_mode == LBM_TAU_POWER_LAW ? tau_in : a.tau);
+  const uint32_t r1 = threadIdx.x;
+  const uint32_t rr = r1 / (uint32_t)Shape::RJ, cc = r1 - rr * (uint32_t)Shape::RJ;
+  const uint32_t qs = Shape::slot(rr, cc);
+  uint32_t code = 0;
+  R refl[NC][9];
+  if (r1 < (uint32_t)Shape::RN) {
+    uint32_t gi, gj;
+    ring_coords<H>(rr, cc, i0, j0, a.nx, a.ny, gi, gj);
+    code = a.code[gi * a.ny + gj];
+    L.code[qs] = (uint16_t)code;
+    if (code & LBM_CODE_ACTIVE) {
+      RingIn<NC, S> in;
+      ring_load<NC, S, BUF>(a, local, src, rsrc, tau_a, gi, gj, code, in);
+      ring_step<COLL, NC, S, H>(a, local, src, q.parity, qs, gi, gj, code, in, L, refl);
+    } else if (code & LBM_CODE_PULLED) {
+      ring_pulled<NC, S, H>(a, (const T*)a.pop[q.parity ^ 1u], qs, gi * a.ny + gj, L);
+    }
+  }
+  __syncthreads();
+  if ((code & (LBM_CODE_ACTIVE | LBM_CODE_BOUNCE_MASK)) > LBM_CODE_ACTIVE) ring_reflect<NC, S, H>(qs, code, refl, L);
+  __syncthreads();
+  tile_second_step<COLL, NC, S, H, BUF>(a, q, i0, j0, tau_out, L);
+}
+
+template <int COLL, int NC, int S, int H, bool BUF>
 __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
   __shared__ TileLds<NC, S, H> L;
   const uint32_t nt = q.tiles_i * q.tiles_j;
   uint32_t b = blockIdx.x;
   if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
   const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  tile_two_steps<COLL, NC, S, H>(a, q, q.local_read != 0, ti * TileShape<H>::TI, tj * kTJ, q.tau_in, q.tau_out, L);
+  tile_two_steps<COLL, NC, S, H, BUF>(a, q, q.local_read != 0, ti * TileShape<H>::TI, tj * kTJ, q.tau_in,
+                                      q.tau_out, L);
 }
 
 // Batched two-step launch: one block per tile of many lattices (tile ->
@@ -712,7 +851,7 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
 // retirements).  Blocks are remapped so that each XCD takes one contiguous
 // range of the tile list.  Power-law lattices are not batched this way
 // (their state would need a second buffer per lattice).
-template <int NC, int S, int H>
+template <int NC, int S, int H, bool BUF>
 __global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
     const Lat* __restrict__ items, const uint32_t* __restrict__ tile_lat, const uint32_t* __restrict__ tile_idx,
     const uint32_t* __restrict__ tile_count, const Launch2 q) {
@@ -725,7 +864,8 @@ __global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8)))
   const Lat& a = items[l];
   const uint32_t tiles_j = (a.ny + kTJ - 1) / kTJ;
   const uint32_t ti = t / tiles_j, tj = t - ti * tiles_j;
-  tile_two_steps<-1, NC, S, H>(a, q, a.start_step == q.step, ti * TileShape<H>::TI, tj * kTJ, a.tau, nullptr, L);
+  tile_two_steps<-1, NC, S, H, BUF>(a, q, a.start_step == q.step, ti * TileShape<H>::TI, tj * kTJ, a.tau, nullptr,
+                                    L);
 }
 
 // Persistent batched step over the 256-node tiles of many lattices.  Each XCD
@@ -1026,30 +1166,43 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
 namespace lbm {
 using Kernel2Fn = void (*)(const Lat, const Launch2);
 
-template <int NC, int S, int H>
+template <int NC, int S, int H, bool BUF>
 Kernel2Fn pick2_coll(int coll) {
   switch (coll) {
-    case LBM_BGK: return k_fused2<LBM_BGK, NC, S, H>;
-    case LBM_TRT: return k_fused2<LBM_TRT, NC, S, H>;
-    default: return k_fused2<LBM_MRT, NC, S, H>;
+    case LBM_BGK: return k_fused2<LBM_BGK, NC, S, H, BUF>;
+    case LBM_TRT: return k_fused2<LBM_TRT, NC, S, H, BUF>;
+    default: return k_fused2<LBM_MRT, NC, S, H, BUF>;
   }
 }
-template <int H>
+template <int H, bool BUF>
 Kernel2Fn pick2_h(int nc, int store, int coll) {
   switch (store) {
-    case LBM_STORE_F32: return nc == 1 ? pick2_coll<1, LBM_STORE_F32, H>(coll) : pick2_coll<2, LBM_STORE_F32, H>(coll);
+    case LBM_STORE_F32:
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F32, H, BUF>(coll) : pick2_coll<2, LBM_STORE_F32, H, BUF>(coll);
     case LBM_STORE_F16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT, H>(coll) : pick2_coll<2, LBM_STORE_F16_SHIFT, H>(coll);
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT, H, BUF>(coll)
+                     : pick2_coll<2, LBM_STORE_F16_SHIFT, H, BUF>(coll);
     case LBM_STORE_BF16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT, H>(coll) : pick2_coll<2, LBM_STORE_BF16_SHIFT, H>(coll);
+      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT, H, BUF>(coll)
+                     : pick2_coll<2, LBM_STORE_BF16_SHIFT, H, BUF>(coll);
     default: return nullptr;
   }
+}
+// buffer addressing (32-bit offsets, 24-bit row products) when the
+// population buffer spans fewer than 2^31 bytes and both sides are below 2^24
+bool fits_buffer(uint64_t nc, uint64_t nx, uint64_t ny, uint64_t ps, int storage) {
+  const uint64_t t = storage == LBM_STORE_F32 ? 4 : (storage == LBM_STORE_F64 ? 8 : 2);
+  return nc * 9ull * ps * t < (1ull << 31) && nx < (1ull << 24) && ny < (1ull << 24);
+}
+bool fits_buffer(const lbm_lattice* lat) {
+  return fits_buffer(lat->nc, lat->nx, lat->ny, lat->plane_stride, lat->storage);
 }
 // nullptr where the two-step kernel is not built: FP64 storage (the
 // high-precision reference path; in a 1024-thread block its registers would
 // spill) runs one step per launch.
-Kernel2Fn pick2(int nc, int store, int coll, int h) {
-  return h == 1 ? pick2_h<1>(nc, store, coll) : pick2_h<2>(nc, store, coll);
+Kernel2Fn pick2(int nc, int store, int coll, int h, bool buf) {
+  if (buf) return h == 1 ? pick2_h<1, true>(nc, store, coll) : pick2_h<2, true>(nc, store, coll);
+  return h == 1 ? pick2_h<1, false>(nc, store, coll) : pick2_h<2, false>(nc, store, coll);
 }
 int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
 uint32_t tile_rows(int h) { return h == 1 ? (uint32_t)TileShape<1>::TI : (uint32_t)TileShape<2>::TI; }
@@ -1067,7 +1220,7 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   hipStream_t s = (hipStream_t)stream;
   const bool pl = lat->tau_mode == LBM_TAU_POWER_LAW;
   const int h = ring_width(lat);
-  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h);
+  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, fits_buffer(lat));
   const bool ok2 = two && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
   if (!ok2) {
     rc = lbm_run(lat, fld, neem, nsteps, parity, first_local, stream);
@@ -1213,6 +1366,7 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
     const Lat& a = b->host[l];
     if (!a.active) continue;
     if (a.tau_mode == LBM_TAU_POWER_LAW) ok2 = false;
+    if (!fits_buffer(b->nc, a.nx, a.ny, a.ps, b->storage)) ok2 = false;   // buffer addressing
     if (a.n_neem > 0) h = 2;
   }
   b->host_t2lat.clear();
@@ -1349,7 +1503,9 @@ namespace {
 using Batch2Fn = void (*)(const lbm::Lat*, const uint32_t*, const uint32_t*, const uint32_t*, const lbm::Launch2);
 template <int NC, int S>
 Batch2Fn pick_batch2_s(int h) {
-  return h == 1 ? lbm::k_batch2<NC, S, 1> : lbm::k_batch2<NC, S, 2>;
+  // batched lattices are small: buffer addressing (each lattice's segment is
+  // checked against 2^31 bytes at commit)
+  return h == 1 ? lbm::k_batch2<NC, S, 1, true> : lbm::k_batch2<NC, S, 2, true>;
 }
 Batch2Fn pick_batch2(int nc, int store, int h) {
   switch (store) {

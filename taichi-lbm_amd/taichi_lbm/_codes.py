@@ -185,6 +185,16 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     for k in range(1, 9):
         bits |= (final[:, k] == BOUNCE).astype(np.uint32) << (k - 1)
     code[active] = (bits[active] | H.CODE_ACTIVE).astype(np.uint16)
+    # nodes outside the group that an active node pulls a population from
+    # (its unchanged f2): the two-step kernel stages them in LDS
+    pulled = np.zeros(N, bool)
+    I, J = act // NY, act % NY
+    for k in range(1, 9):
+        plain = final[act, k] == PULL
+        src = ((I[plain] - C[k, 0]) % NX) * NY + (J[plain] - C[k, 1]) % NY
+        pulled[src] = True
+    pulled &= ~active
+    code[pulled] |= np.uint16(H.CODE_PULLED)
     for e, (flat, nbf) in enumerate(neem_nodes):
         kind = neem_rules[e][0]
         code[flat] |= np.uint16((e + 1) << H.CODE_NEEM_SHIFT)

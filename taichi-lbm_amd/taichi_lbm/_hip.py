@@ -23,9 +23,10 @@ CODE_ACTIVE = 0x0100
 CODE_KEEP_MACRO = 0x0200
 CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
+CODE_PULLED = 0x4000      # inactive node pulled by an active one (plain pull)
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 
 class NeemRule(C.Structure):

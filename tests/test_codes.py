@@ -89,7 +89,6 @@ def _replay(NX, NY, mask, bcs_oracle, active):
     O.neighbor_classify(lat)
     tags = np.arange(NX * NY * 9, dtype=np.float64).reshape(NX, NY, 1, 9) + 1.0
     lat.f2[...] = tags
-    lat.f2[mask == -1] = 0.0
     lat.f[...] = np.nan
     O.apply_bcs(lat, bcs_oracle)
     return lat.f[..., 0, :], lat
@@ -107,7 +106,7 @@ def _expected_from_codes(code, NX, NY, mask):
                     src = (i, j, O.OPP[k])
                 else:
                     src = ((i - O.C[k, 0]) % NX, (j - O.C[k, 1]) % NY, k)
-                exp[i, j, k] = 0.0 if mask[src[0], src[1]] == -1 else (src[0] * NY + src[1]) * 9 + src[2] + 1.0
+                exp[i, j, k] = (src[0] * NY + src[1]) * 9 + src[2] + 1.0
     return exp
 
 
@@ -135,7 +134,16 @@ def test_codes_reproduce_stream_and_bounce_rules(seed):
     exp = _expected_from_codes(code, NX, NY, mask)
     act = mask == 1
     assert np.array_equal(got[act], exp[act])
-    assert np.all(code[(mask == -1).reshape(-1)] == 0)
+    # PULLED marks exactly the solid nodes whose f2 the replay streams into an
+    # active node (tag t = node * 9 + k + 1)
+    src_nodes = (got[act].astype(np.int64).reshape(-1) - 1) // 9
+    pulled = np.zeros(NX * NY, bool)
+    pulled[src_nodes] = True
+    pulled &= (mask == -1).reshape(-1)
+    assert np.array_equal((code & H.CODE_PULLED) != 0, pulled)
+    assert np.all(code[(mask == -1).reshape(-1)] & ~np.uint16(H.CODE_PULLED) == 0)
+    if seed == 1:
+        assert pulled.any()     # periodic rules over wall nodes pull from solid nodes
 
 
 def test_cylinder_codes_flags():

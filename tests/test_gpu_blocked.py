@@ -86,3 +86,50 @@ def test_reference_cylinder_rules_two_steps():
     sa.run(16)
     sb.run(16)
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
+
+
+def test_large_lattice_edge_tiles():
+    """154 x 24 tiles of 13 x 64 nodes, cut by both lattice edges
+    (2000 = 153 x 13 + 11, 1500 = 23 x 64 + 28)."""
+    print("case porous 2000x1500 mrt power law", flush=True)
+    build = lambda: cases.porous(NX=2000, NY=1500, seed=8, collision="mrt", rmin=4.0, rmax=12.0, g=(2e-5, 1e-6))
+    a, b, sa, sb = _pair(build, True)
+    for n in (7, 10):
+        sa.run(n)
+        sb.run(n)
+    _same(a, b, ("f", "f2", "rho", "vel", "total_rho", "shear_rate"))
+    assert np.array_equal(a["coll"].diag.to_numpy(), b["coll"].diag.to_numpy())
+
+
+def test_large_cylinder_node_rules():
+    """137 x 21 tiles of 11 x 64 nodes with a two-node ring: VelocityBB inlet
+    and PressureNEEM outlet rows."""
+    print("case cylinder 1500x1300 trt", flush=True)
+    build = lambda: cases.cylinder(NX=1500, NY=1300, cx=300, cy=650, r=60, collision="trt", inlet="bb")
+    a, b, sa, sb = _pair(build, False)
+    sa.run(9)
+    sb.run(9)
+    _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
+
+
+@pytest.mark.parametrize("coll", ["bgk", "mrt"])
+def test_pulls_from_outside_the_fluid_group(coll):
+    """No BounceBackWall: fluid nodes next to a disk stream the disk's
+    unchanged f2 (nodes marked PULLED, staged in LDS by the two-step
+    kernel); the disks hold the rest equilibrium w_k."""
+    print(f"case porous 150x140 {coll} without walls", flush=True)
+    build = lambda: cases.porous(NX=150, NY=140, seed=9, collision=coll, rmin=3.0, rmax=7.0, g=(1e-5, 0.0),
+                                 walls=False)
+    a, b, sa, sb = _pair(build, False)
+    w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+    for case in (a, b):
+        solid = case["lbf"].mask.arr == -1
+        assert solid.any()
+        for name in ("f", "f2"):
+            x = getattr(case["lbf"], name).to_numpy()
+            x[solid] = w
+            getattr(case["lbf"], name).from_numpy(x)
+    sa.run(11)
+    sb.run(11)
+    _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
+    assert np.all(np.isfinite(a["lbf"].f2.to_numpy()))
