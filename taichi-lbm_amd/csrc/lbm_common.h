@@ -62,26 +62,26 @@ __device__ __forceinline__ R rcp(R x) {
   }
 }
 
-// g^e for the power law nu = m g^(n-1) (FluidModels.py:26-29): exp2(e log2 g)
-// with the hardware log2 / exp2 in FP32 (a few ulp against pow's one; the
-// parity tolerance on tau is 1e-5), pow in FP64.  g = 0 gives +inf for e < 0
-// and 0 for e > 0 as pow does; e = 0 (n = 1) gives 1.
+// g^e for the power law nu = m g^(n-1) (FluidModels.py:26-29), from the
+// squared rate g2 = g^2: exp2((e/2) log2 g2) with the hardware log2 / exp2 in
+// FP32 (a few ulp against pow's one; the parity tolerance on tau is 1e-5),
+// pow(sqrt g2, e) in FP64.  g = 0 gives +inf for e < 0 and 0 for e > 0 as pow
+// does; e = 0 (n = 1) gives 1.
 template <typename R>
-__device__ __forceinline__ R pow_shear(R g, R e) {
+__device__ __forceinline__ R pow_shear2(R g2, R e) {
   if constexpr (sizeof(R) == 4) {
     if (e == R(0)) return R(1);
-    return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(g));
+    return __builtin_amdgcn_exp2f((e * R(0.5)) * __builtin_amdgcn_logf(g2));
   } else {
-    return pow(g, e);
+    return pow(sqrt(g2), e);
   }
 }
 
-// Collision.tau_max_min, CollisionEngine.py:28-34.
+// Collision.tau_max_min, CollisionEngine.py:28-34 (as selects: NaN passes
+// through, as in the reference).
 template <typename R>
 __device__ __forceinline__ R clamp_tau(R t, R tmin, R tmax) {
-  if (t < tmin) t = tmin;
-  else if (t > tmax) t = tmax;
-  return t;
+  return t < tmin ? tmin : (t > tmax ? tmax : t);
 }
 
 // ---- buffer addressing ------------------------------------------------------

@@ -263,7 +263,7 @@ struct StepOut {
   R rho[NC], rtot, ux, uy;    // macroscopic fields of the step
   R Fx[NC], Fy[NC];           // body force (force_density)
   R scx[NC], scy[NC];         // Shan-Chen force
-  R gam[NC];                  // shear rate (power law)
+  R gam[NC];                  // squared shear rate (power law)
   R state[NC];                // new relaxation state (power law)
 };
 
@@ -274,16 +274,18 @@ template <int COLL, int NC, typename R>
 __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32_t n, uint32_t i, uint32_t j,
                                           const R (&f)[NC][9], const R (&state_in)[NC], StepOut<NC, R>& o) {
   const uint32_t ps = a.ps;
-  // S1 density (lbm.py:153-164)
-  o.rtot = R(0);
+  // S1 density (lbm.py:153-164) with the first moments, from the MRT moment
+  // sums: m0 = rho, m3 = jx, m5 = jy (the other moments feed the MRT
+  // collision; BGK / TRT do not compute them)
+  R mom[NC][9];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    R s = R(0);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) s += f[c][k];
-    o.rho[c] = s;
-    o.rtot += s;
+    mrt_m(f[c], mom[c]);
+    o.rho[c] = mom[c][0];
   }
+  o.rtot = o.rho[0];
+#pragma unroll
+  for (int c = 1; c < NC; ++c) o.rtot += o.rho[c];
 
   // Shan-Chen force (ExternalForce.py:78-91 / 124-145); psi read at x + c_k.
 #pragma unroll
@@ -339,15 +341,13 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
   R vx = R(0), vy = R(0);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const R* g = f[c];
-    R tx = g[1] - g[3] + g[5] - g[6] - g[7] + g[8];
-    R ty = g[2] - g[4] + g[5] + g[6] - g[7] - g[8];
+    R tx = mom[c][3], ty = mom[c][5];
     if (force) {
       tx = tx + R(0.5) * o.Fx[c];
       ty = ty + R(0.5) * o.Fy[c];
     }
-    vx += tx;
-    vy += ty;
+    vx = c == 0 ? tx : vx + tx;
+    vy = c == 0 ? ty : vy + ty;
   }
   o.ux = vx * inv_rtot;
   o.uy = vy * inv_rtot;
@@ -363,6 +363,7 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
     const R s_bulk = a.s_bulk[c];
     o.gam[c] = R(0);
     o.state[c] = state_in[c];
+    const MrtNeq<R> nq = mrt_neq(mom[c], o.rho[c], o.ux, o.uy);   // MRT only (dead code otherwise)
     if (a.tau_mode == LBM_TAU_FIELD) {
       const R state = state_in[c];
       if (is_mrt) {
@@ -373,10 +374,10 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
       }
     } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
       const R state = state_in[c];
-      const R g = is_mrt ? shear_rate_of<LBM_MRT, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk)
-                         : shear_rate_of<LBM_BGK, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
-      o.gam[c] = g;
-      const R nu = (R)a.visc_m * pow_shear(g, (R)a.visc_nm1);
+      const R g2 = is_mrt ? mrt_shear2(nq, rcp(o.rho[c]), state, s_bulk)
+                          : shear_rate2_of<LBM_BGK, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
+      o.gam[c] = g2;
+      const R nu = (R)a.visc_m * pow_shear2(g2, (R)a.visc_nm1);
       const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
       if (is_mrt) {
         s_shear = rcp(t);
@@ -387,10 +388,10 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
         if (is_trt) tau_anti = (R)a.magic[c] * rcp(t - R(0.5)) + R(0.5);
       }
     }
-    if constexpr (COLL >= 0) {
+    if (is_mrt) {
+      mrt_collide_m(mom[c], nq, o.f[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, s_shear, s_bulk);
+    } else if constexpr (COLL >= 0) {
       collide<COLL, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
-    } else if (is_mrt) {
-      collide<LBM_MRT, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
     } else if (is_trt) {
       collide<LBM_TRT, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
     } else {
@@ -424,7 +425,7 @@ __device__ __forceinline__ void store_macro(const Lat& a, R* mo, uint32_t n, con
         a.sc_force[(size_t)(2 * c) * ps + n] = (float)o.scx[c];
         a.sc_force[(size_t)(2 * c + 1) * ps + n] = (float)o.scy[c];
       }
-      if (a.tau_mode == LBM_TAU_POWER_LAW && a.shear_rate) a.shear_rate[(size_t)c * ps + n] = (float)o.gam[c];
+      if (a.tau_mode == LBM_TAU_POWER_LAW && a.shear_rate) a.shear_rate[(size_t)c * ps + n] = (float)sqrt(o.gam[c]);
     }
   }
 }

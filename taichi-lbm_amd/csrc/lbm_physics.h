@@ -35,41 +35,111 @@ __device__ __forceinline__ bool neem_unknown(int d, int k) {
   return false;
 }
 
-// M f for the MRT moment basis (CollisionEngine.py:273-282, :379-385).
+// M f for the MRT moment basis (CollisionEngine.py:273-282, :379-385),
+// from shared pair sums and differences (m0 = rho, m3 = jx, m5 = jy).
 template <typename R>
 __device__ __forceinline__ void mrt_m(const R* f, R* m) {
-  const R s14 = f[1] + f[2] + f[3] + f[4];
-  const R s58 = f[5] + f[6] + f[7] + f[8];
+  const R s13 = f[1] + f[3], s24 = f[2] + f[4], d13 = f[1] - f[3], d24 = f[2] - f[4];
+  const R s57 = f[5] + f[7], s68 = f[6] + f[8], d57 = f[5] - f[7], d68 = f[6] - f[8];
+  const R s14 = s13 + s24, s58 = s57 + s68;
+  const R dx = d57 - d68, dy = d57 + d68;   // f5 - f6 - f7 + f8, f5 + f6 - f7 - f8
   m[0] = f[0] + s14 + s58;
   m[1] = R(-4) * f[0] - s14 + R(2) * s58;
   m[2] = R(4) * f[0] - R(2) * s14 + s58;
-  m[3] = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
-  m[4] = R(-2) * f[1] + R(2) * f[3] + f[5] - f[6] - f[7] + f[8];
-  m[5] = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
-  m[6] = R(-2) * f[2] + R(2) * f[4] + f[5] + f[6] - f[7] - f[8];
-  m[7] = f[1] - f[2] + f[3] - f[4];
-  m[8] = f[5] - f[6] + f[7] - f[8];
+  m[3] = d13 + dx;
+  m[4] = R(-2) * d13 + dx;
+  m[5] = d24 + dy;
+  m[6] = R(-2) * d24 + dy;
+  m[7] = s13 - s24;
+  m[8] = s57 - s68;
 }
 
-// f = M^-1 m with the exact inverse M^T diag(1/9,1/36,1/36,1/6,1/12,1/6,1/12,1/4,1/4).
+// f = M^-1 m with the exact inverse M^T diag(1/9,1/36,1/36,1/6,1/12,1/6,1/12,1/4,1/4),
+// the diagonal folded into the shared sums.
 template <typename R>
 __device__ __forceinline__ void mrt_minv(const R* m, R* f) {
-  const R a0 = m[0] * (R(1) / R(9)), a1 = m[1] * (R(1) / R(36)), a2 = m[2] * (R(1) / R(36));
-  const R a3 = m[3] * (R(1) / R(6)), a4 = m[4] * (R(1) / R(12));
-  const R a5 = m[5] * (R(1) / R(6)), a6 = m[6] * (R(1) / R(12));
-  const R a7 = m[7] * R(0.25), a8 = m[8] * R(0.25);
-  const R b = a0 - a1 - R(2) * a2;   // common to k = 1..4
-  const R e = a0 + R(2) * a1 + a2;   // common to k = 5..8
-  const R x = a3 + a4, y = a5 + a6;
-  f[0] = a0 - R(4) * a1 + R(4) * a2;
-  f[1] = b + a3 - R(2) * a4 + a7;
-  f[2] = b + a5 - R(2) * a6 - a7;
-  f[3] = b - a3 + R(2) * a4 + a7;
-  f[4] = b - a5 + R(2) * a6 - a7;
-  f[5] = e + x + y + a8;
-  f[6] = e - x + y - a8;
-  f[7] = e - x - y + a8;
-  f[8] = e + x - y - a8;
+  const R b = m[0] * (R(1) / R(9)) - m[1] * (R(1) / R(36)) - m[2] * (R(1) / R(18));   // k = 1..4
+  const R e = m[0] * (R(1) / R(9)) + m[1] * (R(1) / R(18)) + m[2] * (R(1) / R(36));   // k = 5..8
+  const R u = (m[3] - m[4]) * (R(1) / R(6)), v = (m[5] - m[6]) * (R(1) / R(6));
+  const R x = m[3] * (R(1) / R(6)) + m[4] * (R(1) / R(12));
+  const R y = m[5] * (R(1) / R(6)) + m[6] * (R(1) / R(12));
+  const R bp = b + m[7] * R(0.25), bm = b - m[7] * R(0.25);
+  const R ep = e + m[8] * R(0.25), em = e - m[8] * R(0.25);
+  const R xpy = x + y, xmy = x - y;
+  f[0] = (m[0] - m[1] + m[2]) * (R(1) / R(9));
+  f[1] = bp + u;
+  f[2] = bm + v;
+  f[3] = bp - u;
+  f[4] = bm - v;
+  f[5] = ep + xpy;
+  f[6] = em - xmy;
+  f[7] = ep - xpy;
+  f[8] = em + xmy;
+}
+
+// MRT equilibrium moments that the relaxation reads (CollisionEngine.py:388-405),
+// and the non-equilibrium parts of e, p_xx, p_xy shared by the shear rate and
+// the relaxation.
+template <typename R>
+struct MrtNeq {
+  R meq2, meq3, meq5;   // relaxed with rate 1: the post-collision moment is the equilibrium
+  R n1, n7, n8;         // m1 - meq1, m7 - meq7, m8 - meq8
+};
+template <typename R>
+__device__ __forceinline__ MrtNeq<R> mrt_neq(const R* m, R rho, R ux, R uy) {
+  const R uv = ux * ux + uy * uy;
+  MrtNeq<R> q;
+  q.meq2 = rho * (R(1.0) - R(3.0) * uv);
+  q.meq3 = rho * ux;
+  q.meq5 = rho * uy;
+  q.n1 = m[1] - rho * (-R(2.0) + R(3.0) * uv);
+  q.n7 = m[7] - rho * (ux * ux - uy * uy);
+  q.n8 = m[8] - rho * (ux * uy);
+  return q;
+}
+
+// MRT collision from the moments: S = diag(0, s_bulk, 1, 0, 1, 0, 1, s, s)
+// (CollisionEngine.py:301-318), Guo forcing (1 - S/2) M F in moment space.
+template <typename R>
+__device__ __forceinline__ void mrt_collide_m(const R* m, const MrtNeq<R>& q, R (&out)[9], R ux, R uy, R Fx, R Fy,
+                                              bool force, R s_shear, R s_bulk) {
+  R m2[9];
+  m2[0] = m[0];
+  m2[1] = m[1] - s_bulk * q.n1;
+  m2[2] = q.meq2;
+  m2[3] = m[3];
+  m2[4] = -q.meq3;
+  m2[5] = m[5];
+  m2[6] = -q.meq5;
+  m2[7] = m[7] - s_shear * q.n7;
+  m2[8] = m[8] - s_shear * q.n8;
+  if (force) {
+    const R Fu = Fx * ux + Fy * uy;
+    m2[1] = m2[1] + (R(1.0) - s_bulk / R(2.0)) * (R(6.0) * Fu);
+    m2[2] = m2[2] + R(0.5) * (-R(6.0) * Fu);
+    m2[3] = m2[3] + Fx;
+    m2[4] = m2[4] + R(0.5) * (-Fx);
+    m2[5] = m2[5] + Fy;
+    m2[6] = m2[6] + R(0.5) * (-Fy);
+    m2[7] = m2[7] + (R(1.0) - s_shear / R(2.0)) * (R(2.0) * (Fx * ux - Fy * uy));
+    m2[8] = m2[8] + (R(1.0) - s_shear / R(2.0)) * (Fy * ux + Fx * uy);
+  }
+  mrt_minv(m2, out);
+}
+
+// Squared strain-rate magnitude 2 (Sxx^2 + Syy^2 + 2 Sxy^2) of MRT
+// (CollisionEngine.py:359-377) from the non-equilibrium moments: with
+// v = S M (f - feq) (v0 = 0, v1 = s_bulk n1, v7 = s n7, v8 = s n8) the
+// reference's sums over M^-1 v are Sxx = cof (v1/6 + v7/2), Syy = cof (v1/6 -
+// v7/2), Sxy = cof v8 (column sums of M^-1), cof = -1.5 / rho.  `s` is the
+// relaxation rate of the previous step (the stored state).
+template <typename R>
+__device__ __forceinline__ R mrt_shear2(const MrtNeq<R>& q, R inv_rho, R s, R s_bulk) {
+  const R cof = -R(1.5) * inv_rho;
+  const R a = s_bulk * q.n1 * (R(1) / R(6));
+  const R b = s * q.n7 * R(0.5);
+  const R Sxx = (a + b) * cof, Syy = (a - b) * cof, Sxy = s * q.n8 * cof;
+  return R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy);
 }
 
 // Strain-rate magnitude from a (filtered) non-equilibrium part, shared by
@@ -118,39 +188,16 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
       out[k] = v;
     }
   } else {
-    R m[9], m2[9];
+    R m[9];
     mrt_m(f, m);
-    const R meq1 = rho * (-R(2.0) + R(3.0) * uv), meq2 = rho * (R(1.0) - R(3.0) * uv);
-    const R meq3 = rho * ux, meq5 = rho * uy;
-    const R meq7 = rho * (ux * ux - uy * uy), meq8 = rho * (ux * uy);
-    // S = diag(0, s_bulk, 1, 0, 1, 0, 1, s, s) (CollisionEngine.py:301-318)
-    m2[0] = m[0];
-    m2[1] = m[1] - s_bulk * (m[1] - meq1);
-    m2[2] = m[2] - (m[2] - meq2);
-    m2[3] = m[3];
-    m2[4] = m[4] - (m[4] + meq3);
-    m2[5] = m[5];
-    m2[6] = m[6] - (m[6] + meq5);
-    m2[7] = m[7] - s_shear * (m[7] - meq7);
-    m2[8] = m[8] - s_shear * (m[8] - meq8);
-    if (force) {
-      const R Fu = Fx * ux + Fy * uy;
-      m2[1] = m2[1] + (R(1.0) - s_bulk / R(2.0)) * (R(6.0) * Fu);
-      m2[2] = m2[2] + R(0.5) * (-R(6.0) * Fu);
-      m2[3] = m2[3] + Fx;
-      m2[4] = m2[4] + R(0.5) * (-Fx);
-      m2[5] = m2[5] + Fy;
-      m2[6] = m2[6] + R(0.5) * (-Fy);
-      m2[7] = m2[7] + (R(1.0) - s_shear / R(2.0)) * (R(2.0) * (Fx * ux - Fy * uy));
-      m2[8] = m2[8] + (R(1.0) - s_shear / R(2.0)) * (Fy * ux + Fx * uy);
-    }
-    mrt_minv(m2, out);
+    mrt_collide_m(m, mrt_neq(m, rho, ux, uy), out, ux, uy, Fx, Fy, force, s_shear, s_bulk);
   }
 }
 
-// Power-law shear rate of one component (CollisionEngine.py:80-92, 166-178,
-// 332-377).  `tau_state` is the stored relaxation state of the previous step:
-// BGK tau, TRT tau_sym, MRT diag[7] (= 1/tau).
+// Power-law shear rate of one component, squared: 2 (Sxx^2 + Syy^2 + 2 Sxy^2)
+// (CollisionEngine.py:80-92, 166-178, 332-377; the rate itself is its square
+// root).  `tau_state` is the stored relaxation state of the previous step: BGK
+// tau, TRT tau_sym, MRT diag[7] (= 1/tau).
 //
 // The reference sums the non-equilibrium part over the populations:
 //   Sxx = cof sum_k cx^2 q_k,  Syy = cof sum_k cy^2 q_k,  Sxy = cof sum_k cx cy q_k
@@ -158,39 +205,25 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
 // (MRT, cof = -1.5/rho).  These are evaluated here in moment form:
 //   sum_k cx^2 feq = rho (1/3 + ux^2),  sum_k cy^2 feq = rho (1/3 + uy^2),
 //   sum_k cx cy feq = rho ux uy;
-// and, for MRT, sum_k {cx^2, cy^2, cx cy} (M^-1 v)_k = {v0 2/3 + v1/6 + v7/2,
-// v0 2/3 + v1/6 - v7/2, v8} (the column sums of M^-1 = M^T diag(1/9, 1/36,
-// 1/36, 1/6, 1/12, 1/6, 1/12, 1/4, 1/4)), where v = S M (f - feq) has
-// v0 = 0, v1 = s_bulk (e - e_eq), v7 = s (p_xx - p_xx,eq), v8 = s (p_xy - p_xy,eq).
-// The same quantities with about a tenth of the arithmetic of the matrix form.
+// MRT: see mrt_shear2.  The same quantities with about a tenth of the
+// arithmetic of the matrix form.
 template <int COLL, typename R>
-__device__ __forceinline__ R shear_rate_of(const R (&f)[9], R rho, R ux, R uy, R tau_state, R s_bulk) {
-  const R s14 = f[1] + f[2] + f[3] + f[4];
-  const R s58 = f[5] + f[6] + f[7] + f[8];
-  const R pxy = f[5] - f[6] + f[7] - f[8];
-  R Sxx, Syy, Sxy;
+__device__ __forceinline__ R shear_rate2_of(const R (&f)[9], R rho, R ux, R uy, R tau_state, R s_bulk) {
   if constexpr (COLL == LBM_MRT) {
-    const R uv = ux * ux + uy * uy;
-    const R e = R(-4) * f[0] - s14 + R(2) * s58;
-    const R pxx = f[1] - f[2] + f[3] - f[4];
-    const R m1 = e - rho * (-R(2) + R(3) * uv);
-    const R m7 = pxx - rho * (ux * ux - uy * uy);
-    const R m8 = pxy - rho * (ux * uy);
-    const R cof = -R(1.5) * rcp(rho);
-    const R a = s_bulk * m1 * (R(1) / R(6));
-    const R b = tau_state * m7 * R(0.5);
-    Sxx = (a + b) * cof;
-    Syy = (a - b) * cof;
-    Sxy = tau_state * m8 * cof;
+    R m[9];
+    mrt_m(f, m);
+    return mrt_shear2(mrt_neq(m, rho, ux, uy), rcp(rho), tau_state, s_bulk);
   } else {
+    const R s58 = f[5] + f[6] + f[7] + f[8];
+    const R pxy = f[5] - f[6] + f[7] - f[8];
     const R cof = -R(1.5) * rcp(rho * tau_state);
     const R pxx = f[1] + f[3] + s58;     // sum_k cx^2 f_k
     const R pyy = f[2] + f[4] + s58;     // sum_k cy^2 f_k
-    Sxx = (pxx - rho * (R(1) / R(3) + ux * ux)) * cof;
-    Syy = (pyy - rho * (R(1) / R(3) + uy * uy)) * cof;
-    Sxy = (pxy - rho * (ux * uy)) * cof;
+    const R Sxx = (pxx - rho * (R(1) / R(3) + ux * ux)) * cof;
+    const R Syy = (pyy - rho * (R(1) / R(3) + uy * uy)) * cof;
+    const R Sxy = (pxy - rho * (ux * uy)) * cof;
+    return R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy);
   }
-  return sqrt(R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy));
 }
 
 // SC_psi.get_psi, EOS.py:40-47.
