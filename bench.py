@@ -120,7 +120,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     import torch
     from taichi_lbm import _hip as H
     from taichi_lbm import cases
-    from taichi_lbm.batch import BatchRunner, autotune_grid, spec_from_case
+    from taichi_lbm.batch import BatchRunner, autotune_grid, autotune_tiles, spec_from_case
     rng = np.random.default_rng(seed)
     specs = []
     t0 = time.time()
@@ -140,11 +140,14 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         runner = BatchRunner(specs, storage="f32")
         runner.run(2)
         mode = runner.mode
+        tile_cols = 0
         if mode == 0:          # the persistent one-step grid is tuned; two-step launches have a block per tile
             grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
             log(f"autotuned grid {grid}: {timings}")
-        else:
+        else:                  # two-step launches: the tile width is tuned
             grid = 0
+            tile_cols, timings = autotune_tiles(runner, candidates=(32, 64), steps=10)
+            log(f"autotuned tile columns {tile_cols}: {timings}")
             runner.run(10)
         g = runner.capture(graph_steps)
         g.replay()
@@ -181,7 +184,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         for l in slots:
             H.check(lib.lbm_run_blocked(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
                                         ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
-                                                    ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, None,
+                                                    ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, 0, None,
                                         ctypes.byref(out), s))
         e1.record(stream)
         torch.cuda.synchronize()
@@ -208,7 +211,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             "solves_per_s_batched": round(sps_b, 3), "solves_per_s_plain": round(sps_p, 3),
             "speedup": round(sps_b / sps_p, 2), "launch_mode": mode,
             "ms_per_step_plain_two_step": round(ms_p2, 4),
-            "speedup_vs_plain_two_step": round(ms_p2 / ms_b, 2), "grid_blocks": grid, "graph_steps": graph_steps,
+            "speedup_vs_plain_two_step": round(ms_p2 / ms_b, 2), "grid_blocks": grid, "tile_cols": tile_cols, "graph_steps": graph_steps,
             "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps,
             "finite_after_solve": int(sum(finite)), "fp64_sample": len(sample),
             "max_abs_u_err_vs_fp64": err64}

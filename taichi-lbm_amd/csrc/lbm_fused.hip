@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // ---------------------------------------------------------------------------
 // Two steps per launch (temporal blocking through LDS).
 //
-// A block owns a tile of TI rows x kTJ columns.  Phase A runs step n on the
+// A block owns a tile of TI rows x W columns.  Phase A runs step n on the
 // tile and a ring of H nodes around it (pull gathers and node rules read HBM,
 // exactly as the one-step kernel) and keeps the post-collision populations,
 // relaxation state and -- with node rules -- the macroscopic fields in LDS;
@@ -541,16 +541,20 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // one step per launch.  The result of a launch goes to pop[parity^1] like a
 // one-step launch: a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
-constexpr int kTJ = 64;       // tile columns: one wavefront per tile row
 constexpr int kBlock2 = 1024;
+constexpr int kDefaultTileCols = 32;
 
-template <int H>
+// Tile of W = 64 or 32 columns (one or half a wavefront per tile row); the
+// rows are as many as let the ring-extended tile fit one round of the block:
+// W = 64: (13 + 2) x 66 = 990 and (11 + 4) x 68 = 1020 nodes; W = 32:
+// (28 + 2) x 34 = 1020 and (24 + 4) x 36 = 1008.  Narrow tiles re-read fewer
+// ring nodes per tile node and leave fewer lanes idle at a lattice's last
+// tile column; wide ones read longer rows.
+template <int H, int W>
 struct TileShape {
-  // tile rows chosen so the ring-extended tile fits one round of the block:
-  // (13 + 2) x 66 = 990 and (11 + 4) x 68 = 1020 nodes
-  static constexpr int TI = H == 1 ? 13 : 11;
+  static constexpr int TI = kBlock2 / (W + 2 * H) - 2 * H;
   static constexpr int RI = TI + 2 * H;     // ring-extended rows
-  static constexpr int RJ = kTJ + 2 * H;    // ring-extended columns
+  static constexpr int RJ = W + 2 * H;      // ring-extended columns
   static constexpr int RN = RI * RJ;
   static constexpr int PJ = RJ + 2;         // LDS row pitch (one pad column each side)
   static constexpr int PN = (RI + 2) * PJ;  // LDS slots (one pad row above and below)
@@ -561,10 +565,10 @@ struct TileShape {
   static constexpr int back(int k) { return cx(k) * PJ + cy(k); }
 };
 
-template <int NC, int S, int H>
+template <int NC, int S, int H, int W>
 struct TileLds {
   using R = typename Pop<S>::R;
-  static constexpr int PN = TileShape<H>::PN;
+  static constexpr int PN = TileShape<H, W>::PN;
   R f[NC * 9][PN];                      // what each node's pull of step n+1 receives (see above)
   R mac[H == 2 ? NC + 3 : 1][H == 2 ? PN : 1];   // rho_c, total_rho, ux, uy of step n (node rules)
   R st[NC][PN];                         // relaxation state after step n
@@ -572,21 +576,21 @@ struct TileLds {
 };
 
 // Step n as phase B reads it: the LDS tile, nodes addressed by their slot.
-template <int NC, int S, int H>
+template <int NC, int S, int H, int W>
 struct LdsView {
   using R = typename Pop<S>::R;
   struct Node {
     uint32_t q;
   };
-  const TileLds<NC, S, H>* L;
+  const TileLds<NC, S, H, W>* L;
   __device__ __forceinline__ Node back(const Node& x, int di, int dj) const {
-    return Node{(uint32_t)((int)x.q - di * TileShape<H>::PJ - dj)};
+    return Node{(uint32_t)((int)x.q - di * TileShape<H, W>::PJ - dj)};
   }
   __device__ __forceinline__ uint32_t code_of(const Node& x) const { return L->code[x.q]; }
   __device__ __forceinline__ R macro(const Node& x, int plane) const { return L->mac[plane][x.q]; }
   // post-stream population k (bounces were resolved by phase A's reflections)
   __device__ __forceinline__ R pop(const Node& x, uint32_t, int c, int k) const {
-    return L->f[c * 9 + k][x.q - TileShape<H>::back(k)];
+    return L->f[c * 9 + k][x.q - TileShape<H, W>::back(k)];
   }
 };
 
@@ -682,10 +686,10 @@ __device__ __forceinline__ void ring_load(const Lat& a, bool local, const typena
 // Phase A arithmetic of one active ring node: node rules (reading step n-1's
 // state in HBM), step n, results into LDS slot q.  The populations that the
 // node bounces are returned in `refl` for the reflection writes.
-template <int COLL, int NC, int S, int H>
+template <int COLL, int NC, int S, int H, int W>
 __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typename Pop<S>::T* __restrict__ src,
                                           uint32_t parity, uint32_t q, uint32_t gi, uint32_t gj, uint32_t code,
-                                          RingIn<NC, S>& in, TileLds<NC, S, H>& L,
+                                          RingIn<NC, S>& in, TileLds<NC, S, H, W>& L,
                                           typename Pop<S>::R (&refl)[NC][9]) {
   using R = typename Pop<S>::R;
   const uint32_t n = gi * a.ny + gj;
@@ -719,23 +723,23 @@ __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typena
 // Reflections of one active ring node (after the barrier that follows the
 // own writes): population k that the node bounces goes to slot q - c_k of
 // plane k, the slot its pull reads.
-template <int NC, int S, int H>
+template <int NC, int S, int H, int W>
 __device__ __forceinline__ void ring_reflect(uint32_t q, uint32_t code, const typename Pop<S>::R (&refl)[NC][9],
-                                             TileLds<NC, S, H>& L) {
+                                             TileLds<NC, S, H, W>& L) {
 #pragma unroll
   for (int k = 1; k < 9; ++k) {
     if (code & (1u << (k - 1))) {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) L.f[c * 9 + k][q - TileShape<H>::back(k)] = refl[c][k];
+      for (int c = 0; c < NC; ++c) L.f[c * 9 + k][q - TileShape<H, W>::back(k)] = refl[c][k];
     }
   }
 }
 
 // A node outside the fluid group that an active node pulls: its unchanged f2
 // (the stored value in HBM, as the one-step kernel reads it) into its slot.
-template <int NC, int S, int H>
+template <int NC, int S, int H, int W>
 __device__ __forceinline__ void ring_pulled(const Lat& a, const typename Pop<S>::T* __restrict__ stale, uint32_t q,
-                                            uint32_t n, TileLds<NC, S, H>& L) {
+                                            uint32_t n, TileLds<NC, S, H, W>& L) {
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -745,24 +749,24 @@ __device__ __forceinline__ void ring_pulled(const Lat& a, const typename Pop<S>:
 // Phase B: step n+1 on the tile from LDS; results to pop[parity^1] (and the
 // relaxation state to tau_out under the power law, macroscopic fields when
 // the run ends).
-template <int COLL, int NC, int S, int H, bool BUF>
+template <int COLL, int NC, int S, int H, int W, bool BUF>
 __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q, uint32_t i0, uint32_t j0,
-                                                 void* tau_out, const TileLds<NC, S, H>& L) {
+                                                 void* tau_out, const TileLds<NC, S, H, W>& L) {
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
-  using Shape = TileShape<H>;
+  using Shape = TileShape<H, W>;
   const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
   const bool pl = a.tau_mode == LBM_TAU_POWER_LAW;
   T* __restrict__ dst = (T*)a.pop[q.parity ^ 1u];
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
   R* mo = (R*)a.mac[q.parity ^ 1u];
-  const LdsView<NC, S, H> v{&L};
-  for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * kTJ); t += kBlock2) {
-    const uint32_t rr = t / kTJ, cc = t - rr * kTJ;
+  const LdsView<NC, S, H, W> v{&L};
+  for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * W); t += kBlock2) {
+    const uint32_t rr = t / W, cc = t - rr * W;
     const uint32_t gi = i0 + rr, gj = j0 + cc;
     if (gi >= nx || gj >= ny) continue;
     const uint32_t n = gi * ny + gj;
-    const typename LdsView<NC, S, H>::Node x{Shape::slot(rr + H, cc + H)};
+    const typename LdsView<NC, S, H, W>::Node x{Shape::slot(rr + H, cc + H)};
     const uint32_t code = L.code[x.q];
     if (!(code & LBM_CODE_ACTIVE)) {
       if (q.write_macro) {
@@ -801,12 +805,12 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
   }
 }
 
-template <int COLL, int NC, int S, int H, bool BUF>
+template <int COLL, int NC, int S, int H, int W, bool BUF>
 __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
-                                               const void* tau_in, void* tau_out, TileLds<NC, S, H>& L) {
+                                               const void* tau_in, void* tau_out, TileLds<NC, S, H, W>& L) {
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
-  using Shape = TileShape<H>;
+  using Shape = TileShape<H, W>;
   const T* __restrict__ src = (const T*)a.pop[q.parity];
   const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * a.ps * (uint32_t)sizeof(T) : 0u);
   const R* tau_a = (const R*)(a.tau_mode == LBM_TAU_POWER_LAW ? tau_in : a.tau);
@@ -823,25 +827,25 @@ __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, b
     if (code & LBM_CODE_ACTIVE) {
       RingIn<NC, S> in;
       ring_load<NC, S, BUF>(a, local, src, rsrc, tau_a, gi, gj, code, in);
-      ring_step<COLL, NC, S, H>(a, local, src, q.parity, qs, gi, gj, code, in, L, refl);
+      ring_step<COLL, NC, S, H, W>(a, local, src, q.parity, qs, gi, gj, code, in, L, refl);
     } else if (code & LBM_CODE_PULLED) {
-      ring_pulled<NC, S, H>(a, (const T*)a.pop[q.parity ^ 1u], qs, gi * a.ny + gj, L);
+      ring_pulled<NC, S, H, W>(a, (const T*)a.pop[q.parity ^ 1u], qs, gi * a.ny + gj, L);
     }
   }
   __syncthreads();
-  if ((code & (LBM_CODE_ACTIVE | LBM_CODE_BOUNCE_MASK)) > LBM_CODE_ACTIVE) ring_reflect<NC, S, H>(qs, code, refl, L);
+  if ((code & (LBM_CODE_ACTIVE | LBM_CODE_BOUNCE_MASK)) > LBM_CODE_ACTIVE) ring_reflect<NC, S, H, W>(qs, code, refl, L);
   __syncthreads();
-  tile_second_step<COLL, NC, S, H, BUF>(a, q, i0, j0, tau_out, L);
+  tile_second_step<COLL, NC, S, H, W, BUF>(a, q, i0, j0, tau_out, L);
 }
 
-template <int COLL, int NC, int S, int H, bool BUF>
+template <int COLL, int NC, int S, int H, int W, bool BUF>
 __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
-  __shared__ TileLds<NC, S, H> L;
+  __shared__ TileLds<NC, S, H, W> L;
   const uint32_t nt = q.tiles_i * q.tiles_j;
   uint32_t b = blockIdx.x;
   if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
   const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  tile_two_steps<COLL, NC, S, H, BUF>(a, q, q.local_read != 0, ti * TileShape<H>::TI, tj * kTJ, q.tau_in,
+  tile_two_steps<COLL, NC, S, H, W, BUF>(a, q, q.local_read != 0, ti * TileShape<H, W>::TI, tj * W, q.tau_in,
                                       q.tau_out, L);
 }
 
@@ -852,20 +856,20 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
 // retirements).  Blocks are remapped so that each XCD takes one contiguous
 // range of the tile list.  Power-law lattices are not batched this way
 // (their state would need a second buffer per lattice).
-template <int NC, int S, int H, bool BUF>
+template <int NC, int S, int H, int W, bool BUF>
 __global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
     const Lat* __restrict__ items, const uint32_t* __restrict__ tile_lat, const uint32_t* __restrict__ tile_idx,
     const uint32_t* __restrict__ tile_count, const Launch2 q) {
-  __shared__ TileLds<NC, S, H> L;
+  __shared__ TileLds<NC, S, H, W> L;
   const uint32_t per = gridDim.x >> 3;   // the host launches a multiple of 8 blocks
   const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   if (w >= *tile_count) return;
   const uint32_t l = __builtin_amdgcn_readfirstlane(tile_lat[w]);
   const uint32_t t = __builtin_amdgcn_readfirstlane(tile_idx[w]);
   const Lat& a = items[l];
-  const uint32_t tiles_j = (a.ny + kTJ - 1) / kTJ;
+  const uint32_t tiles_j = (a.ny + W - 1) / W;
   const uint32_t ti = t / tiles_j, tj = t - ti * tiles_j;
-  tile_two_steps<-1, NC, S, H, BUF>(a, q, a.start_step == q.step, ti * TileShape<H>::TI, tj * kTJ, a.tau, nullptr,
+  tile_two_steps<-1, NC, S, H, W, BUF>(a, q, a.start_step == q.step, ti * TileShape<H, W>::TI, tj * W, a.tau, nullptr,
                                     L);
 }
 
@@ -1135,6 +1139,7 @@ struct lbm_batch {
   size_t tile2_cap = 0;
   uint32_t tiles2 = 0;
   int mode2 = 0;                     // 0: one step per launch, else the ring width H
+  int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (32 or 64)
   std::vector<uint32_t> host_t2lat, host_t2idx;
   bool captured = false;             // a graph holds the table addresses: no reallocation
 };
@@ -1167,25 +1172,25 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
 namespace lbm {
 using Kernel2Fn = void (*)(const Lat, const Launch2);
 
-template <int NC, int S, int H, bool BUF>
+template <int NC, int S, int H, int W, bool BUF>
 Kernel2Fn pick2_coll(int coll) {
   switch (coll) {
-    case LBM_BGK: return k_fused2<LBM_BGK, NC, S, H, BUF>;
-    case LBM_TRT: return k_fused2<LBM_TRT, NC, S, H, BUF>;
-    default: return k_fused2<LBM_MRT, NC, S, H, BUF>;
+    case LBM_BGK: return k_fused2<LBM_BGK, NC, S, H, W, BUF>;
+    case LBM_TRT: return k_fused2<LBM_TRT, NC, S, H, W, BUF>;
+    default: return k_fused2<LBM_MRT, NC, S, H, W, BUF>;
   }
 }
-template <int H, bool BUF>
+template <int H, int W, bool BUF>
 Kernel2Fn pick2_h(int nc, int store, int coll) {
   switch (store) {
     case LBM_STORE_F32:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_F32, H, BUF>(coll) : pick2_coll<2, LBM_STORE_F32, H, BUF>(coll);
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F32, H, W, BUF>(coll) : pick2_coll<2, LBM_STORE_F32, H, W, BUF>(coll);
     case LBM_STORE_F16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT, H, BUF>(coll)
-                     : pick2_coll<2, LBM_STORE_F16_SHIFT, H, BUF>(coll);
+      return nc == 1 ? pick2_coll<1, LBM_STORE_F16_SHIFT, H, W, BUF>(coll)
+                     : pick2_coll<2, LBM_STORE_F16_SHIFT, H, W, BUF>(coll);
     case LBM_STORE_BF16_SHIFT:
-      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT, H, BUF>(coll)
-                     : pick2_coll<2, LBM_STORE_BF16_SHIFT, H, BUF>(coll);
+      return nc == 1 ? pick2_coll<1, LBM_STORE_BF16_SHIFT, H, W, BUF>(coll)
+                     : pick2_coll<2, LBM_STORE_BF16_SHIFT, H, W, BUF>(coll);
     default: return nullptr;
   }
 }
@@ -1200,18 +1205,24 @@ bool fits_buffer(const lbm_lattice* lat) {
 }
 // nullptr where the two-step kernel is not built: FP64 storage (the
 // high-precision reference path; in a 1024-thread block its registers would
-// spill) runs one step per launch.
-Kernel2Fn pick2(int nc, int store, int coll, int h, bool buf) {
-  if (buf) return h == 1 ? pick2_h<1, true>(nc, store, coll) : pick2_h<2, true>(nc, store, coll);
-  return h == 1 ? pick2_h<1, false>(nc, store, coll) : pick2_h<2, false>(nc, store, coll);
+// spill) runs one step per launch.  Lattices beyond buffer addressing use
+// 64-column tiles.
+Kernel2Fn pick2(int nc, int store, int coll, int h, int w, bool buf) {
+  if (!buf) return h == 1 ? pick2_h<1, 64, false>(nc, store, coll) : pick2_h<2, 64, false>(nc, store, coll);
+  if (w == 32) return h == 1 ? pick2_h<1, 32, true>(nc, store, coll) : pick2_h<2, 32, true>(nc, store, coll);
+  return h == 1 ? pick2_h<1, 64, true>(nc, store, coll) : pick2_h<2, 64, true>(nc, store, coll);
+}
+// tile columns: 32 or 64 as asked, else the default
+int tile_cols_of(int32_t asked) { return asked == 32 || asked == 64 ? asked : kDefaultTileCols; }
+uint32_t tile_rows(int h, int w) {
+  return (uint32_t)(kBlock2 / (w + 2 * h) - 2 * h);
 }
 int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
-uint32_t tile_rows(int h) { return h == 1 ? (uint32_t)TileShape<1>::TI : (uint32_t)TileShape<2>::TI; }
 }  // namespace lbm
 
 extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
-                               int32_t nsteps, int32_t parity, int32_t first_local, void* tau_next,
-                               int32_t* out_parity, void* stream) {
+                               int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
+                               void* tau_next, int32_t* out_parity, void* stream) {
   using namespace lbm;
   int rc = validate(lat, fld, neem);
   if (rc) return rc;
@@ -1221,7 +1232,10 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   hipStream_t s = (hipStream_t)stream;
   const bool pl = lat->tau_mode == LBM_TAU_POWER_LAW;
   const int h = ring_width(lat);
-  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, fits_buffer(lat));
+  if (tile_cols != 0 && tile_cols != 32 && tile_cols != 64) return fail("tile_cols must be 0, 32 or 64");
+  const bool buf = fits_buffer(lat);
+  const int w = buf ? tile_cols_of(tile_cols) : 64;
+  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, w, buf);
   const bool ok2 = two && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
   if (!ok2) {
     rc = lbm_run(lat, fld, neem, nsteps, parity, first_local, stream);
@@ -1234,8 +1248,8 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   void* tnext = tau_next;
   Launch2 q2;
   memset(&q2, 0, sizeof q2);
-  q2.tiles_i = (a.nx + tile_rows(h) - 1) / tile_rows(h);
-  q2.tiles_j = (a.ny + kTJ - 1) / kTJ;
+  q2.tiles_i = (a.nx + tile_rows(h, w) - 1) / tile_rows(h, w);
+  q2.tiles_j = (a.ny + w - 1) / w;
   const uint64_t nt = (uint64_t)q2.tiles_i * q2.tiles_j;
   if (nt >= (1ull << 31)) return fail("lattice too large for the two-step tiling");
   q2.xcd_remap = (nt % 8 == 0 && nt >= 64) ? 1 : 0;
@@ -1376,7 +1390,8 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
     for (int l = 0; l < b->count; ++l) {
       const Lat& a = b->host[l];
       if (!a.active) continue;
-      const uint32_t nt = ((a.nx + tile_rows(h) - 1) / tile_rows(h)) * ((a.ny + kTJ - 1) / kTJ);
+      const uint32_t w = (uint32_t)b->tile_cols;
+      const uint32_t nt = ((a.nx + tile_rows(h, w) - 1) / tile_rows(h, w)) * ((a.ny + w - 1) / w);
       for (uint32_t t = 0; t < nt; ++t) {
         b->host_t2lat.push_back((uint32_t)l);
         b->host_t2idx.push_back(t);
@@ -1503,18 +1518,20 @@ int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t 
 namespace {
 using Batch2Fn = void (*)(const lbm::Lat*, const uint32_t*, const uint32_t*, const uint32_t*, const lbm::Launch2);
 template <int NC, int S>
-Batch2Fn pick_batch2_s(int h) {
+Batch2Fn pick_batch2_s(int h, int w) {
   // batched lattices are small: buffer addressing (each lattice's segment is
   // checked against 2^31 bytes at commit)
-  return h == 1 ? lbm::k_batch2<NC, S, 1, true> : lbm::k_batch2<NC, S, 2, true>;
+  if (w == 32) return h == 1 ? lbm::k_batch2<NC, S, 1, 32, true> : lbm::k_batch2<NC, S, 2, 32, true>;
+  return h == 1 ? lbm::k_batch2<NC, S, 1, 64, true> : lbm::k_batch2<NC, S, 2, 64, true>;
 }
-Batch2Fn pick_batch2(int nc, int store, int h) {
+Batch2Fn pick_batch2(int nc, int store, int h, int w) {
   switch (store) {
-    case LBM_STORE_F32: return nc == 1 ? pick_batch2_s<1, LBM_STORE_F32>(h) : pick_batch2_s<2, LBM_STORE_F32>(h);
+    case LBM_STORE_F32:
+      return nc == 1 ? pick_batch2_s<1, LBM_STORE_F32>(h, w) : pick_batch2_s<2, LBM_STORE_F32>(h, w);
     case LBM_STORE_F16_SHIFT:
-      return nc == 1 ? pick_batch2_s<1, LBM_STORE_F16_SHIFT>(h) : pick_batch2_s<2, LBM_STORE_F16_SHIFT>(h);
+      return nc == 1 ? pick_batch2_s<1, LBM_STORE_F16_SHIFT>(h, w) : pick_batch2_s<2, LBM_STORE_F16_SHIFT>(h, w);
     case LBM_STORE_BF16_SHIFT:
-      return nc == 1 ? pick_batch2_s<1, LBM_STORE_BF16_SHIFT>(h) : pick_batch2_s<2, LBM_STORE_BF16_SHIFT>(h);
+      return nc == 1 ? pick_batch2_s<1, LBM_STORE_BF16_SHIFT>(h, w) : pick_batch2_s<2, LBM_STORE_BF16_SHIFT>(h, w);
     default: return nullptr;
   }
 }
@@ -1525,7 +1542,7 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
                    int32_t* out) {
   using namespace lbm;
   if (!b->tile_item) return fail("commit the batch before stepping it");
-  Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2) : nullptr;
+  Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2, b->tile_cols) : nullptr;
   uint32_t p = (uint32_t)parity;
   int t0 = 0;
   if (two) {
@@ -1558,6 +1575,24 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
 }  // namespace
 
 extern "C" int lbm_batch_mode(const lbm_batch* b) { return b ? b->mode2 : -1; }
+
+extern "C" int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols) {
+  using namespace lbm;
+  if (!b) return fail("null batch");
+  if (tile_cols != 0 && tile_cols != 32 && tile_cols != 64) return fail("tile_cols must be 0, 32 or 64");
+  const int w = tile_cols_of(tile_cols);
+  if (w == b->tile_cols) return LBM_OK;
+  if (b->captured) return fail("cannot change the tile shape after a graph capture");
+  b->tile_cols = w;
+  return LBM_OK;
+}
+
+extern "C" int lbm_tile_rows(int32_t ring, int32_t tile_cols) {
+  using namespace lbm;
+  if ((ring != 1 && ring != 2) || (tile_cols != 0 && tile_cols != 32 && tile_cols != 64))
+    return fail("lbm_tile_rows: ring 1 or 2, tile_cols 0, 32 or 64");
+  return (int)tile_rows(ring, tile_cols_of(tile_cols));
+}
 
 extern "C" int lbm_batch_run_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks,
                                      int32_t* out_parity, void* stream) {

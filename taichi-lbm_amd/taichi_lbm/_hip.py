@@ -26,7 +26,7 @@ CODE_NEEM_MASK = 0x3C00
 CODE_PULLED = 0x4000      # inactive node pulled by an active one (plain pull)
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 
 class NeemRule(C.Structure):
@@ -94,7 +94,8 @@ def lib():
         "lbm_abi_version": ([], C.c_int),
         "lbm_last_error": ([], C.c_char_p),
         "lbm_run": ([LP, FP, NP, I32, I32, I32, P], C.c_int),
-        "lbm_run_blocked": ([LP, FP, NP, I32, I32, I32, P, C.POINTER(C.c_int32), P], C.c_int),
+        "lbm_run_blocked": ([LP, FP, NP, I32, I32, I32, I32, P, C.POINTER(C.c_int32), P], C.c_int),
+        "lbm_tile_rows": ([I32, I32], C.c_int),
         "lbm_finish": ([LP, FP, NP, I32, P], C.c_int),
         "lbm_reduce_macro": ([LP, P, P, D, P], C.c_int),
         "lbm_residual": ([I64, I64, P, P, D, P], C.c_int),
@@ -120,6 +121,7 @@ def lib():
         "lbm_batch_graph": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p)], C.c_int),
         "lbm_batch_mode": ([BP], C.c_int),
         "lbm_batch_reserve2": ([BP, I32], C.c_int),
+        "lbm_batch_tile_cols": ([BP, I32], C.c_int),
         "lbm_batch_run_blocked": ([BP, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
         "lbm_batch_graph_blocked": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)], C.c_int),
         "lbm_graph_launch": ([P, P], C.c_int),
@@ -145,7 +147,7 @@ def exported_symbols():
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
-            "lbm_batch_mode",
+            "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
 

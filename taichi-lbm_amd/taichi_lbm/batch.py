@@ -78,11 +78,15 @@ def footprint_bytes(spec_or_n, storage="f32") -> int:
     return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + r)
 
 
-def two_step_tiles(specs) -> int:
-    """Tiles of the two-step launch for these lattices (13 x 64 tiles, or
-    11 x 64 when any lattice has node rules)."""
-    ti = 11 if any(len(s.neem) for s in specs) else 13
-    return sum(-(-s.NX // ti) * -(-s.NY // 64) for s in specs)
+def two_step_tiles(specs, tile_cols: int = 0) -> int:
+    """Tiles of the two-step launch for these lattices: tile_cols = 32 or 64
+    columns (0: the library's default) by lbm_tile_rows rows (ring width 2
+    when any lattice has node rules)."""
+    ring = 2 if any(len(s.neem) for s in specs) else 1
+    tj = tile_cols or 32
+    ti = H.lib().lbm_tile_rows(ring, tile_cols)
+    H.check(min(ti, 0), "lbm_tile_rows")
+    return sum(-(-s.NX // ti) * -(-s.NY // tj) for s in specs)
 
 
 class _Arena:
@@ -121,7 +125,7 @@ class BatchRunner:
     dynamic use (``add`` / ``remove``)."""
 
     def __init__(self, specs: Sequence[LatticeSpec] = (), storage: str = "f32", grid_blocks: int = 0,
-                 capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True):
+                 capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True, tile_cols: int = 0):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         t = torch()
@@ -162,8 +166,12 @@ class BatchRunner:
         for l in range(n):
             H.check(H.lib().lbm_batch_set(h, l, 0, -1, None), "lbm_batch_set")
         H.check(H.lib().lbm_batch_reserve(h, cap // 256 + n + 8), "lbm_batch_reserve")
+        self.tile_cols = int(tile_cols)
+        H.check(H.lib().lbm_batch_tile_cols(h, self.tile_cols), "lbm_batch_tile_cols")
         if self.blocked and specs:
-            H.check(H.lib().lbm_batch_reserve2(h, two_step_tiles(specs) + 64), "lbm_batch_reserve2")
+            # room for either tile width, so that autotune_tiles can switch
+            need = max(two_step_tiles(specs, 32), two_step_tiles(specs, 64))
+            H.check(H.lib().lbm_batch_reserve2(h, need + 64), "lbm_batch_reserve2")
         self.slots: Dict[int, dict] = {}
         self.parity = 0
         self.finished = False
@@ -287,6 +295,12 @@ class BatchRunner:
                         "lbm_batch_set")
             self._pending_local.clear()
             self._dirty = True
+
+    def set_tile_cols(self, tile_cols: int):
+        """Two-step tile width (32 or 64; 0: default) from the next commit on."""
+        H.check(H.lib().lbm_batch_tile_cols(self.h, int(tile_cols)), "lbm_batch_tile_cols")
+        self.tile_cols = int(tile_cols)
+        self._dirty = True
 
     @property
     def mode(self) -> int:
@@ -416,6 +430,27 @@ class BatchGraph:
             self.close()
         except Exception:
             pass
+
+
+def autotune_tiles(runner: BatchRunner, candidates=(32, 64), steps: int = 10):
+    """Times the two-step tile width on the runner's own batch (steps a
+    multiple of 2) and keeps the fastest; before any graph capture.  Advances
+    the batch by len(candidates) * (steps + 2) steps."""
+    t = torch()
+    timings = {}
+    for w in candidates:
+        runner.set_tile_cols(w)
+        runner.run(2)
+        s = t.cuda.current_stream()
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(s)
+        runner.run(steps)
+        e1.record(s)
+        t.cuda.synchronize()
+        timings[w] = e0.elapsed_time(e1) / steps
+    best = min(timings, key=timings.get)
+    runner.set_tile_cols(best)
+    return best, timings
 
 
 def autotune_grid(runner: BatchRunner, candidates=(2, 4, 8, 16, 32), steps: int = 10,

@@ -40,11 +40,13 @@ def _same_nodes(g1, g2, NY) -> bool:
 
 class Stepper:
     def __init__(self, lb_field, macroscopic_engine, collision_engine, boundary_engine, power_law=False,
-                 boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True):
+                 boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0):
         """blocked: run two steps per launch where the lattice allows it
-        (lbm_run_blocked; bit-identical to one step per launch)."""
+        (lbm_run_blocked; bit-identical to one step per launch), on tiles of
+        tile_cols = 32 or 64 columns (0: the library's default)."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
+        self.tile_cols = int(tile_cols)
         self._tau_next = None
         self.macro = macroscopic_engine
         self.coll = collision_engine
@@ -127,7 +129,8 @@ class Stepper:
                 tau_next = H.ptr(self._tau_next)
             out = ctypes.c_int32(-1)
             H.check(H.lib().lbm_run_blocked(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
-                                            tau_next, ctypes.byref(out), stream()), "lbm_run_blocked")
+                                            self.tile_cols, tau_next, ctypes.byref(out), stream()),
+                    "lbm_run_blocked")
             st["f2_idx"] = int(out.value)
         else:
             H.check(H.lib().lbm_run(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,

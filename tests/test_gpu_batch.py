@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from taichi_lbm import cases
-from taichi_lbm.batch import BatchRunner, autotune_grid, spec_from_case
+from taichi_lbm.batch import BatchRunner, autotune_grid, autotune_tiles, spec_from_case
 from taichi_lbm.stepper import Stepper
 from oracle_bridge import oracle_solver
 from tolerances import check
@@ -148,11 +148,13 @@ def test_autotune_picks_a_candidate():
     assert best in timings and best % 8 == 0
 
 
-def test_blocked_batch_equals_one_step_batch():
+@pytest.mark.parametrize("tile_cols", [32, 64])
+def test_blocked_batch_equals_one_step_batch(tile_cols):
     """Two-step batched launches (ring width 2: the channels have node rules)
-    give the fields of one-step batched launches, bit for bit."""
+    give the fields of one-step batched launches, bit for bit, with either
+    tile width."""
     cs = _mix()
-    a = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=True)
+    a = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=True, tile_cols=tile_cols)
     b = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=False)
     assert a.mode == 2 and b.mode == 0
     for n in (7, 10):
@@ -178,6 +180,9 @@ def test_blocked_batch_without_rules_and_graph():
     assert a.mode == 1
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
+        best, timings = autotune_tiles(a, candidates=(32, 64), steps=2)   # switches widths before capture
+        assert best in (32, 64) and set(timings) == {32, 64}
+        b.run(2 * (2 + 2))
         a.run(2)
         b.run(2)
         g = a.capture(12)

@@ -20,11 +20,11 @@ def _quiet(fn, *a, **k):
         return fn(*a, **k)
 
 
-def _pair(build, power_law):
+def _pair(build, power_law, tile_cols=0):
     a, b = _quiet(build), _quiet(build)
     _quiet(cases.initialise, a)
     _quiet(cases.initialise, b)
-    sa = Stepper(a["lbf"], a["macro"], a["coll"], a["bce"], power_law=power_law, blocked=True)
+    sa = Stepper(a["lbf"], a["macro"], a["coll"], a["bce"], power_law=power_law, blocked=True, tile_cols=tile_cols)
     sb = Stepper(b["lbf"], b["macro"], b["coll"], b["bce"], power_law=power_law, blocked=False)
     return a, b, sa, sb
 
@@ -35,11 +35,11 @@ def _same(a, b, names):
         assert np.array_equal(x, y), (name, np.abs(x - y).max())
 
 
-@pytest.mark.parametrize("coll", ["bgk", "trt", "mrt"])
-def test_blocked_equals_one_step_power_law(coll):
-    print(f"case porous 200x136 {coll} power law", flush=True)
+@pytest.mark.parametrize("coll,tile_cols", [("bgk", 64), ("trt", 32), ("mrt", 64), ("mrt", 32)])
+def test_blocked_equals_one_step_power_law(coll, tile_cols):
+    print(f"case porous 200x136 {coll} power law, {tile_cols}-column tiles", flush=True)
     build = lambda: cases.porous(NX=200, NY=136, seed=5, collision=coll, rmin=3.0, rmax=8.0, g=(2e-5, 1e-6))
-    a, b, sa, sb = _pair(build, True)
+    a, b, sa, sb = _pair(build, True, tile_cols)
     for n in (7, 10):                     # odd, then even without a readback in between
         sa.run(n)
         sb.run(n)
@@ -59,22 +59,23 @@ def test_blocked_equals_one_step_constant_tau():
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
 
 
-def test_lattice_smaller_than_a_tile():
+@pytest.mark.parametrize("tile_cols", [32, 64])
+def test_lattice_smaller_than_a_tile(tile_cols):
     build = lambda: cases.porous(NX=11, NY=30, seed=7, collision="mrt", rmin=1.5, rmax=2.5, g=(1e-5, 0.0))
-    a, b, sa, sb = _pair(build, True)
+    a, b, sa, sb = _pair(build, True, tile_cols)
     sa.run(9)
     sb.run(9)
     _same(a, b, ("f", "f2", "rho", "vel"))
 
 
-@pytest.mark.parametrize("inlet", ["bb", "neem"])
-def test_node_rules_two_steps(inlet):
+@pytest.mark.parametrize("inlet,tile_cols,NX", [("bb", 64, 23), ("neem", 64, 23), ("bb", 32, 49)])
+def test_node_rules_two_steps(inlet, tile_cols, NX):
     """Node rules (VelocityBB / VelocityNEEM inlet, PressureNEEM outlet) in
-    the two-step kernel with a two-node ring; 23 rows so the outlet row is
-    the first row of an 11-row tile and its interior neighbour lies in the
-    ring."""
-    build = lambda: cases.cylinder(NX=23, NY=70, cx=9, cy=35, r=4, collision="trt", inlet=inlet)
-    a, b, sa, sb = _pair(build, False)
+    the two-step kernel with a two-node ring; NX - 1 is a multiple of the
+    tile rows (11 rows with 64 columns, 24 with 32), so the outlet row is the
+    first row of a tile and its interior neighbour lies in the ring."""
+    build = lambda: cases.cylinder(NX=NX, NY=70, cx=9, cy=35, r=4, collision="trt", inlet=inlet)
+    a, b, sa, sb = _pair(build, False, tile_cols)
     sa.run(21)
     sb.run(21)
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
@@ -101,26 +102,27 @@ def test_large_lattice_edge_tiles():
     assert np.array_equal(a["coll"].diag.to_numpy(), b["coll"].diag.to_numpy())
 
 
-def test_large_cylinder_node_rules():
-    """137 x 21 tiles of 11 x 64 nodes with a two-node ring: VelocityBB inlet
-    and PressureNEEM outlet rows."""
-    print("case cylinder 1500x1300 trt", flush=True)
+@pytest.mark.parametrize("tile_cols", [32, 64])
+def test_large_cylinder_node_rules(tile_cols):
+    """A two-node ring over many tiles (137 x 21 of 11 x 64 nodes, or 63 x 41
+    of 24 x 32): VelocityBB inlet and PressureNEEM outlet rows."""
+    print(f"case cylinder 1500x1300 trt, {tile_cols}-column tiles", flush=True)
     build = lambda: cases.cylinder(NX=1500, NY=1300, cx=300, cy=650, r=60, collision="trt", inlet="bb")
-    a, b, sa, sb = _pair(build, False)
+    a, b, sa, sb = _pair(build, False, tile_cols)
     sa.run(9)
     sb.run(9)
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
 
 
-@pytest.mark.parametrize("coll", ["bgk", "mrt"])
-def test_pulls_from_outside_the_fluid_group(coll):
+@pytest.mark.parametrize("coll,tile_cols", [("bgk", 64), ("mrt", 32)])
+def test_pulls_from_outside_the_fluid_group(coll, tile_cols):
     """No BounceBackWall: fluid nodes next to a disk stream the disk's
     unchanged f2 (nodes marked PULLED, staged in LDS by the two-step
     kernel); the disks hold the rest equilibrium w_k."""
     print(f"case porous 150x140 {coll} without walls", flush=True)
     build = lambda: cases.porous(NX=150, NY=140, seed=9, collision=coll, rmin=3.0, rmax=7.0, g=(1e-5, 0.0),
                                  walls=False)
-    a, b, sa, sb = _pair(build, False)
+    a, b, sa, sb = _pair(build, False, tile_cols)
     w = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
     for case in (a, b):
         solid = case["lbf"].mask.arr == -1
