@@ -363,7 +363,7 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
     const R s_bulk = a.s_bulk[c];
     o.gam[c] = R(0);
     o.state[c] = state_in[c];
-    const MrtNeq<R> nq = mrt_neq(mom[c], o.rho[c], o.ux, o.uy);   // MRT only (dead code otherwise)
+    const MrtNeq<R> nq = mrt_neq(mom[c], o.rho[c], o.ux, o.uy);   // MRT, and the power-law shear rate
     if (a.tau_mode == LBM_TAU_FIELD) {
       const R state = state_in[c];
       if (is_mrt) {
@@ -374,8 +374,9 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
       }
     } else if (a.tau_mode == LBM_TAU_POWER_LAW) {
       const R state = state_in[c];
+      const R inv_tau = is_mrt ? R(0) : rcp(state);
       const R g2 = is_mrt ? mrt_shear2(nq, rcp(o.rho[c]), state, s_bulk)
-                          : shear_rate2_of<LBM_BGK, R>(f[c], o.rho[c], o.ux, o.uy, state, s_bulk);
+                          : mrt_shear2(nq, rcp(o.rho[c]), inv_tau, inv_tau);
       o.gam[c] = g2;
       const R nu = (R)a.visc_m * pow_shear2(g2, (R)a.visc_nm1);
       const R t = clamp_tau<R>(nu * R(3) + R(0.5), (R)a.tau_min, (R)a.tau_max);
@@ -869,8 +870,15 @@ __global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8)))
   const Lat& a = items[l];
   const uint32_t tiles_j = (a.ny + W - 1) / W;
   const uint32_t ti = t / tiles_j, tj = t - ti * tiles_j;
-  tile_two_steps<-1, NC, S, H, W, BUF>(a, q, a.start_step == q.step, ti * TileShape<H, W>::TI, tj * W, a.tau, nullptr,
-                                    L);
+  const uint32_t i0 = ti * TileShape<H, W>::TI, j0 = tj * W;
+  const bool local = a.start_step == q.step;
+  // the collision model is the lattice's (uniform over the block): one
+  // specialised tile path per model
+  switch (a.coll) {
+    case LBM_BGK: tile_two_steps<LBM_BGK, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
+    case LBM_TRT: tile_two_steps<LBM_TRT, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
+    default: tile_two_steps<LBM_MRT, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
+  }
 }
 
 // Persistent batched step over the 256-node tiles of many lattices.  Each XCD

@@ -127,23 +127,25 @@ __device__ __forceinline__ void mrt_collide_m(const R* m, const MrtNeq<R>& q, R 
   mrt_minv(m2, out);
 }
 
-// Squared strain-rate magnitude 2 (Sxx^2 + Syy^2 + 2 Sxy^2) of MRT
-// (CollisionEngine.py:359-377) from the non-equilibrium moments: with
-// v = S M (f - feq) (v0 = 0, v1 = s_bulk n1, v7 = s n7, v8 = s n8) the
-// reference's sums over M^-1 v are Sxx = cof (v1/6 + v7/2), Syy = cof (v1/6 -
-// v7/2), Sxy = cof v8 (column sums of M^-1), cof = -1.5 / rho.  `s` is the
-// relaxation rate of the previous step (the stored state).
+// Squared strain-rate magnitude 2 (Sxx^2 + Syy^2 + 2 Sxy^2) from the
+// non-equilibrium moments n1, n7, n8 (BGK CollisionEngine.py:80-92, TRT
+// :166-178, MRT :359-377).  The reference sums cof x {cx^2, cy^2, cx cy} over
+// a non-equilibrium part v of the populations; as moments, Sxx + Syy =
+// cof v1 / 3, Sxx - Syy = cof v7, Sxy = cof v8, so the square is
+// cof^2 (v1^2 / 9 + v7^2 + 4 v8^2).  MRT: v = S M (f - feq), i.e. v1 =
+// s_bulk n1, v7 = s n7, v8 = s n8 with s the previous step's rate, and cof =
+// -1.5 / rho; BGK / TRT: v = f - feq and cof = -1.5 / (rho tau) -- the same
+// with s = s_bulk = 1 / tau of the previous step.
 template <typename R>
 __device__ __forceinline__ R mrt_shear2(const MrtNeq<R>& q, R inv_rho, R s, R s_bulk) {
-  const R cof = -R(1.5) * inv_rho;
-  const R a = s_bulk * q.n1 * (R(1) / R(6));
-  const R b = s * q.n7 * R(0.5);
-  const R Sxx = (a + b) * cof, Syy = (a - b) * cof, Sxy = s * q.n8 * cof;
-  return R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy);
+  const R cof = R(1.5) * inv_rho;
+  const R v1 = s_bulk * q.n1 * (R(1) / R(3)), v7 = s * q.n7, v8 = s * q.n8;
+  return (cof * cof) * (v1 * v1 + v7 * v7 + R(4) * (v8 * v8));
 }
 
-// Strain-rate magnitude from a (filtered) non-equilibrium part, shared by
-// BGK/TRT (CollisionEngine.py:86-92) and MRT (:359-377).
+// Strain-rate magnitude from a (filtered) non-equilibrium part in population
+// form, as the reference sums it (BGK/TRT CollisionEngine.py:86-92, MRT
+// :359-377); the granular kernels' shear_rate.
 template <typename R>
 __device__ __forceinline__ R shear_from(const R* q, R cof) {
   const R Sxx = (q[1] + q[3] + q[5] + q[6] + q[7] + q[8]) * cof;
@@ -152,77 +154,56 @@ __device__ __forceinline__ R shear_from(const R* q, R cof) {
   return sqrt(R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy));
 }
 
+// BGK / TRT collision over the direction pairs (k, opp k) (BGK
+// CollisionEngine.py:95-103, TRT :181-200, Guo forcing :20-25).  With
+// fs, fa = (f_k +- f_opp)/2 and the matching parts of feq and of the force
+// term, es = w rho (1 - 1.5 u.u + 4.5 (c.u)^2), ea = 3 w rho c.u,
+// Fs = w (9 (c.u)(c.F) - 3 u.F), Fa = 3 w c.F:
+//   f_k'   = f_k   + S + A,   f_opp' = f_opp + S - A,
+//   S = -ws (fs - es) + (1 - ws/2) Fs,   A = -wa (fa - ea) + (1 - wa/2) Fa;
+// BGK is ws = wa = 1/tau.  Rest population: S alone with c = 0.
+template <typename R>
+__device__ __forceinline__ void pair_collide(const R (&f)[9], R (&out)[9], R rho, R ux, R uy, R Fx, R Fy,
+                                             bool force, R ws, R wa) {
+  const R A0 = R(1) - R(1.5) * (ux * ux + uy * uy);
+  const R uF3 = R(3) * (ux * Fx + uy * Fy);
+  const R hs = R(1) - R(0.5) * ws, ha = R(1) - R(0.5) * wa;
+  {
+    const R w = R(4.0 / 9.0);
+    R v = f[0] - ws * (f[0] - (w * rho) * A0);
+    if (force) v = v + hs * (w * -uF3);
+    out[0] = v;
+  }
+#pragma unroll
+  for (int k : {1, 2, 5, 6}) {
+    const int o = opp(k);
+    const R w = wgt<R>(k), wr = w * rho;
+    const R cu = (R)cx(k) * ux + (R)cy(k) * uy;
+    const R es = wr * (A0 + R(4.5) * cu * cu), ea = R(3) * wr * cu;
+    const R fs = (f[k] + f[o]) * R(0.5), fa = (f[k] - f[o]) * R(0.5);
+    R S = -ws * (fs - es), An = -wa * (fa - ea);
+    if (force) {
+      const R cF = (R)cx(k) * Fx + (R)cy(k) * Fy;
+      S = S + hs * (w * (R(9) * cu * cF - uF3));
+      An = An + ha * (R(3) * w * cF);
+    }
+    out[k] = f[k] + S + An;
+    out[o] = f[o] + S - An;
+  }
+}
+
 template <int COLL, typename R>
 __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R ux, R uy, R Fx, R Fy, bool force,
                                         R tau, R tau_anti, R s_shear, R s_bulk) {
-  const R uv = ux * ux + uy * uy;
   if constexpr (COLL == LBM_BGK) {
     const R omega = rcp(tau);
-    const R uF = ux * Fx + uy * Fy;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const R fe = feq_k(k, ux, uy, rho, uv);
-      R v = f[k] + (-omega * (f[k] - fe));
-      if (force) v = v + (R(1.0) - R(0.5) * omega) * fforce_k(k, Fx, Fy, ux, uy, uF);
-      out[k] = v;
-    }
+    pair_collide(f, out, rho, ux, uy, Fx, Fy, force, omega, omega);
   } else if constexpr (COLL == LBM_TRT) {
-    const R ws = rcp(tau), wa = rcp(tau_anti);
-    const R uF = ux * Fx + uy * Fy;
-    R fe[9], ff[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      fe[k] = feq_k(k, ux, uy, rho, uv);
-      ff[k] = force ? fforce_k(k, Fx, Fy, ux, uy, uF) : R(0.0);
-    }
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const int o = opp(k);
-      const R fs = k == 0 ? f[0] : (f[k] + f[o]) / R(2.0);
-      const R es = k == 0 ? fe[0] : (fe[k] + fe[o]) / R(2.0);
-      const R Fs = k == 0 ? ff[0] : (ff[k] + ff[o]) / R(2.0);
-      const R fa = f[k] - fs, ea = fe[k] - es, Fa = ff[k] - Fs;
-      const R coll = -ws * (fs - es) - wa * (fa - ea);
-      R v = f[k] + coll;
-      if (force) v = v + ((R(1.0) - R(0.5) * ws) * Fs + (R(1.0) - R(0.5) * wa) * Fa);
-      out[k] = v;
-    }
+    pair_collide(f, out, rho, ux, uy, Fx, Fy, force, rcp(tau), rcp(tau_anti));
   } else {
     R m[9];
     mrt_m(f, m);
     mrt_collide_m(m, mrt_neq(m, rho, ux, uy), out, ux, uy, Fx, Fy, force, s_shear, s_bulk);
-  }
-}
-
-// Power-law shear rate of one component, squared: 2 (Sxx^2 + Syy^2 + 2 Sxy^2)
-// (CollisionEngine.py:80-92, 166-178, 332-377; the rate itself is its square
-// root).  `tau_state` is the stored relaxation state of the previous step: BGK
-// tau, TRT tau_sym, MRT diag[7] (= 1/tau).
-//
-// The reference sums the non-equilibrium part over the populations:
-//   Sxx = cof sum_k cx^2 q_k,  Syy = cof sum_k cy^2 q_k,  Sxy = cof sum_k cx cy q_k
-// with q = f - feq (BGK/TRT, cof = -1.5/(rho tau)) or q = M^-1 S M (f - feq)
-// (MRT, cof = -1.5/rho).  These are evaluated here in moment form:
-//   sum_k cx^2 feq = rho (1/3 + ux^2),  sum_k cy^2 feq = rho (1/3 + uy^2),
-//   sum_k cx cy feq = rho ux uy;
-// MRT: see mrt_shear2.  The same quantities with about a tenth of the
-// arithmetic of the matrix form.
-template <int COLL, typename R>
-__device__ __forceinline__ R shear_rate2_of(const R (&f)[9], R rho, R ux, R uy, R tau_state, R s_bulk) {
-  if constexpr (COLL == LBM_MRT) {
-    R m[9];
-    mrt_m(f, m);
-    return mrt_shear2(mrt_neq(m, rho, ux, uy), rcp(rho), tau_state, s_bulk);
-  } else {
-    const R s58 = f[5] + f[6] + f[7] + f[8];
-    const R pxy = f[5] - f[6] + f[7] - f[8];
-    const R cof = -R(1.5) * rcp(rho * tau_state);
-    const R pxx = f[1] + f[3] + s58;     // sum_k cx^2 f_k
-    const R pyy = f[2] + f[4] + s58;     // sum_k cy^2 f_k
-    const R Sxx = (pxx - rho * (R(1) / R(3) + ux * ux)) * cof;
-    const R Syy = (pyy - rho * (R(1) / R(3) + uy * uy)) * cof;
-    const R Sxy = (pxy - rho * (ux * uy)) * cof;
-    return R(2) * (Sxx * Sxx + Syy * Syy + R(2) * Sxy * Sxy);
   }
 }
 
