@@ -146,7 +146,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             log(f"autotuned grid {grid}: {timings}")
         else:                  # two-step launches: the tile width is tuned
             grid = 0
-            tile_cols, timings = autotune_tiles(runner, candidates=(32, 64), steps=10)
+            tile_cols, timings = autotune_tiles(runner, candidates=(16, 32, 64), steps=10)
             log(f"autotuned tile columns {tile_cols}: {timings}")
             runner.run(10)
         g = runner.capture(graph_steps)

@@ -167,8 +167,9 @@ int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* 
  * A two-step launch flips the population / macro buffers once, so the index
  * of the buffer holding the newest f2 -- what lbm_finish takes -- is returned
  * in *out_parity.
- * tile_cols: 32 or 64 tile columns (0: the default; lattices beyond 32-bit
- * buffer offsets always use 64).  With LBM_TAU_POWER_LAW the relaxation state is
+ * tile_cols: 16, 32 or 64 tile columns (0: the default; lattices beyond
+ * 32-bit buffer offsets always use 64); the block has one thread per node of
+ * the ring-extended tile (256 to 1024).  With LBM_TAU_POWER_LAW the relaxation state is
  * double-buffered: tau_next is a second state buffer of the same size as
  * fields->tau (required, else one step per launch); the final state is left
  * in fields->tau.  Bit-identical to lbm_run.  No counterpart in the
@@ -176,9 +177,8 @@ int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* 
 int lbm_run_blocked(const lbm_lattice* lattice, const lbm_fields* fields, const lbm_neem_rule* neem,
                     int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols, void* tau_next,
                     int32_t* out_parity, void* stream);
-/* Rows of a two-step tile with `ring` = 1 or 2 and `tile_cols` = 32 or 64
- * columns (0: the library's default width); the ring-extended tile has at
- * most 1024 nodes.                                                        */
+/* Rows of a two-step tile with `ring` = 1 or 2 and `tile_cols` = 16, 32 or
+ * 64 columns (0: the library's default width).                            */
 int lbm_tile_rows(int32_t ring, int32_t tile_cols);
 
 /* The reference's boundary pass alone (BoundaryEngine.apply_boundary_conditions,
@@ -239,7 +239,7 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
  * stays valid while the mode does not change.  No counterpart in the
  * reference. */
 int lbm_batch_mode(const lbm_batch* b);
-/* Tile columns (0: default, 32 or 64) of the batch's two-step launches from
+/* Tile columns (0: default, 16, 32 or 64) of the batch's two-step launches from
  * the next commit on; refused after a graph capture.                      */
 int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols);
 int lbm_batch_run_blocked(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_blocks, int32_t* out_parity,

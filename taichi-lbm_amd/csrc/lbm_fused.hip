@@ -543,23 +543,29 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // one-step launch: a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
 constexpr int kBlock2 = 1024;
+// Threads of a two-step block (one per node of the ring-extended tile).  A
+// tile's two steps are a chain of dependent phases (code, gathers,
+// arithmetic, barrier, LDS, arithmetic, stores), so several smaller blocks
+// per CU overlap better than two large ones; the ring's share of the tile
+// sets the lower limit.  Ring 1: 256 threads for 16 columns, 512 for 32 and
+// 64; ring 2: 512 for 16 and 32 columns, 1024 for 64.
+constexpr int block2_threads(int h, int w) { return h == 1 ? (w == 16 ? 256 : 512) : (w == 64 ? 1024 : 512); }
 constexpr int kDefaultTileCols = 32;
 
-// Tile of W = 64 or 32 columns (one or half a wavefront per tile row); the
-// rows are as many as let the ring-extended tile fit one round of the block:
-// W = 64: (13 + 2) x 66 = 990 and (11 + 4) x 68 = 1020 nodes; W = 32:
-// (28 + 2) x 34 = 1020 and (24 + 4) x 36 = 1008.  Narrow tiles re-read fewer
-// ring nodes per tile node and leave fewer lanes idle at a lattice's last
-// tile column; wide ones read longer rows.
+// Tile of W = 16, 32 or 64 columns; the rows are as many as let the
+// ring-extended tile fit one round of the block (block2_threads).  Narrow
+// tiles leave fewer lanes idle at a lattice's last tile column; wide ones
+// read longer rows.
 template <int H, int W>
 struct TileShape {
-  static constexpr int TI = kBlock2 / (W + 2 * H) - 2 * H;
+  static constexpr int NT = block2_threads(H, W);   // threads of the block
+  static constexpr int TI = NT / (W + 2 * H) - 2 * H;
   static constexpr int RI = TI + 2 * H;     // ring-extended rows
   static constexpr int RJ = W + 2 * H;      // ring-extended columns
   static constexpr int RN = RI * RJ;
   static constexpr int PJ = RJ + 2;         // LDS row pitch (one pad column each side)
   static constexpr int PN = (RI + 2) * PJ;  // LDS slots (one pad row above and below)
-  static_assert(RN <= kBlock2, "ring-extended tile larger than the block");
+  static_assert(RN <= NT, "ring-extended tile larger than the block");
   // LDS slot of ring node (rr, cc), rr < RI, cc < RJ
   static __device__ __forceinline__ uint32_t slot(uint32_t rr, uint32_t cc) { return (rr + 1) * PJ + cc + 1; }
   // slot offset of x - c_k from x
@@ -762,7 +768,7 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
   R* mo = (R*)a.mac[q.parity ^ 1u];
   const LdsView<NC, S, H, W> v{&L};
-  for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * W); t += kBlock2) {
+  for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * W); t += Shape::NT) {
     const uint32_t rr = t / W, cc = t - rr * W;
     const uint32_t gi = i0 + rr, gj = j0 + cc;
     if (gi >= nx || gj >= ny) continue;
@@ -840,7 +846,7 @@ __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, b
 }
 
 template <int COLL, int NC, int S, int H, int W, bool BUF>
-__global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q) {
+__global__ __launch_bounds__(block2_threads(H, W)) void k_fused2(const Lat a, const Launch2 q) {
   __shared__ TileLds<NC, S, H, W> L;
   const uint32_t nt = q.tiles_i * q.tiles_j;
   uint32_t b = blockIdx.x;
@@ -854,17 +860,22 @@ __global__ __launch_bounds__(kBlock2) void k_fused2(const Lat a, const Launch2 q
 // lattice and tile -> index tables and the tile count in device memory; a
 // captured graph launches one block per entry of the table's capacity and
 // blocks past the current count exit, so the graph follows admissions and
-// retirements).  Blocks are remapped so that each XCD takes one contiguous
-// range of the tile list.  Power-law lattices are not batched this way
-// (their state would need a second buffer per lattice).
+// retirements).  Each XCD takes one contiguous eighth of the current tile
+// list, so the tiles spread over all XCDs whatever the capacity.  Power-law
+// lattices are not batched this way (their state would need a second buffer
+// per lattice).
 template <int NC, int S, int H, int W, bool BUF>
-__global__ __launch_bounds__(kBlock2) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
+__global__ __launch_bounds__(block2_threads(H, W)) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
     const Lat* __restrict__ items, const uint32_t* __restrict__ tile_lat, const uint32_t* __restrict__ tile_idx,
     const uint32_t* __restrict__ tile_count, const Launch2 q) {
   __shared__ TileLds<NC, S, H, W> L;
-  const uint32_t per = gridDim.x >> 3;   // the host launches a multiple of 8 blocks
-  const uint32_t w = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-  if (w >= *tile_count) return;
+  // XCD x (blocks b with b % 8 == x) takes the x-th eighth of the current
+  // tile list; the host launches a multiple of 8 blocks, at least the count
+  const uint32_t total = *tile_count, xcd = blockIdx.x & 7u;
+  const uint32_t lo = (uint32_t)(((uint64_t)total * xcd) >> 3);
+  const uint32_t hi = (uint32_t)(((uint64_t)total * (xcd + 1)) >> 3);
+  const uint32_t w = lo + (blockIdx.x >> 3);
+  if (w >= hi) return;
   const uint32_t l = __builtin_amdgcn_readfirstlane(tile_lat[w]);
   const uint32_t t = __builtin_amdgcn_readfirstlane(tile_idx[w]);
   const Lat& a = items[l];
@@ -1147,7 +1158,7 @@ struct lbm_batch {
   size_t tile2_cap = 0;
   uint32_t tiles2 = 0;
   int mode2 = 0;                     // 0: one step per launch, else the ring width H
-  int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (32 or 64)
+  int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (16, 32 or 64)
   std::vector<uint32_t> host_t2lat, host_t2idx;
   bool captured = false;             // a graph holds the table addresses: no reallocation
 };
@@ -1217,13 +1228,15 @@ bool fits_buffer(const lbm_lattice* lat) {
 // 64-column tiles.
 Kernel2Fn pick2(int nc, int store, int coll, int h, int w, bool buf) {
   if (!buf) return h == 1 ? pick2_h<1, 64, false>(nc, store, coll) : pick2_h<2, 64, false>(nc, store, coll);
+  if (w == 16) return h == 1 ? pick2_h<1, 16, true>(nc, store, coll) : pick2_h<2, 16, true>(nc, store, coll);
   if (w == 32) return h == 1 ? pick2_h<1, 32, true>(nc, store, coll) : pick2_h<2, 32, true>(nc, store, coll);
   return h == 1 ? pick2_h<1, 64, true>(nc, store, coll) : pick2_h<2, 64, true>(nc, store, coll);
 }
-// tile columns: 32 or 64 as asked, else the default
-int tile_cols_of(int32_t asked) { return asked == 32 || asked == 64 ? asked : kDefaultTileCols; }
+bool valid_tile_cols(int32_t w) { return w == 0 || w == 16 || w == 32 || w == 64; }
+// tile columns: 16, 32 or 64 as asked, else the default
+int tile_cols_of(int32_t asked) { return asked == 16 || asked == 32 || asked == 64 ? asked : kDefaultTileCols; }
 uint32_t tile_rows(int h, int w) {
-  return (uint32_t)(kBlock2 / (w + 2 * h) - 2 * h);
+  return (uint32_t)(block2_threads(h, w) / (w + 2 * h) - 2 * h);
 }
 int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
 }  // namespace lbm
@@ -1240,7 +1253,7 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   hipStream_t s = (hipStream_t)stream;
   const bool pl = lat->tau_mode == LBM_TAU_POWER_LAW;
   const int h = ring_width(lat);
-  if (tile_cols != 0 && tile_cols != 32 && tile_cols != 64) return fail("tile_cols must be 0, 32 or 64");
+  if (!valid_tile_cols(tile_cols)) return fail("tile_cols must be 0, 16, 32 or 64");
   const bool buf = fits_buffer(lat);
   const int w = buf ? tile_cols_of(tile_cols) : 64;
   Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, w, buf);
@@ -1269,7 +1282,7 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
     q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
     q2.tau_in = tcur;
     q2.tau_out = tnext;
-    hipLaunchKernelGGL(two, dim3((unsigned)nt), dim3(kBlock2), 0, s, a, q2);
+    hipLaunchKernelGGL(two, dim3((unsigned)nt), dim3(block2_threads(h, w)), 0, s, a, q2);
     if ((rc = hip_check(hipGetLastError(), "two-step launch"))) return rc;
     if (pl) {
       void* w = tnext;
@@ -1529,6 +1542,7 @@ template <int NC, int S>
 Batch2Fn pick_batch2_s(int h, int w) {
   // batched lattices are small: buffer addressing (each lattice's segment is
   // checked against 2^31 bytes at commit)
+  if (w == 16) return h == 1 ? lbm::k_batch2<NC, S, 1, 16, true> : lbm::k_batch2<NC, S, 2, 16, true>;
   if (w == 32) return h == 1 ? lbm::k_batch2<NC, S, 1, 32, true> : lbm::k_batch2<NC, S, 2, 32, true>;
   return h == 1 ? lbm::k_batch2<NC, S, 1, 64, true> : lbm::k_batch2<NC, S, 2, 64, true>;
 }
@@ -1564,7 +1578,9 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
       q2.parity = p;
       q2.step = t;
       q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
-      hipLaunchKernelGGL(two, dim3(g2), dim3(kBlock2), 0, s, b->items, b->tile2_lat, b->tile2_idx, b->tile2_count, q2);
+      hipLaunchKernelGGL(two, dim3(g2), dim3(block2_threads(b->mode2, b->tile_cols)), 0, s, b->items, b->tile2_lat,
+                         b->tile2_idx,
+                         b->tile2_count, q2);
       int rc = hip_check(hipGetLastError(), "batched two-step launch");
       if (rc) return rc;
       p ^= 1u;
@@ -1587,7 +1603,7 @@ extern "C" int lbm_batch_mode(const lbm_batch* b) { return b ? b->mode2 : -1; }
 extern "C" int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols) {
   using namespace lbm;
   if (!b) return fail("null batch");
-  if (tile_cols != 0 && tile_cols != 32 && tile_cols != 64) return fail("tile_cols must be 0, 32 or 64");
+  if (!valid_tile_cols(tile_cols)) return fail("tile_cols must be 0, 16, 32 or 64");
   const int w = tile_cols_of(tile_cols);
   if (w == b->tile_cols) return LBM_OK;
   if (b->captured) return fail("cannot change the tile shape after a graph capture");
@@ -1597,8 +1613,8 @@ extern "C" int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols) {
 
 extern "C" int lbm_tile_rows(int32_t ring, int32_t tile_cols) {
   using namespace lbm;
-  if ((ring != 1 && ring != 2) || (tile_cols != 0 && tile_cols != 32 && tile_cols != 64))
-    return fail("lbm_tile_rows: ring 1 or 2, tile_cols 0, 32 or 64");
+  if ((ring != 1 && ring != 2) || !valid_tile_cols(tile_cols))
+    return fail("lbm_tile_rows: ring 1 or 2, tile_cols 0, 16, 32 or 64");
   return (int)tile_rows(ring, tile_cols_of(tile_cols));
 }
 

@@ -79,8 +79,8 @@ def footprint_bytes(spec_or_n, storage="f32") -> int:
 
 
 def two_step_tiles(specs, tile_cols: int = 0) -> int:
-    """Tiles of the two-step launch for these lattices: tile_cols = 32 or 64
-    columns (0: the library's default) by lbm_tile_rows rows (ring width 2
+    """Tiles of the two-step launch for these lattices: tile_cols = 16, 32 or
+    64 columns (0: the library's default) by lbm_tile_rows rows (ring width 2
     when any lattice has node rules)."""
     ring = 2 if any(len(s.neem) for s in specs) else 1
     tj = tile_cols or 32
@@ -169,8 +169,8 @@ class BatchRunner:
         self.tile_cols = int(tile_cols)
         H.check(H.lib().lbm_batch_tile_cols(h, self.tile_cols), "lbm_batch_tile_cols")
         if self.blocked and specs:
-            # room for either tile width, so that autotune_tiles can switch
-            need = max(two_step_tiles(specs, 32), two_step_tiles(specs, 64))
+            # room for any tile width, so that autotune_tiles can switch
+            need = max(two_step_tiles(specs, w) for w in (16, 32, 64))
             H.check(H.lib().lbm_batch_reserve2(h, need + 64), "lbm_batch_reserve2")
         self.slots: Dict[int, dict] = {}
         self.parity = 0
@@ -297,7 +297,7 @@ class BatchRunner:
             self._dirty = True
 
     def set_tile_cols(self, tile_cols: int):
-        """Two-step tile width (32 or 64; 0: default) from the next commit on."""
+        """Two-step tile width (16, 32 or 64; 0: default) from the next commit on."""
         H.check(H.lib().lbm_batch_tile_cols(self.h, int(tile_cols)), "lbm_batch_tile_cols")
         self.tile_cols = int(tile_cols)
         self._dirty = True
@@ -432,7 +432,7 @@ class BatchGraph:
             pass
 
 
-def autotune_tiles(runner: BatchRunner, candidates=(32, 64), steps: int = 10):
+def autotune_tiles(runner: BatchRunner, candidates=(16, 32, 64), steps: int = 10):
     """Times the two-step tile width on the runner's own batch (steps a
     multiple of 2) and keeps the fastest; before any graph capture.  Advances
     the batch by len(candidates) * (steps + 2) steps."""

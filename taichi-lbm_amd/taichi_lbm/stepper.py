@@ -43,7 +43,7 @@ class Stepper:
                  boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0):
         """blocked: run two steps per launch where the lattice allows it
         (lbm_run_blocked; bit-identical to one step per launch), on tiles of
-        tile_cols = 32 or 64 columns (0: the library's default)."""
+        tile_cols = 16, 32 or 64 columns (0: the library's default)."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
         self.tile_cols = int(tile_cols)

@@ -148,7 +148,7 @@ def test_autotune_picks_a_candidate():
     assert best in timings and best % 8 == 0
 
 
-@pytest.mark.parametrize("tile_cols", [32, 64])
+@pytest.mark.parametrize("tile_cols", [16, 32, 64])
 def test_blocked_batch_equals_one_step_batch(tile_cols):
     """Two-step batched launches (ring width 2: the channels have node rules)
     give the fields of one-step batched launches, bit for bit, with either
@@ -180,9 +180,9 @@ def test_blocked_batch_without_rules_and_graph():
     assert a.mode == 1
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
-        best, timings = autotune_tiles(a, candidates=(32, 64), steps=2)   # switches widths before capture
-        assert best in (32, 64) and set(timings) == {32, 64}
-        b.run(2 * (2 + 2))
+        best, timings = autotune_tiles(a, candidates=(16, 32, 64), steps=2)   # switches widths before capture
+        assert best in (16, 32, 64) and set(timings) == {16, 32, 64}
+        b.run(3 * (2 + 2))
         a.run(2)
         b.run(2)
         g = a.capture(12)
