@@ -85,11 +85,13 @@ __device__ __forceinline__ R clamp_tau(R t, R tmin, R tmax) {
 }
 
 // ---- buffer addressing ------------------------------------------------------
-// A raw buffer resource over a range of fewer than 2^31 bytes: loads and
+// A raw buffer resource over a range of fewer than 2^32 bytes: loads and
 // stores then take a 32-bit per-lane offset plus a wave-uniform (scalar)
 // offset instead of a 64-bit address per lane, and accesses past the range
 // read 0 / are dropped by the hardware.
 using Rsrc = __amdgpu_buffer_rsrc_t;
+// raw buffer of `bytes` bytes (< 2^32; the record count is an unsigned
+// 32-bit field, passed as its bit pattern)
 __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }

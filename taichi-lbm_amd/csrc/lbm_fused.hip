@@ -635,7 +635,7 @@ __device__ __forceinline__ void ring_coords(uint32_t rr, uint32_t cc, uint32_t i
 }
 
 // Pull gather of one node with buffer addressing (the population buffer spans
-// fewer than 2^31 bytes; nx, ny < 2^24).  Byte offsets: plane p at node m is
+// fewer than 2^32 bytes; nx, ny < 2^24).  Byte offsets: plane p at node m is
 // p ps B + m B.  Population k pulls node x - c_k of plane k, or, bounced,
 // node x of plane opp(k); both are addressed from the lower of the two planes
 // (a scalar offset), the per-lane part selected by the bounce bit.
@@ -668,8 +668,8 @@ __device__ __forceinline__ void gather_buf(const Rsrc& rs, uint32_t ps, uint32_t
   }
 }
 
-// Phase A loads of one active ring node.  BUF: buffer addressing (32-bit
-// offsets; see gather_buf); otherwise 64-bit addresses.
+// Phase A loads of one active ring node.  BUF: buffer addressing (unsigned
+// 32-bit offsets; see gather_buf); otherwise 64-bit addresses.
 template <int NC, int S, bool BUF>
 __device__ __forceinline__ void ring_load(const Lat& a, bool local, const typename Pop<S>::T* __restrict__ src,
                                           const Rsrc& rsrc, const typename Pop<S>::R* tau_a, uint32_t gi,
@@ -1213,11 +1213,12 @@ Kernel2Fn pick2_h(int nc, int store, int coll) {
     default: return nullptr;
   }
 }
-// buffer addressing (32-bit offsets, 24-bit row products) when the
-// population buffer spans fewer than 2^31 bytes and both sides are below 2^24
+// buffer addressing (unsigned 32-bit offsets and record count, 24-bit row
+// products) when the population buffer spans fewer than 2^32 bytes and both
+// sides are below 2^24
 bool fits_buffer(uint64_t nc, uint64_t nx, uint64_t ny, uint64_t ps, int storage) {
   const uint64_t t = storage == LBM_STORE_F32 ? 4 : (storage == LBM_STORE_F64 ? 8 : 2);
-  return nc * 9ull * ps * t < (1ull << 31) && nx < (1ull << 24) && ny < (1ull << 24);
+  return nc * 9ull * ps * t < (1ull << 32) && nx < (1ull << 24) && ny < (1ull << 24);
 }
 bool fits_buffer(const lbm_lattice* lat) {
   return fits_buffer(lat->nc, lat->nx, lat->ny, lat->plane_stride, lat->storage);
@@ -1541,7 +1542,7 @@ using Batch2Fn = void (*)(const lbm::Lat*, const uint32_t*, const uint32_t*, con
 template <int NC, int S>
 Batch2Fn pick_batch2_s(int h, int w) {
   // batched lattices are small: buffer addressing (each lattice's segment is
-  // checked against 2^31 bytes at commit)
+  // checked against 2^32 bytes at commit)
   if (w == 16) return h == 1 ? lbm::k_batch2<NC, S, 1, 16, true> : lbm::k_batch2<NC, S, 2, 16, true>;
   if (w == 32) return h == 1 ? lbm::k_batch2<NC, S, 1, 32, true> : lbm::k_batch2<NC, S, 2, 32, true>;
   return h == 1 ? lbm::k_batch2<NC, S, 1, 64, true> : lbm::k_batch2<NC, S, 2, 64, true>;
