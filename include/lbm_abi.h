@@ -214,7 +214,10 @@ int lbm_batch_tiles(const lbm_batch* b);
 int lbm_batch_capacity(const lbm_batch* b);
 /* Reserve room for `tiles` tiles in the one-step tile table, or `tiles` in
  * the two-step table (reserve2), before capturing a graph: after a capture a
- * commit that needs more room fails instead of moving the tables.          */
+ * commit that needs more room fails instead of moving the tables.  reserve2
+ * also shrinks the two-step table (to no fewer entries than the current
+ * tile count) while no graph holds it: a captured graph launches one block
+ * per entry of the capacity.                                               */
 int lbm_batch_reserve(lbm_batch* b, int32_t tiles);
 int lbm_batch_reserve2(lbm_batch* b, int32_t tiles);
 /* nsteps launches (launch t reads parity (parity + t) % 2; the last writes
@@ -229,9 +232,13 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
                     void** graph_exec);
 /* Two steps per launch for the whole batch where it allows it (as
  * lbm_run_blocked: every active lattice without power law, storage other
- * than FP64; ring width 2 if any lattice has node rules).  lbm_batch_mode
- * reports what the last commit allows: 0 = one step per launch, 1 or 2 =
- * two-step launches with that ring width.  A two-step launch flips the
+ * than FP64).  lbm_batch_mode reports what the last commit chose: 0 = one
+ * step per launch; 1 = two-step launches with a one-node ring (no node
+ * rules); 2 = a two-node ring (node rules); 3 = node rules with a one-node
+ * ring: each lattice's tile grid is shifted (found at commit by scanning the
+ * rule nodes) so that every NEEM / OpenNeumann node shares its tile with the
+ * interior neighbour it reads.  lbm_batch_ring(b, 2) forces mode 2 instead of
+ * 3 (0 = automatic); lbm_batch_tiles2 is the two-step tile count.  A two-step launch flips the
  * buffers once; the parity of the buffer with the newest f2 is returned in
  * *out_parity.  grid_blocks applies to the one-step launches only; a
  * two-step launch has one block per tile (per entry of the table's capacity
@@ -239,6 +246,8 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
  * stays valid while the mode does not change.  No counterpart in the
  * reference. */
 int lbm_batch_mode(const lbm_batch* b);
+int lbm_batch_ring(lbm_batch* b, int32_t ring);
+int lbm_batch_tiles2(const lbm_batch* b);
 /* Tile columns (0: default, 16, 32 or 64) of the batch's two-step launches from
  * the next commit on; refused after a graph capture.                      */
 int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols);

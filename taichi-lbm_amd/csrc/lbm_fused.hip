@@ -33,6 +33,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,8 @@ struct Lat {
   int32_t start_step;    // batched runs: launch index whose populations are read in place
   uint32_t block0;       // batched runs: first 256-node tile of this lattice
   uint32_t active;
+  uint32_t tile_oi, tile_oj;   // batched two-step runs: shift of the tile grid (tile (ti, tj) starts at
+                               // row ti*TI - tile_oi, column tj*W - tile_oj)
   float tau0[LBM_MAX_NC], tau_anti0[LBM_MAX_NC], magic[LBM_MAX_NC];
   float s_shear[LBM_MAX_NC], s_bulk[LBM_MAX_NC];
   float visc_m, visc_nm1, tau_min, tau_max, gx, gy;
@@ -543,6 +546,12 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 // one-step launch: a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
 constexpr int kBlock2 = 1024;
+// Two-step mode M: 1 = one-node ring, no node rules; 2 = two-node ring with
+// node rules (a rule node's interior neighbour may lie in the ring); 3 =
+// one-node ring with node rules (every such neighbour lies in its rule
+// node's tile: batches shift a lattice's tile grid to make it so).
+constexpr int ring_of(int m) { return m == 2 ? 2 : 1; }
+constexpr bool rules_of(int m) { return m != 1; }
 // Threads of a two-step block (one per node of the ring-extended tile).  A
 // tile's two steps are a chain of dependent phases (code, gathers,
 // arithmetic, barrier, LDS, arithmetic, stores), so several smaller blocks
@@ -572,24 +581,26 @@ struct TileShape {
   static constexpr int back(int k) { return cx(k) * PJ + cy(k); }
 };
 
-template <int NC, int S, int H, int W>
+template <int NC, int S, int M, int W>
 struct TileLds {
   using R = typename Pop<S>::R;
+  static constexpr int H = ring_of(M);
   static constexpr int PN = TileShape<H, W>::PN;
   R f[NC * 9][PN];                      // what each node's pull of step n+1 receives (see above)
-  R mac[H == 2 ? NC + 3 : 1][H == 2 ? PN : 1];   // rho_c, total_rho, ux, uy of step n (node rules)
+  R mac[rules_of(M) ? NC + 3 : 1][rules_of(M) ? PN : 1];   // rho_c, total_rho, ux, uy of step n (node rules)
   R st[NC][PN];                         // relaxation state after step n
   uint16_t code[PN];
 };
 
 // Step n as phase B reads it: the LDS tile, nodes addressed by their slot.
-template <int NC, int S, int H, int W>
+template <int NC, int S, int M, int W>
 struct LdsView {
+  static constexpr int H = ring_of(M);
   using R = typename Pop<S>::R;
   struct Node {
     uint32_t q;
   };
-  const TileLds<NC, S, H, W>* L;
+  const TileLds<NC, S, M, W>* L;
   __device__ __forceinline__ Node back(const Node& x, int di, int dj) const {
     return Node{(uint32_t)((int)x.q - di * TileShape<H, W>::PJ - dj)};
   }
@@ -693,14 +704,15 @@ __device__ __forceinline__ void ring_load(const Lat& a, bool local, const typena
 // Phase A arithmetic of one active ring node: node rules (reading step n-1's
 // state in HBM), step n, results into LDS slot q.  The populations that the
 // node bounces are returned in `refl` for the reflection writes.
-template <int COLL, int NC, int S, int H, int W>
+template <int COLL, int NC, int S, int M, int W>
 __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typename Pop<S>::T* __restrict__ src,
                                           uint32_t parity, uint32_t q, uint32_t gi, uint32_t gj, uint32_t code,
-                                          RingIn<NC, S>& in, TileLds<NC, S, H, W>& L,
+                                          RingIn<NC, S>& in, TileLds<NC, S, M, W>& L,
                                           typename Pop<S>::R (&refl)[NC][9]) {
+  constexpr int H = ring_of(M);
   using R = typename Pop<S>::R;
   const uint32_t n = gi * a.ny + gj;
-  if constexpr (H == 2) {
+  if constexpr (rules_of(M)) {
     const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
     if (e && !local) {
       const GlobalView<S> v{src, (const R*)a.mac[parity], a.code, a.ps, a.nx, a.ny};
@@ -718,7 +730,7 @@ __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typena
     }
     L.st[c][q] = o.state[c];
   }
-  if constexpr (H == 2) {
+  if constexpr (rules_of(M)) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) L.mac[c][q] = o.rho[c];
     L.mac[NC][q] = o.rtot;
@@ -730,9 +742,10 @@ __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typena
 // Reflections of one active ring node (after the barrier that follows the
 // own writes): population k that the node bounces goes to slot q - c_k of
 // plane k, the slot its pull reads.
-template <int NC, int S, int H, int W>
+template <int NC, int S, int M, int W>
 __device__ __forceinline__ void ring_reflect(uint32_t q, uint32_t code, const typename Pop<S>::R (&refl)[NC][9],
-                                             TileLds<NC, S, H, W>& L) {
+                                             TileLds<NC, S, M, W>& L) {
+  constexpr int H = ring_of(M);
 #pragma unroll
   for (int k = 1; k < 9; ++k) {
     if (code & (1u << (k - 1))) {
@@ -744,9 +757,10 @@ __device__ __forceinline__ void ring_reflect(uint32_t q, uint32_t code, const ty
 
 // A node outside the fluid group that an active node pulls: its unchanged f2
 // (the stored value in HBM, as the one-step kernel reads it) into its slot.
-template <int NC, int S, int H, int W>
+template <int NC, int S, int M, int W>
 __device__ __forceinline__ void ring_pulled(const Lat& a, const typename Pop<S>::T* __restrict__ stale, uint32_t q,
-                                            uint32_t n, TileLds<NC, S, H, W>& L) {
+                                            uint32_t n, TileLds<NC, S, M, W>& L) {
+  constexpr int H = ring_of(M);
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -756,9 +770,10 @@ __device__ __forceinline__ void ring_pulled(const Lat& a, const typename Pop<S>:
 // Phase B: step n+1 on the tile from LDS; results to pop[parity^1] (and the
 // relaxation state to tau_out under the power law, macroscopic fields when
 // the run ends).
-template <int COLL, int NC, int S, int H, int W, bool BUF>
+template <int COLL, int NC, int S, int M, int W, bool BUF>
 __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q, uint32_t i0, uint32_t j0,
-                                                 void* tau_out, const TileLds<NC, S, H, W>& L) {
+                                                 void* tau_out, const TileLds<NC, S, M, W>& L) {
+  constexpr int H = ring_of(M);
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
   using Shape = TileShape<H, W>;
@@ -767,13 +782,13 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
   T* __restrict__ dst = (T*)a.pop[q.parity ^ 1u];
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
   R* mo = (R*)a.mac[q.parity ^ 1u];
-  const LdsView<NC, S, H, W> v{&L};
+  const LdsView<NC, S, M, W> v{&L};
   for (uint32_t t = threadIdx.x; t < (uint32_t)(Shape::TI * W); t += Shape::NT) {
     const uint32_t rr = t / W, cc = t - rr * W;
     const uint32_t gi = i0 + rr, gj = j0 + cc;
     if (gi >= nx || gj >= ny) continue;
     const uint32_t n = gi * ny + gj;
-    const typename LdsView<NC, S, H, W>::Node x{Shape::slot(rr + H, cc + H)};
+    const typename LdsView<NC, S, M, W>::Node x{Shape::slot(rr + H, cc + H)};
     const uint32_t code = L.code[x.q];
     if (!(code & LBM_CODE_ACTIVE)) {
       if (q.write_macro) {
@@ -787,7 +802,7 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
     for (int c = 0; c < NC; ++c)
 #pragma unroll
       for (int k = 0; k < 9; ++k) f[c][k] = v.pop(x, 0u, c, k);
-    if constexpr (H == 2) {
+    if constexpr (rules_of(M)) {
       const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
       if (e) apply_rule<NC, S>(a, v, a.neem[e - 1], x, n, f, nullptr);
     }
@@ -812,9 +827,14 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
   }
 }
 
-template <int COLL, int NC, int S, int H, int W, bool BUF>
+template <int COLL, int NC, int S, int M, int W, bool BUF>
+// codes / cnx / cny: the lattice's node codes and sides for the tile's first
+// loads (the same as a.code, a.nx, a.ny; a batch passes them from the tile
+// record so those loads do not wait for the lattice's descriptor).
 __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
-                                               const void* tau_in, void* tau_out, TileLds<NC, S, H, W>& L) {
+                                               const void* tau_in, void* tau_out, TileLds<NC, S, M, W>& L,
+                                               const uint16_t* __restrict__ codes, uint32_t cnx, uint32_t cny) {
+  constexpr int H = ring_of(M);
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
   using Shape = TileShape<H, W>;
@@ -828,33 +848,46 @@ __device__ __forceinline__ void tile_two_steps(const Lat& a, const Launch2& q, b
   R refl[NC][9];
   if (r1 < (uint32_t)Shape::RN) {
     uint32_t gi, gj;
-    ring_coords<H>(rr, cc, i0, j0, a.nx, a.ny, gi, gj);
-    code = a.code[gi * a.ny + gj];
+    ring_coords<H>(rr, cc, i0, j0, cnx, cny, gi, gj);
+    code = codes[gi * cny + gj];
     L.code[qs] = (uint16_t)code;
     if (code & LBM_CODE_ACTIVE) {
       RingIn<NC, S> in;
       ring_load<NC, S, BUF>(a, local, src, rsrc, tau_a, gi, gj, code, in);
-      ring_step<COLL, NC, S, H, W>(a, local, src, q.parity, qs, gi, gj, code, in, L, refl);
+      ring_step<COLL, NC, S, M, W>(a, local, src, q.parity, qs, gi, gj, code, in, L, refl);
     } else if (code & LBM_CODE_PULLED) {
-      ring_pulled<NC, S, H, W>(a, (const T*)a.pop[q.parity ^ 1u], qs, gi * a.ny + gj, L);
+      ring_pulled<NC, S, M, W>(a, (const T*)a.pop[q.parity ^ 1u], qs, gi * a.ny + gj, L);
     }
   }
   __syncthreads();
-  if ((code & (LBM_CODE_ACTIVE | LBM_CODE_BOUNCE_MASK)) > LBM_CODE_ACTIVE) ring_reflect<NC, S, H, W>(qs, code, refl, L);
+  if ((code & (LBM_CODE_ACTIVE | LBM_CODE_BOUNCE_MASK)) > LBM_CODE_ACTIVE) ring_reflect<NC, S, M, W>(qs, code, refl, L);
   __syncthreads();
-  tile_second_step<COLL, NC, S, H, W, BUF>(a, q, i0, j0, tau_out, L);
+  tile_second_step<COLL, NC, S, M, W, BUF>(a, q, i0, j0, tau_out, L);
 }
 
-template <int COLL, int NC, int S, int H, int W, bool BUF>
-__global__ __launch_bounds__(block2_threads(H, W)) void k_fused2(const Lat a, const Launch2 q) {
-  __shared__ TileLds<NC, S, H, W> L;
+template <int COLL, int NC, int S, int M, int W, bool BUF>
+__global__ __launch_bounds__(block2_threads(ring_of(M), W)) void k_fused2(const Lat a, const Launch2 q) {
+  constexpr int H = ring_of(M);
+  __shared__ TileLds<NC, S, M, W> L;
   const uint32_t nt = q.tiles_i * q.tiles_j;
   uint32_t b = blockIdx.x;
   if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
   const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  tile_two_steps<COLL, NC, S, H, W, BUF>(a, q, q.local_read != 0, ti * TileShape<H, W>::TI, tj * W, q.tau_in,
-                                      q.tau_out, L);
+  tile_two_steps<COLL, NC, S, M, W, BUF>(a, q, q.local_read != 0, ti * TileShape<H, W>::TI, tj * W, q.tau_in,
+                                         q.tau_out, L, a.code, a.nx, a.ny);
 }
+
+// One tile of a batched two-step launch (32 bytes): what the tile's first
+// loads need, so that the node codes are fetched together with the
+// lattice's descriptor instead of after it.
+struct Tile2 {
+  const uint16_t* code;   // the lattice's node codes
+  uint32_t l;             // lattice index (kEmptyTile: no tile)
+  uint32_t i0, j0;        // first row / column (shifted grids wrap below zero)
+  uint32_t nx, ny;
+  uint32_t pad;
+};
+constexpr uint32_t kEmptyTile = 0xFFFFFFFFu;
 
 // Batched two-step launch: one block per tile of many lattices (tile ->
 // lattice and tile -> index tables and the tile count in device memory; a
@@ -863,32 +896,58 @@ __global__ __launch_bounds__(block2_threads(H, W)) void k_fused2(const Lat a, co
 // retirements).  Each XCD takes one contiguous eighth of the current tile
 // list, so the tiles spread over all XCDs whatever the capacity.  Power-law
 // lattices are not batched this way (their state would need a second buffer
-// per lattice).
-template <int NC, int S, int H, int W, bool BUF>
-__global__ __launch_bounds__(block2_threads(H, W)) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
-    const Lat* __restrict__ items, const uint32_t* __restrict__ tile_lat, const uint32_t* __restrict__ tile_idx,
-    const uint32_t* __restrict__ tile_count, const Launch2 q) {
-  __shared__ TileLds<NC, S, H, W> L;
-  // XCD x (blocks b with b % 8 == x) takes the x-th eighth of the current
-  // tile list; the host launches a multiple of 8 blocks, at least the count
-  const uint32_t total = *tile_count, xcd = blockIdx.x & 7u;
-  const uint32_t lo = (uint32_t)(((uint64_t)total * xcd) >> 3);
-  const uint32_t hi = (uint32_t)(((uint64_t)total * (xcd + 1)) >> 3);
-  const uint32_t w = lo + (blockIdx.x >> 3);
-  if (w >= hi) return;
-  const uint32_t l = __builtin_amdgcn_readfirstlane(tile_lat[w]);
-  const uint32_t t = __builtin_amdgcn_readfirstlane(tile_idx[w]);
-  const Lat& a = items[l];
-  const uint32_t tiles_j = (a.ny + W - 1) / W;
-  const uint32_t ti = t / tiles_j, tj = t - ti * tiles_j;
-  const uint32_t i0 = ti * TileShape<H, W>::TI, j0 = tj * W;
+// per lattice).  Each entry carries the tile's origin and the lattice's code
+// pointer and sides; the table is striped by XCD with empty entries, so no
+// tile count has to be read before the tile's own loads.
+template <int NC, int S, int M, int W, bool BUF>
+__global__ __launch_bounds__(block2_threads(ring_of(M), W)) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch2(
+    const Lat* __restrict__ items, const Tile2* __restrict__ tiles, uint32_t stride, const Launch2 q) {
+  __shared__ TileLds<NC, S, M, W> L;
+  // the table holds 8 stripes of `stride` entries, stripe x the x-th eighth
+  // of the tile list (the rest empty); blocks b with b % 8 == x run on XCD x
+  const Tile2 tr = tiles[(blockIdx.x & 7u) * stride + (blockIdx.x >> 3)];
+  if (tr.l == kEmptyTile) return;
+  const Lat& a = items[tr.l];
   const bool local = a.start_step == q.step;
   // the collision model is the lattice's (uniform over the block): one
   // specialised tile path per model
   switch (a.coll) {
-    case LBM_BGK: tile_two_steps<LBM_BGK, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
-    case LBM_TRT: tile_two_steps<LBM_TRT, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
-    default: tile_two_steps<LBM_MRT, NC, S, H, W, BUF>(a, q, local, i0, j0, a.tau, nullptr, L); break;
+    case LBM_BGK:
+      tile_two_steps<LBM_BGK, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      break;
+    case LBM_TRT:
+      tile_two_steps<LBM_TRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      break;
+    default:
+      tile_two_steps<LBM_MRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      break;
+  }
+}
+
+// Tile-grid shifts that keep every rule node's interior neighbour in its own
+// tile (two-step mode 3).  For each active rule node that reads x - c_d
+// (VelocityNEEM, PressureNEEM, OpenNeumann), the one row shift (d = 1, 3;
+// tiles of ti rows) or column shift (d = 2, 4; w columns) that would put x
+// on the tile edge facing its neighbour is marked forbidden in
+// masks[3l] (rows) and masks[3l + 1..2] (columns, 64 bits).
+__global__ void k_rule_shifts(const Lat* __restrict__ items, uint32_t ti, uint32_t w, uint32_t* masks) {
+  const uint32_t l = blockIdx.y;
+  const Lat& a = items[l];
+  if (!a.active || a.n_neem == 0) return;
+  for (uint32_t n = blockIdx.x * blockDim.x + threadIdx.x; n < a.N; n += gridDim.x * blockDim.x) {
+    const uint32_t code = a.code[n];
+    const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
+    if (!e || !(code & LBM_CODE_ACTIVE)) continue;
+    const int kind = a.neem[e - 1].kind, d = a.neem[e - 1].direction;
+    if (kind == LBM_RULE_VELOCITY_BB) continue;   // reads its own node only
+    const uint32_t i = n / a.ny, j = n - i * a.ny;
+    if (d == 1 || d == 3) {
+      const uint32_t o = d == 1 ? (ti - i % ti) % ti : (2 * ti - 1 - i % ti) % ti;   // x first / last row
+      atomicOr(&masks[3 * l], 1u << o);
+    } else {
+      const uint32_t o = d == 2 ? (w - j % w) % w : (2 * w - 1 - j % w) % w;        // x first / last column
+      atomicOr(&masks[3 * l + 1 + (o >> 5)], 1u << (o & 31u));
+    }
   }
 }
 
@@ -1152,16 +1211,51 @@ struct lbm_batch {
   uint32_t tiles = 0;
   std::vector<uint32_t> host_tiles;
   // two-step launches: (lattice, tile) tables over the ring-extended tiling
-  uint32_t* tile2_lat = nullptr;
-  uint32_t* tile2_idx = nullptr;
-  uint32_t* tile2_count = nullptr;
-  size_t tile2_cap = 0;
+  lbm::Tile2* tile2 = nullptr;       // device: 8 stripes of tile2_stride entries
+  size_t tile2_stride = 0;
+  size_t tile2_cap = 0;              // 8 * tile2_stride
   uint32_t tiles2 = 0;
   int mode2 = 0;                     // 0: one step per launch, else the ring width H
   int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (16, 32 or 64)
-  std::vector<uint32_t> host_t2lat, host_t2idx;
+  int ring_pref = 0;                 // 0: one-node ring with rules where the tile grids allow it; 2: two-node
+  uint32_t* shift_masks = nullptr;   // device: 3 words per lattice (k_rule_shifts)
+  int shift_cap = 0;
+  std::vector<lbm::Tile2> host_t2, host_t2lay;   // tile list; its striped layout (uploaded)
   bool captured = false;             // a graph holds the table addresses: no reallocation
 };
+
+namespace {
+// two-step table with room for at least `tiles` tiles (8 stripes); the
+// contents are not kept
+int tiles2_alloc(lbm_batch* b, size_t tiles) {
+  using namespace lbm;
+  const size_t stride = std::max<size_t>(1, (tiles + 7) / 8);
+  Tile2* t = nullptr;
+  int rc = hip_check(hipMalloc((void**)&t, sizeof(Tile2) * 8 * stride), "hipMalloc(tile2 table)");
+  if (rc) return rc;
+  (void)hipFree(b->tile2);
+  b->tile2 = t;
+  b->tile2_stride = stride;
+  b->tile2_cap = 8 * stride;
+  return LBM_OK;
+}
+// lays the tile list out in the striped table (stripe x: the x-th eighth of
+// the list, then empty entries) and uploads it, asynchronously on `s`:
+// host_t2lay stays untouched until the stream has passed the copy
+int tiles2_upload(lbm_batch* b, hipStream_t s) {
+  using namespace lbm;
+  const size_t st = b->tile2_stride, n = b->tiles2;
+  Tile2 empty{};
+  empty.l = kEmptyTile;
+  b->host_t2lay.assign(8 * st, empty);
+  for (size_t x = 0; x < 8; ++x) {
+    const size_t lo = n * x / 8, hi = n * (x + 1) / 8;
+    for (size_t k = lo; k < hi; ++k) b->host_t2lay[x * st + (k - lo)] = b->host_t2[k];
+  }
+  return hip_check(hipMemcpyAsync(b->tile2, b->host_t2lay.data(), sizeof(Tile2) * 8 * st, hipMemcpyHostToDevice, s),
+                   "upload tile2 table");
+}
+}  // namespace
 
 extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
                        int32_t nsteps, int32_t parity, int32_t first_local, void* stream) {
@@ -1345,12 +1439,10 @@ extern "C" int lbm_batch_create(const lbm_lattice* lats, const lbm_fields* flds,
   int rc = hip_check(hipMalloc((void**)&b->items, sizeof(Lat) * count), "hipMalloc(batch items)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->tile_count, sizeof(uint32_t)), "hipMalloc(tile count)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->acc, sizeof(double) * 2 * count), "hipMalloc(residuals)");
-  if (!rc) rc = hip_check(hipMalloc((void**)&b->tile2_count, sizeof(uint32_t)), "hipMalloc(tile2 count)");
   if (rc) {
     (void)hipFree(b->items);
     (void)hipFree(b->tile_count);
     (void)hipFree(b->acc);
-    (void)hipFree(b->tile2_count);
     delete b;
     return rc;
   }
@@ -1397,47 +1489,16 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
   }
   b->tiles = (uint32_t)b->host_tiles.size();
   // two-step tiling: every active lattice without power law or fp64 storage
-  bool ok2 = b->storage != LBM_STORE_F64;
-  int h = 1;
+  bool ok2 = b->storage != LBM_STORE_F64, rules = false;
   for (int l = 0; l < b->count; ++l) {
-    const Lat& a = b->host[l];
+    Lat& a = b->host[l];
+    a.tile_oi = a.tile_oj = 0;
     if (!a.active) continue;
     if (a.tau_mode == LBM_TAU_POWER_LAW) ok2 = false;
     if (!fits_buffer(b->nc, a.nx, a.ny, a.ps, b->storage)) ok2 = false;   // buffer addressing
-    if (a.n_neem > 0) h = 2;
+    if (a.n_neem > 0) rules = true;
   }
-  b->host_t2lat.clear();
-  b->host_t2idx.clear();
-  if (ok2) {
-    for (int l = 0; l < b->count; ++l) {
-      const Lat& a = b->host[l];
-      if (!a.active) continue;
-      const uint32_t w = (uint32_t)b->tile_cols;
-      const uint32_t nt = ((a.nx + tile_rows(h, w) - 1) / tile_rows(h, w)) * ((a.ny + w - 1) / w);
-      for (uint32_t t = 0; t < nt; ++t) {
-        b->host_t2lat.push_back((uint32_t)l);
-        b->host_t2idx.push_back(t);
-      }
-    }
-  }
-  b->mode2 = ok2 ? h : 0;
-  b->tiles2 = (uint32_t)b->host_t2lat.size();
-  if (b->tiles2 > b->tile2_cap) {
-    if (b->captured)
-      return fail("two-step tile table full (%u > %zu) after a graph capture: reserve before capturing",
-                  b->tiles2, b->tile2_cap);
-    (void)hipFree(b->tile2_lat);
-    (void)hipFree(b->tile2_idx);
-    b->tile2_cap = b->tiles2 + b->tiles2 / 16 + 64;
-    int rc = hip_check(hipMalloc((void**)&b->tile2_lat, sizeof(uint32_t) * b->tile2_cap), "hipMalloc(tile2 table)");
-    if (!rc) rc = hip_check(hipMalloc((void**)&b->tile2_idx, sizeof(uint32_t) * b->tile2_cap), "hipMalloc(tile2 table)");
-    if (rc) {
-      b->tile2_lat = b->tile2_idx = nullptr;
-      b->tile2_cap = 0;
-      b->mode2 = 0;
-      return rc;
-    }
-  }
+  int mode = ok2 ? (rules ? 2 : 1) : 0;
   if (b->tiles > b->tile_cap) {
     // a captured graph holds the old table's address: grow only before capture
     if (b->captured)
@@ -1460,16 +1521,79 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
   if (!rc)
     rc = hip_check(hipMemcpyAsync(b->tile_count, &b->tiles, sizeof(uint32_t), hipMemcpyHostToDevice, s),
                    "upload tile count");
-  if (!rc && b->tiles2) {
-    rc = hip_check(hipMemcpyAsync(b->tile2_lat, b->host_t2lat.data(), sizeof(uint32_t) * b->tiles2,
-                                  hipMemcpyHostToDevice, s), "upload tile2 table");
+  if (rc) return rc;
+  const uint32_t w = (uint32_t)b->tile_cols;
+  if (mode == 2 && b->ring_pref == 0) {
+    // node rules with a one-node ring (mode 3) where every lattice's tile grid
+    // can be shifted so that each rule node's interior neighbour shares its tile
+    const uint32_t ti1 = tile_rows(1, (int)w);
+    if (b->shift_cap < b->count) {
+      (void)hipFree(b->shift_masks);
+      b->shift_masks = nullptr;
+      b->shift_cap = 0;
+      if ((rc = hip_check(hipMalloc((void**)&b->shift_masks, sizeof(uint32_t) * 3 * (size_t)b->count),
+                          "hipMalloc(shift masks)")))
+        return rc;
+      b->shift_cap = b->count;
+    }
+    std::vector<uint32_t> masks(3 * (size_t)b->count, 0u);
+    rc = hip_check(hipMemsetAsync(b->shift_masks, 0, sizeof(uint32_t) * masks.size(), s), "clear shift masks");
+    if (!rc) {
+      hipLaunchKernelGGL(k_rule_shifts, dim3(64, (unsigned)b->count), dim3(256), 0, s, b->items, ti1, w,
+                         b->shift_masks);
+      rc = hip_check(hipGetLastError(), "rule-shift scan");
+    }
     if (!rc)
-      rc = hip_check(hipMemcpyAsync(b->tile2_idx, b->host_t2idx.data(), sizeof(uint32_t) * b->tiles2,
-                                    hipMemcpyHostToDevice, s), "upload tile2 table");
+      rc = hip_check(hipMemcpyAsync(masks.data(), b->shift_masks, sizeof(uint32_t) * masks.size(),
+                                    hipMemcpyDeviceToHost, s), "read shift masks");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (rc) return rc;
+    bool ok3 = true;
+    for (int l = 0; l < b->count && ok3; ++l) {
+      Lat& a = b->host[l];
+      if (!a.active || a.n_neem == 0) continue;
+      const uint64_t cols = (uint64_t)masks[3 * l + 1] | ((uint64_t)masks[3 * l + 2] << 32);
+      uint32_t oi = 0, oj = 0;
+      while (oi < ti1 && ((masks[3 * l] >> oi) & 1u)) ++oi;
+      while (oj < w && ((cols >> oj) & 1u)) ++oj;
+      // no free shift, or a shifted grid over a lattice narrower than a tile
+      if (oi == ti1 || oj == w || (oi && a.nx < ti1) || (oj && a.ny < w)) ok3 = false;
+      a.tile_oi = oi;
+      a.tile_oj = oj;
+    }
+    if (ok3) {
+      mode = 3;
+    } else {
+      for (int l = 0; l < b->count; ++l) b->host[l].tile_oi = b->host[l].tile_oj = 0;
+    }
   }
-  if (!rc)
-    rc = hip_check(hipMemcpyAsync(b->tile2_count, &b->tiles2, sizeof(uint32_t), hipMemcpyHostToDevice, s),
-                   "upload tile2 count");
+  b->host_t2.clear();
+  if (mode) {
+    const uint32_t ti = tile_rows(ring_of(mode), (int)w);
+    for (int l = 0; l < b->count; ++l) {
+      const Lat& a = b->host[l];
+      if (!a.active) continue;
+      const uint32_t nti = (a.nx + a.tile_oi + ti - 1) / ti, ntj = (a.ny + a.tile_oj + w - 1) / w;
+      for (uint32_t r = 0; r < nti; ++r)
+        for (uint32_t c = 0; c < ntj; ++c)
+          b->host_t2.push_back(Tile2{a.code, (uint32_t)l, r * ti - a.tile_oi, c * w - a.tile_oj, a.nx, a.ny, 0u});
+    }
+  }
+  b->mode2 = mode;
+  b->tiles2 = (uint32_t)b->host_t2.size();
+  if ((b->tiles2 + 7) / 8 > b->tile2_stride || !b->tile2) {
+    if (b->captured)
+      return fail("two-step tile table full (%u > %zu) after a graph capture: reserve before capturing",
+                  b->tiles2, b->tile2_cap);
+    if ((rc = tiles2_alloc(b, (size_t)b->tiles2 + b->tiles2 / 16 + 64))) {
+      b->mode2 = 0;
+      return rc;
+    }
+  }
+  if (mode == 3)   // the descriptors now carry the grid shifts
+    rc = hip_check(hipMemcpyAsync(b->items, b->host.data(), sizeof(Lat) * b->count, hipMemcpyHostToDevice, s),
+                   "upload batch items");
+  if (!rc) rc = tiles2_upload(b, s);
   // the host vectors are read by the copies above: wait before they can change
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   return rc;
@@ -1497,26 +1621,17 @@ extern "C" int lbm_batch_reserve(lbm_batch* b, int32_t tiles) {
 extern "C" int lbm_batch_reserve2(lbm_batch* b, int32_t tiles) {
   using namespace lbm;
   if (!b || tiles < 0) return fail("bad lbm_batch_reserve2 arguments");
-  if ((size_t)tiles <= b->tile2_cap) return LBM_OK;
-  if (b->captured) return fail("cannot grow the two-step tile table after a graph capture");
-  uint32_t *t2l = nullptr, *t2i = nullptr;
-  int rc = hip_check(hipMalloc((void**)&t2l, sizeof(uint32_t) * (size_t)tiles), "hipMalloc(tile2 table)");
-  if (!rc) rc = hip_check(hipMalloc((void**)&t2i, sizeof(uint32_t) * (size_t)tiles), "hipMalloc(tile2 table)");
-  if (!rc && b->tiles2) {
-    rc = hip_check(hipMemcpy(t2l, b->tile2_lat, sizeof(uint32_t) * b->tiles2, hipMemcpyDeviceToDevice), "copy");
-    if (!rc) rc = hip_check(hipMemcpy(t2i, b->tile2_idx, sizeof(uint32_t) * b->tiles2, hipMemcpyDeviceToDevice), "copy");
+  if ((uint32_t)tiles < b->tiles2) tiles = (int32_t)b->tiles2;   // never below the current count
+  const size_t stride = std::max<size_t>(1, ((size_t)tiles + 7) / 8);
+  if (stride == b->tile2_stride && b->tile2) return LBM_OK;
+  if (b->captured) {
+    if (stride < b->tile2_stride) return LBM_OK;   // a graph holds the table: keep it
+    return fail("cannot grow the two-step tile table after a graph capture");
   }
-  if (rc) {
-    (void)hipFree(t2l);
-    (void)hipFree(t2i);
-    return rc;
-  }
-  (void)hipFree(b->tile2_lat);
-  (void)hipFree(b->tile2_idx);
-  b->tile2_lat = t2l;
-  b->tile2_idx = t2i;
-  b->tile2_cap = (size_t)tiles;
-  return LBM_OK;
+  int rc = tiles2_alloc(b, (size_t)tiles);
+  if (!rc) rc = tiles2_upload(b, nullptr);
+  if (!rc) rc = hip_check(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+  return rc;
 }
 
 namespace {
@@ -1538,14 +1653,19 @@ int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t 
 }  // namespace
 
 namespace {
-using Batch2Fn = void (*)(const lbm::Lat*, const uint32_t*, const uint32_t*, const uint32_t*, const lbm::Launch2);
+using Batch2Fn = void (*)(const lbm::Lat*, const lbm::Tile2*, uint32_t, const lbm::Launch2);
+template <int NC, int S, int W>
+Batch2Fn pick_batch2_w(int m) {
+  return m == 1 ? lbm::k_batch2<NC, S, 1, W, true> : (m == 2 ? lbm::k_batch2<NC, S, 2, W, true>
+                                                              : lbm::k_batch2<NC, S, 3, W, true>);
+}
 template <int NC, int S>
-Batch2Fn pick_batch2_s(int h, int w) {
+Batch2Fn pick_batch2_s(int m, int w) {
   // batched lattices are small: buffer addressing (each lattice's segment is
   // checked against 2^32 bytes at commit)
-  if (w == 16) return h == 1 ? lbm::k_batch2<NC, S, 1, 16, true> : lbm::k_batch2<NC, S, 2, 16, true>;
-  if (w == 32) return h == 1 ? lbm::k_batch2<NC, S, 1, 32, true> : lbm::k_batch2<NC, S, 2, 32, true>;
-  return h == 1 ? lbm::k_batch2<NC, S, 1, 64, true> : lbm::k_batch2<NC, S, 2, 64, true>;
+  if (w == 16) return pick_batch2_w<NC, S, 16>(m);
+  if (w == 32) return pick_batch2_w<NC, S, 32>(m);
+  return pick_batch2_w<NC, S, 64>(m);
 }
 Batch2Fn pick_batch2(int nc, int store, int h, int w) {
   switch (store) {
@@ -1569,9 +1689,10 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
   uint32_t p = (uint32_t)parity;
   int t0 = 0;
   if (two) {
-    // one block per tile: the current count, or the capacity for a graph
-    const size_t nb = capture ? b->tile2_cap : b->tiles2;
-    const int g2 = nb ? (int)((nb + 7) / 8 * 8) : 8;   // an empty batch: 8 blocks that exit
+    // one block per entry of the striped table (empty entries exit): a graph
+    // follows later commits while the tiles fit the stripes
+    (void)capture;
+    const int g2 = (int)(8 * b->tile2_stride);
     const int pairs = nsteps / 2;
     for (int t = 0; t < pairs; ++t) {
       Launch2 q2;
@@ -1579,9 +1700,8 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
       q2.parity = p;
       q2.step = t;
       q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
-      hipLaunchKernelGGL(two, dim3(g2), dim3(block2_threads(b->mode2, b->tile_cols)), 0, s, b->items, b->tile2_lat,
-                         b->tile2_idx,
-                         b->tile2_count, q2);
+      hipLaunchKernelGGL(two, dim3(g2), dim3(block2_threads(ring_of(b->mode2), b->tile_cols)), 0, s, b->items,
+                         b->tile2, (uint32_t)b->tile2_stride, q2);
       int rc = hip_check(hipGetLastError(), "batched two-step launch");
       if (rc) return rc;
       p ^= 1u;
@@ -1611,6 +1731,18 @@ extern "C" int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols) {
   b->tile_cols = w;
   return LBM_OK;
 }
+
+extern "C" int lbm_batch_ring(lbm_batch* b, int32_t ring) {
+  using namespace lbm;
+  if (!b) return fail("null batch");
+  if (ring != 0 && ring != 2) return fail("ring must be 0 (automatic) or 2");
+  if (ring == b->ring_pref) return LBM_OK;
+  if (b->captured) return fail("cannot change the ring after a graph capture");
+  b->ring_pref = ring;
+  return LBM_OK;
+}
+
+extern "C" int lbm_batch_tiles2(const lbm_batch* b) { return b ? (int)b->tiles2 : -1; }
 
 extern "C" int lbm_tile_rows(int32_t ring, int32_t tile_cols) {
   using namespace lbm;
@@ -1734,9 +1866,8 @@ extern "C" int lbm_batch_destroy(lbm_batch* b) {
   (void)hipFree(b->tile_item);
   (void)hipFree(b->tile_count);
   (void)hipFree(b->acc);
-  (void)hipFree(b->tile2_lat);
-  (void)hipFree(b->tile2_idx);
-  (void)hipFree(b->tile2_count);
+  (void)hipFree(b->tile2);
+  (void)hipFree(b->shift_masks);
   delete b;
   return LBM_OK;
 }

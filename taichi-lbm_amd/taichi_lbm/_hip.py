@@ -122,6 +122,8 @@ def lib():
         "lbm_batch_mode": ([BP], C.c_int),
         "lbm_batch_reserve2": ([BP, I32], C.c_int),
         "lbm_batch_tile_cols": ([BP, I32], C.c_int),
+        "lbm_batch_ring": ([BP, I32], C.c_int),
+        "lbm_batch_tiles2": ([BP], C.c_int),
         "lbm_batch_run_blocked": ([BP, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
         "lbm_batch_graph_blocked": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)], C.c_int),
         "lbm_graph_launch": ([P, P], C.c_int),
@@ -147,7 +149,7 @@ def exported_symbols():
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
-            "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows",
+            "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
 

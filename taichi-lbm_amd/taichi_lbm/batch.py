@@ -169,7 +169,8 @@ class BatchRunner:
         self.tile_cols = int(tile_cols)
         H.check(H.lib().lbm_batch_tile_cols(h, self.tile_cols), "lbm_batch_tile_cols")
         if self.blocked and specs:
-            # room for any tile width, so that autotune_tiles can switch
+            # room for any tile width, so that autotune_tiles can switch (a
+            # commit before capture grows the table further if it must)
             need = max(two_step_tiles(specs, w) for w in (16, 32, 64))
             H.check(H.lib().lbm_batch_reserve2(h, need + 64), "lbm_batch_reserve2")
         self.slots: Dict[int, dict] = {}
@@ -296,6 +297,21 @@ class BatchRunner:
             self._pending_local.clear()
             self._dirty = True
 
+    def fit_tables(self, margin: int = 64):
+        """Size the two-step tile table to the current tiling plus `margin`
+        entries (a graph captured afterwards launches one block per entry;
+        before any capture)."""
+        self.commit()
+        H.check(H.lib().lbm_batch_reserve2(self.h, H.lib().lbm_batch_tiles2(self.h) + int(margin)),
+                "lbm_batch_reserve2")
+
+    def set_ring(self, ring: int):
+        """0: node rules with a one-node ring where the lattices' tile grids
+        can be shifted to allow it (mode 3), else a two-node ring; 2: always
+        a two-node ring."""
+        H.check(H.lib().lbm_batch_ring(self.h, int(ring)), "lbm_batch_ring")
+        self._dirty = True
+
     def set_tile_cols(self, tile_cols: int):
         """Two-step tile width (16, 32 or 64; 0: default) from the next commit on."""
         H.check(H.lib().lbm_batch_tile_cols(self.h, int(tile_cols)), "lbm_batch_tile_cols")
@@ -304,7 +320,9 @@ class BatchRunner:
 
     @property
     def mode(self) -> int:
-        """0: one step per launch; 1 / 2: two-step launches with that ring width."""
+        """0: one step per launch; 1: two-step launches, one-node ring; 2:
+        two-node ring (node rules); 3: node rules on a one-node ring with
+        shifted tile grids."""
         self.commit()
         return H.lib().lbm_batch_mode(self.h) if self.blocked else 0
 
@@ -450,6 +468,7 @@ def autotune_tiles(runner: BatchRunner, candidates=(16, 32, 64), steps: int = 10
         timings[w] = e0.elapsed_time(e1) / steps
     best = min(timings, key=timings.get)
     runner.set_tile_cols(best)
+    runner.fit_tables()
     return best, timings
 
 

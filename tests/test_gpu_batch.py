@@ -148,15 +148,19 @@ def test_autotune_picks_a_candidate():
     assert best in timings and best % 8 == 0
 
 
-@pytest.mark.parametrize("tile_cols", [16, 32, 64])
-def test_blocked_batch_equals_one_step_batch(tile_cols):
-    """Two-step batched launches (ring width 2: the channels have node rules)
-    give the fields of one-step batched launches, bit for bit, with either
-    tile width."""
+@pytest.mark.parametrize("tile_cols,ring", [(16, 0), (32, 0), (64, 0), (32, 2), (64, 2)])
+def test_blocked_batch_equals_one_step_batch(tile_cols, ring):
+    """Two-step batched launches of channels with node rules give the fields
+    of one-step batched launches, bit for bit: with a one-node ring on
+    shifted tile grids (mode 3; with 32 columns the 13-row tiles put the
+    40-row channel's outlet row on a tile's first row unless its grid is
+    shifted) and with a two-node ring (mode 2)."""
     cs = _mix()
     a = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=True, tile_cols=tile_cols)
+    if ring:
+        a.set_ring(ring)
     b = BatchRunner([spec_from_case(c) for c in cs], storage="f32", blocked=False)
-    assert a.mode == 2 and b.mode == 0
+    assert a.mode == (3 if ring == 0 else 2) and b.mode == 0
     for n in (7, 10):
         a.run(n)
         b.run(n)
@@ -196,3 +200,26 @@ def test_blocked_batch_without_rules_and_graph():
             for name in ("f", "f2", "rho", "vel"):
                 assert np.array_equal(fa[name], fb[name]), (q, name)
         g.close()
+
+
+@pytest.mark.parametrize("tile_cols", [32, 64])
+def test_open_neumann_outlets_on_shifted_grids(tile_cols):
+    """The reference cylinder's boundary list (VelocityBB inlet, OpenNeumann
+    outlet reading x - c_1) on lattices whose outlet row NX - 1 is a multiple
+    of the tile rows (13 with 32 columns, 5 with 64), so mode 3 shifts their
+    tile grids; against one-step launches."""
+    import contextlib, io
+    with contextlib.redirect_stdout(io.StringIO()):
+        cs = [cases.cylinder_reference_bcs(NX=NX, NY=NY, name=f"r{q}")
+              for q, (NX, NY) in enumerate([(66, 40), (131, 48), (79, 36)])]
+    specs = [spec_from_case(c) for c in cs]
+    a = BatchRunner(specs, storage="f32", blocked=True, tile_cols=tile_cols)
+    b = BatchRunner(specs, storage="f32", blocked=False)
+    assert a.mode == 3
+    for n in (9, 12):
+        a.run(n)
+        b.run(n)
+    for q in range(len(cs)):
+        fa, fb = a.fields(q), b.fields(q)
+        for name in ("f", "f2", "rho", "vel", "total_rho"):
+            assert np.array_equal(fa[name], fb[name]), (q, name)
