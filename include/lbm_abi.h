@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 8
+#define LBM_ABI_VERSION 9
 
 /* status codes */
 #define LBM_OK 0
@@ -134,6 +134,9 @@ typedef struct lbm_fields {
   float* sc_force;      /* nc*2 planes                                      */
   float* shear_rate;    /* nc planes                                        */
   const uint16_t* code; /* one node code per node                           */
+  void* tau2;           /* batches: a second relaxation-state buffer (as    */
+                        /* tau) that lets power-law lattices take two steps */
+                        /* per launch; NULL: one step per launch            */
 } lbm_fields;
 
 /* ---- version / errors ---------------------------------------------------*/
@@ -233,7 +236,8 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
 /* Two steps per launch for the whole batch where it allows it (as
  * lbm_run_blocked: every active lattice without power law, storage other
  * than FP64).  lbm_batch_mode reports what the last commit chose: 0 = one
- * step per launch; 1 = two-step launches with a one-node ring (no node
+ * step per launch (also when a power-law lattice has no fields.tau2);
+ * 1 = two-step launches with a one-node ring (no node
  * rules); 2 = a two-node ring (node rules); 3 = node rules with a one-node
  * ring: each lattice's tile grid is shifted (found at commit by scanning the
  * rule nodes) so that every NEEM / OpenNeumann node shares its tile with the
@@ -248,6 +252,9 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
 int lbm_batch_mode(const lbm_batch* b);
 int lbm_batch_ring(lbm_batch* b, int32_t ring);
 int lbm_batch_tiles2(const lbm_batch* b);
+/* The caller has destroyed every graph captured from this batch: its tables
+ * and tile shape may change again (a capture pins them).                  */
+int lbm_batch_release(lbm_batch* b);
 /* Tile columns (0: default, 16, 32 or 64) of the batch's two-step launches from
  * the next commit on; refused after a graph capture.                      */
 int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols);

@@ -57,6 +57,7 @@ struct Lat {
   int32_t start_step;    // batched runs: launch index whose populations are read in place
   uint32_t block0;       // batched runs: first 256-node tile of this lattice
   uint32_t active;
+  void* tau2;                  // batched two-step runs of power-law lattices: second state buffer
   uint32_t tile_oi, tile_oj;   // batched two-step runs: shift of the tile grid (tile (ti, tj) starts at
                                // row ti*TI - tile_oi, column tj*W - tile_oj)
   float tau0[LBM_MAX_NC], tau_anti0[LBM_MAX_NC], magic[LBM_MAX_NC];
@@ -621,6 +622,7 @@ struct Launch2 {
   uint32_t tiles_i, tiles_j;  // single lattice
   const void* tau_in;         // single lattice, power law: state before step n
   void* tau_out;              // single lattice, power law: state after step n+1
+  uint32_t tau_flip;          // batches, power law: read tau2, write tau (else the reverse)
 };
 
 // Phase A inputs of one ring node: gathered populations and relaxation state.
@@ -895,8 +897,7 @@ constexpr uint32_t kEmptyTile = 0xFFFFFFFFu;
 // blocks past the current count exit, so the graph follows admissions and
 // retirements).  Each XCD takes one contiguous eighth of the current tile
 // list, so the tiles spread over all XCDs whatever the capacity.  Power-law
-// lattices are not batched this way (their state would need a second buffer
-// per lattice).  Each entry carries the tile's origin and the lattice's code
+// lattices alternate their relaxation state between tau and tau2.  Each entry carries the tile's origin and the lattice's code
 // pointer and sides; the table is striped by XCD with empty entries, so no
 // tile count has to be read before the tile's own loads.
 template <int NC, int S, int M, int W, bool BUF>
@@ -909,17 +910,20 @@ __global__ __launch_bounds__(block2_threads(ring_of(M), W)) __attribute__((amdgp
   if (tr.l == kEmptyTile) return;
   const Lat& a = items[tr.l];
   const bool local = a.start_step == q.step;
+  // power law: the state alternates between tau and tau2 from launch to launch
+  const void* tin = q.tau_flip ? a.tau2 : a.tau;
+  void* tout = q.tau_flip ? a.tau : a.tau2;
   // the collision model is the lattice's (uniform over the block): one
   // specialised tile path per model
   switch (a.coll) {
     case LBM_BGK:
-      tile_two_steps<LBM_BGK, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      tile_two_steps<LBM_BGK, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, tin, tout, L, tr.code, tr.nx, tr.ny);
       break;
     case LBM_TRT:
-      tile_two_steps<LBM_TRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      tile_two_steps<LBM_TRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, tin, tout, L, tr.code, tr.nx, tr.ny);
       break;
     default:
-      tile_two_steps<LBM_MRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, a.tau, nullptr, L, tr.code, tr.nx, tr.ny);
+      tile_two_steps<LBM_MRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, tin, tout, L, tr.code, tr.nx, tr.ny);
       break;
   }
 }
@@ -948,6 +952,24 @@ __global__ void k_rule_shifts(const Lat* __restrict__ items, uint32_t ti, uint32
       const uint32_t o = d == 2 ? (w - j % w) % w : (2 * w - 1 - j % w) % w;        // x first / last column
       atomicOr(&masks[3 * l + 1 + (o >> 5)], 1u << (o & 31u));
     }
+  }
+}
+
+// After an odd number of batched two-step launches the power-law state of the
+// active nodes is in tau2: copy it back to tau (the one-step, finish and
+// readback paths read tau), over the batch's 256-node tiles.
+__global__ __launch_bounds__(kBlock) void k_batch_tau_back(const Lat* __restrict__ items,
+                                                           const uint32_t* __restrict__ tile_item,
+                                                           const uint32_t* __restrict__ tile_count, uint32_t nc) {
+  const uint32_t total = *tile_count;
+  for (uint32_t w = blockIdx.x; w < total; w += gridDim.x) {
+    const uint32_t l = __builtin_amdgcn_readfirstlane(tile_item[w]);
+    const Lat& a = items[l];
+    if (a.tau_mode != LBM_TAU_POWER_LAW || !a.tau2) continue;
+    const uint32_t n = (w - a.block0) * kBlock + threadIdx.x;
+    // active nodes only: the others keep their state, as in one-step runs
+    if (n >= a.N || !(a.code[n] & LBM_CODE_ACTIVE)) continue;
+    for (uint32_t c = 0; c < nc; ++c) ((float*)a.tau)[(size_t)c * a.ps + n] = ((const float*)a.tau2)[(size_t)c * a.ps + n];
   }
 }
 
@@ -1097,6 +1119,7 @@ Lat make_lat(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
     a.psi[q] = fld->psi[q];
   }
   a.tau = fld->tau;
+  a.tau2 = fld->tau2;
   a.body_force = fld->body_force;
   a.sc_force = fld->sc_force;
   a.shear_rate = fld->shear_rate;
@@ -1218,6 +1241,7 @@ struct lbm_batch {
   int mode2 = 0;                     // 0: one step per launch, else the ring width H
   int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (16, 32 or 64)
   int ring_pref = 0;                 // 0: one-node ring with rules where the tile grids allow it; 2: two-node
+  bool power_law = false;            // an active lattice has a power-law state (two-step: tau / tau2)
   uint32_t* shift_masks = nullptr;   // device: 3 words per lattice (k_rule_shifts)
   int shift_cap = 0;
   std::vector<lbm::Tile2> host_t2, host_t2lay;   // tile list; its striped layout (uploaded)
@@ -1490,11 +1514,15 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
   b->tiles = (uint32_t)b->host_tiles.size();
   // two-step tiling: every active lattice without power law or fp64 storage
   bool ok2 = b->storage != LBM_STORE_F64, rules = false;
+  b->power_law = false;
   for (int l = 0; l < b->count; ++l) {
     Lat& a = b->host[l];
     a.tile_oi = a.tile_oj = 0;
     if (!a.active) continue;
-    if (a.tau_mode == LBM_TAU_POWER_LAW) ok2 = false;
+    if (a.tau_mode == LBM_TAU_POWER_LAW) {
+      if (!a.tau2) ok2 = false;   // no second state buffer
+      b->power_law = true;
+    }
     if (!fits_buffer(b->nc, a.nx, a.ny, a.ps, b->storage)) ok2 = false;   // buffer addressing
     if (a.n_neem > 0) rules = true;
   }
@@ -1700,11 +1728,18 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
       q2.parity = p;
       q2.step = t;
       q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
+      q2.tau_flip = (uint32_t)(t & 1);
       hipLaunchKernelGGL(two, dim3(g2), dim3(block2_threads(ring_of(b->mode2), b->tile_cols)), 0, s, b->items,
                          b->tile2, (uint32_t)b->tile2_stride, q2);
       int rc = hip_check(hipGetLastError(), "batched two-step launch");
       if (rc) return rc;
       p ^= 1u;
+    }
+    if ((pairs & 1) && b->power_law) {
+      hipLaunchKernelGGL(k_batch_tau_back, dim3(2048), dim3(kBlock), 0, s, b->items, b->tile_item, b->tile_count,
+                         (uint32_t)b->nc);
+      int rc = hip_check(hipGetLastError(), "relaxation-state copy-back");
+      if (rc) return rc;
     }
     t0 = pairs;
     nsteps -= 2 * pairs;
@@ -1743,6 +1778,13 @@ extern "C" int lbm_batch_ring(lbm_batch* b, int32_t ring) {
 }
 
 extern "C" int lbm_batch_tiles2(const lbm_batch* b) { return b ? (int)b->tiles2 : -1; }
+
+extern "C" int lbm_batch_release(lbm_batch* b) {
+  using namespace lbm;
+  if (!b) return fail("null batch");
+  b->captured = false;
+  return LBM_OK;
+}
 
 extern "C" int lbm_tile_rows(int32_t ring, int32_t tile_cols) {
   using namespace lbm;

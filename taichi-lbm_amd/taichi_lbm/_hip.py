@@ -26,7 +26,7 @@ CODE_NEEM_MASK = 0x3C00
 CODE_PULLED = 0x4000      # inactive node pulled by an active one (plain pull)
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 
 class NeemRule(C.Structure):
@@ -55,7 +55,7 @@ class Fields(C.Structure):
     _fields_ = [
         ("f", C.c_void_p * 2), ("macro", C.c_void_p * 2), ("tau", C.c_void_p),
         ("psi", C.c_void_p * 2), ("body_force", C.c_void_p), ("sc_force", C.c_void_p),
-        ("shear_rate", C.c_void_p), ("code", C.c_void_p),
+        ("shear_rate", C.c_void_p), ("code", C.c_void_p), ("tau2", C.c_void_p),
     ]
 
 
@@ -124,6 +124,7 @@ def lib():
         "lbm_batch_tile_cols": ([BP, I32], C.c_int),
         "lbm_batch_ring": ([BP, I32], C.c_int),
         "lbm_batch_tiles2": ([BP], C.c_int),
+        "lbm_batch_release": ([BP], C.c_int),
         "lbm_batch_run_blocked": ([BP, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
         "lbm_batch_graph_blocked": ([BP, I32, I32, I32, P, C.POINTER(C.c_void_p), C.POINTER(C.c_int32)], C.c_int),
         "lbm_graph_launch": ([P, P], C.c_int),
@@ -149,7 +150,7 @@ def exported_symbols():
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
-            "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2",
+            "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2", "lbm_batch_release",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
 

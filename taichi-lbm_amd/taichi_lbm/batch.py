@@ -75,7 +75,7 @@ def footprint_bytes(spec_or_n, storage="f32") -> int:
     n = spec_or_n if isinstance(spec_or_n, int) else spec_or_n.N
     ps = plane_stride(n)
     r = 8 if storage == "f64" else 4          # relaxation state and macro planes: fp64 with fp64 storage
-    return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + r)
+    return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + 2 * r)
 
 
 def two_step_tiles(specs, tile_cols: int = 0) -> int:
@@ -148,6 +148,8 @@ class BatchRunner:
         self.velseg = t.zeros(2 * cap, dtype=t.float32, device=dev)
         self.code = t.zeros(cap, dtype=t.int16, device=dev)
         self.tau = t.zeros(cap, dtype=rdt, device=dev)
+        # second relaxation-state buffer: power-law lattices take two steps per launch
+        self.tau2 = t.zeros(cap, dtype=rdt, device=dev) if self.blocked else None
         self.arena = _Arena(cap)
         n = self.max_slots
         self.lats = (H.Lattice * n)()
@@ -174,6 +176,7 @@ class BatchRunner:
             need = max(two_step_tiles(specs, w) for w in (16, 32, 64))
             H.check(H.lib().lbm_batch_reserve2(h, need + 64), "lbm_batch_reserve2")
         self.slots: Dict[int, dict] = {}
+        self._open_graphs = 0
         self.parity = 0
         self.finished = False
         self._dirty = True
@@ -193,6 +196,7 @@ class BatchRunner:
         F.macro[0] = self.macseg[0].data_ptr() + 4 * o * me
         F.macro[1] = self.macseg[1].data_ptr() + 4 * o * me
         F.tau = self.tau.data_ptr() + o * self.tau.element_size()
+        F.tau2 = self.tau2.data_ptr() + o * self.tau2.element_size() if self.tau2 is not None else None
         F.code = self.code.data_ptr() + o * 2
 
     def _vel_ptr(self, o):
@@ -361,7 +365,13 @@ class BatchRunner:
         else:
             H.check(H.lib().lbm_batch_graph(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g)),
                     "lbm_batch_graph")
+        self._open_graphs += 1
         return BatchGraph(self, g, int(nsteps), mode)
+
+    def _graph_closed(self):
+        self._open_graphs -= 1
+        if self._open_graphs == 0 and getattr(self, "h", None):
+            H.check(H.lib().lbm_batch_release(self.h), "lbm_batch_release")
 
     def finish(self):
         """Boundary pass for every active lattice (f, rho, vel as the reference reads them)."""
@@ -442,6 +452,7 @@ class BatchGraph:
         if self.g:
             H.lib().lbm_graph_destroy(self.g)
             self.g = None
+            self.runner._graph_closed()
 
     def __del__(self):
         try:

@@ -75,17 +75,20 @@ class ServeStats:
 class SolveServer:
     def __init__(self, storage: str = "f32", hbm_budget_bytes: int = 8 << 30, max_slots: int = 256,
                  tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True):
-        if tick_steps % 2:
-            raise ValueError("tick_steps must be even (graph replays keep one parity schedule)")
+        if tick_steps % 4:
+            raise ValueError("tick_steps must be a multiple of 4 (graph replays of two-step launches keep one "
+                             "parity schedule)")
         self.storage = storage
         self.budget = int(hbm_budget_bytes)
         # arena sized from the budget: bytes per node of a resident lattice
         per_node = footprint_bytes(64, storage) // 64
         self.capacity_nodes = self.budget // per_node // 64 * 64
-        # one step per launch: requests come and go after the graph is
-        # captured, and the two-step tiling of future lattices is unknown
+        # two steps per launch where the storage allows it; the tick's graph
+        # is captured again whenever the resident set changes (the two-step
+        # tiling, its table and mode follow the resident lattices)
         self.runner = BatchRunner([], storage=storage, grid_blocks=grid_blocks,
-                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=False)
+                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=True)
+        self.changed = False
         self.tick_steps = int(tick_steps)
         self.keep_results = keep_results
         self.graph = None
@@ -94,6 +97,7 @@ class SolveServer:
 
     def _admit(self, req: Request, now: float):
         slot = self.runner.add(req.spec)
+        self.changed = True
         req.admitted = now
         self.resident[slot] = req
         self.used_bytes += footprint_bytes(req.spec, self.storage)
@@ -103,6 +107,7 @@ class SolveServer:
         if self.keep_results:
             req.result = self.runner.velocity(slot)
         self.runner.remove(slot)
+        self.changed = True
         self.used_bytes -= footprint_bytes(req.spec, self.storage)
         req.finished = now
 
@@ -127,9 +132,15 @@ class SolveServer:
                     continue
                 stats.peak_resident = max(stats.peak_resident, len(self.resident))
                 stats.peak_bytes = max(stats.peak_bytes, self.used_bytes)
+                if self.changed and self.graph is not None:
+                    self.graph.close()      # releases the batch's tables for the new tiling
+                    self.graph = None
+                self.changed = False
                 if self.graph is None:
-                    # first tick runs eagerly (fresh lattices read in place), then capture
+                    # first tick after a change runs eagerly (fresh lattices read
+                    # in place), then the tick is captured at the tiling's size
                     self.runner.run(self.tick_steps)
+                    self.runner.fit_tables()
                     self.graph = self.runner.capture(self.tick_steps)
                 else:
                     self.graph.replay()
@@ -158,9 +169,11 @@ class SolveServer:
 
 
 def solve_alone(spec: LatticeSpec, max_steps: int, tol: float, tick_steps: int, storage: str = "f32",
-                grid_blocks: int = 0):
-    """The same request on its own: one lattice, the same tick / residual schedule."""
-    r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks)
+                grid_blocks: int = 0, blocked: bool = False):
+    """The same request on its own: one lattice, the same tick / residual
+    schedule; by default the plain path (one step per launch), which gives
+    the same fields bit for bit as two steps per launch (blocked=True)."""
+    r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks, blocked=blocked)
     steps, res = 0, float("inf")
     while True:
         r.run(tick_steps)

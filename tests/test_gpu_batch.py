@@ -223,3 +223,45 @@ def test_open_neumann_outlets_on_shifted_grids(tile_cols):
         fa, fb = a.fields(q), b.fields(q)
         for name in ("f", "f2", "rho", "vel", "total_rho"):
             assert np.array_equal(fa[name], fb[name]), (q, name)
+
+
+def test_power_law_batch_two_steps():
+    """Power-law lattices take two steps per launch in a batch (their
+    relaxation state alternates between tau and tau2 and is copied back after
+    an odd number of launches): bit-identical to one-step batched launches,
+    relaxation state included, run eagerly and replayed from a graph."""
+    import torch
+    import contextlib, io
+    with contextlib.redirect_stdout(io.StringIO()):
+        cs = [cases.contraction(W=W, H=Hh, power_n=n, u_in=0.03, name=f"p{q}")
+              for q, (W, Hh, n) in enumerate([(96, 32, 0.7), (128, 40, 1.3), (80, 24, 0.5)])]
+        cs.append(cases.porous(NX=70, NY=90, seed=3, collision="mrt", rmin=3.0, rmax=6.0, g=(1e-5, 0.0)))
+    specs = [spec_from_case(c) for c in cs]
+    a = BatchRunner(specs, storage="f32", blocked=True)
+    b = BatchRunner(specs, storage="f32", blocked=False)
+    assert a.mode in (2, 3) and b.mode == 0
+
+    def same():
+        for q in range(len(cs)):
+            fa, fb = a.fields(q), b.fields(q)
+            for name in ("f", "f2", "rho", "vel", "total_rho"):
+                assert np.array_equal(fa[name], fb[name]), (q, name)
+        assert torch.equal(a.tau, b.tau)
+
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for n in (6, 8, 7):       # 3 launches (state copied back), 4, 3 + one single step
+            a.run(n)
+            b.run(n)
+        a.run(1)                  # back to an even parity for the capture
+        b.run(1)
+        torch.cuda.synchronize()
+        same()
+        g = a.capture(8)
+        for _ in range(2):
+            g.replay()
+            torch.cuda.synchronize()
+        b.run(16)
+        torch.cuda.synchronize()
+        same()
+        g.close()
