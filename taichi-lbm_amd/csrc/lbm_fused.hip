@@ -973,6 +973,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_tau_back(const Lat* __restrict
   }
 }
 
+// The same for one lattice (lbm_run_blocked): a.tau2 holds the state to copy.
+__global__ __launch_bounds__(kBlock) void k_tau_back(const Lat a, uint32_t nc) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= a.N || !(a.code[n] & LBM_CODE_ACTIVE)) return;
+  for (uint32_t c = 0; c < nc; ++c) ((float*)a.tau)[(size_t)c * a.ps + n] = ((const float*)a.tau2)[(size_t)c * a.ps + n];
+}
+
 // Persistent batched step over the 256-node tiles of many lattices.  Each XCD
 // takes one contiguous range of the tile list; its blocks stride through it.
 // The tile -> lattice table and the tile count live in device memory so a
@@ -1383,7 +1390,6 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
     return rc;
   }
   Lat a = make_lat(lat, fld, neem);
-  const size_t rbytes = lat->storage == LBM_STORE_F64 ? 8 : 4;
   const void* tcur = fld->tau;
   void* tnext = tau_next;
   Launch2 q2;
@@ -1422,9 +1428,13 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
     p ^= 1u;
   }
   if (pl && tcur != fld->tau) {
-    const size_t bytes = (size_t)lat->nc * (size_t)lat->plane_stride * rbytes;
-    if ((rc = hip_check(hipMemcpyAsync(fld->tau, tcur, bytes, hipMemcpyDeviceToDevice, s), "copy tau state")))
-      return rc;
+    // the state of the active nodes is in tau_next: copy it back (the other
+    // nodes keep theirs; tau_next holds nothing defined there)
+    Lat t = a;
+    t.tau = fld->tau;   // (the odd step above pointed a.tau at the current buffer)
+    t.tau2 = const_cast<void*>(tcur);
+    hipLaunchKernelGGL(k_tau_back, dim3((a.N + kBlock - 1) / kBlock), dim3(kBlock), 0, s, t, (uint32_t)lat->nc);
+    if ((rc = hip_check(hipGetLastError(), "copy tau state"))) return rc;
   }
   *out_parity = (int32_t)p;
   return LBM_OK;
