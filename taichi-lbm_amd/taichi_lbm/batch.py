@@ -70,9 +70,52 @@ def spec_from_case(case, power_law=None) -> LatticeSpec:
                        tag=dict(case.get("spec", {})))
 
 
+ROW_ALIGN = 32     # nodes: a 128-byte line of FP32 populations
+
+
+def _no_wrap_across_rows(spec: LatticeSpec) -> bool:
+    """True when no active node streams across the lattice's j boundary: every
+    population that would wrap from column NY-1 to 0 or back is bounced."""
+    code = spec.code.reshape(spec.NX, spec.NY)
+    for j, ks in ((0, (2, 5, 6)), (spec.NY - 1, (4, 7, 8))):   # pulls with cy = +1 at j = 0, cy = -1 at NY-1
+        col = code[:, j].astype(np.uint32)
+        act = (col & H.CODE_ACTIVE) != 0
+        for k in ks:
+            if np.any(act & (((col >> (k - 1)) & 1) == 0)):
+                return False
+    return True
+
+
+def pad_rows(spec: LatticeSpec, align: int = ROW_ALIGN) -> LatticeSpec:
+    """The lattice stored with rows of a multiple of `align` nodes: columns
+    NY..NY'-1 are added outside the fluid group.  Only where no active node
+    streams across the j boundary (the new columns then are never read and the
+    fields are those of the unpadded lattice, bit for bit); otherwise, or when
+    NY is already a multiple, the spec itself.  Aligned rows let every tile
+    row of the two-step kernel start on a cache line."""
+    NYp = -(-spec.NY // align) * align
+    if NYp == spec.NY or spec.NY < 2 or not _no_wrap_across_rows(spec):
+        return spec
+    code = np.zeros((spec.NX, NYp), np.uint16)
+    code[:, :spec.NY] = spec.code.reshape(spec.NX, spec.NY)
+    rho0 = np.ones((spec.NX, NYp), spec.rho0.dtype)
+    rho0[:, :spec.NY] = spec.rho0
+    u0 = np.zeros((spec.NX, NYp, 2), spec.u0.dtype)
+    u0[:, :spec.NY] = spec.u0
+    L = H.Lattice()
+    ctypes.memmove(ctypes.byref(L), ctypes.byref(spec.lattice), ctypes.sizeof(H.Lattice))
+    L.ny = NYp
+    return LatticeSpec(spec.NX, NYp, code.reshape(-1), spec.neem, L, rho0, u0, tag=dict(spec.tag, view_ny=spec.NY))
+
+
+def stored_nodes(spec: LatticeSpec) -> int:
+    """Nodes a lattice occupies in a batch (rows padded where that is exact)."""
+    return pad_rows(spec).N
+
+
 def footprint_bytes(spec_or_n, storage="f32") -> int:
     """Device bytes one lattice occupies in a batch (its arena segment)."""
-    n = spec_or_n if isinstance(spec_or_n, int) else spec_or_n.N
+    n = spec_or_n if isinstance(spec_or_n, int) else stored_nodes(spec_or_n)
     ps = plane_stride(n)
     r = 8 if storage == "f64" else 4          # relaxation state and macro planes: fp64 with fp64 storage
     return ps * (2 * 9 * POP_BYTES[storage] + 2 * 4 * r + 2 * 4 + 2 + 2 * r)
@@ -125,7 +168,8 @@ class BatchRunner:
     dynamic use (``add`` / ``remove``)."""
 
     def __init__(self, specs: Sequence[LatticeSpec] = (), storage: str = "f32", grid_blocks: int = 0,
-                 capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True, tile_cols: int = 0):
+                 capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True, tile_cols: int = 0,
+                 align_rows: bool = True):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         t = torch()
@@ -134,7 +178,8 @@ class BatchRunner:
         self.storage = storage
         self.grid = int(grid_blocks)
         self.blocked = bool(blocked)    # two steps per launch where the batch allows it
-        cap = int(capacity_nodes) or sum(plane_stride(s.N) for s in specs)
+        self.align_rows = bool(align_rows)   # rows padded to whole cache lines where that is exact
+        cap = int(capacity_nodes) or sum(plane_stride(stored_nodes(s) if align_rows else s.N) for s in specs)
         cap = max(64, (cap + 63) // 64 * 64)
         self.capacity = cap
         self.max_slots = int(max_slots) or max(1, len(specs))
@@ -204,11 +249,15 @@ class BatchRunner:
 
     # ---- lattices --------------------------------------------------------------
     def fits(self, spec: LatticeSpec) -> bool:
-        return len(self.slots) < self.max_slots and any(l >= plane_stride(spec.N) for _, l in self.arena.free)
+        n = stored_nodes(spec) if self.align_rows else spec.N
+        return len(self.slots) < self.max_slots and any(l >= plane_stride(n) for _, l in self.arena.free)
 
     def add(self, spec: LatticeSpec) -> int:
         """Makes the lattice resident (f = feq(u0, rho0), init_LBM) and active
         from the next launch on; returns its slot."""
+        given = spec
+        if self.align_rows:
+            spec = pad_rows(spec)
         ps = plane_stride(spec.N)
         slot = next((l for l in range(self.max_slots) if l not in self.slots), None)
         if slot is None:
@@ -216,7 +265,8 @@ class BatchRunner:
         o = self.arena.alloc(ps)
         if o is None:
             raise RuntimeError("batch arena full")
-        self.slots[slot] = {"spec": spec, "off": o, "ps": ps}
+        # "spec" as given, "stored" as laid out in the arena (rows possibly padded)
+        self.slots[slot] = {"spec": given, "stored": spec, "off": o, "ps": ps}
         self.ps[slot] = ps
         L = self.lats[slot]
         ctypes.memmove(ctypes.byref(L), ctypes.byref(spec.lattice), ctypes.sizeof(H.Lattice))
@@ -249,7 +299,7 @@ class BatchRunner:
         next launch reads; relaxation state = initTau's value; u_old = 0."""
         t = torch()
         info = self.slots[slot]
-        s, o, ps = info["spec"], info["off"], info["ps"]
+        s, o, ps = info["stored"], info["off"], info["ps"]
         f64 = self.storage == "f64"
         dt = np.float64 if f64 else np.float32
         w = _W64 if f64 else _W32
@@ -400,19 +450,20 @@ class BatchRunner:
         self.finish()
         t = torch()
         info = self.slots[slot]
-        s, o, ps = info["spec"], info["off"], info["ps"]
+        s, o, ps = info["stored"], info["off"], info["ps"]
         NX, NY, p = s.NX, s.NY, self.parity
+        VY = info["spec"].NY   # rows padded in the arena: the caller sees NX x VY
 
         def pops(q):
             a = self.popseg[q, 9 * o: 9 * (o + ps)].reshape(9, ps)[:, :s.N].to(t.float64).cpu().numpy()
             if self.storage in ("f16", "bf16"):
                 act = (s.code & H.CODE_ACTIVE) != 0
                 a = a + np.where(act[None, :], _W32.astype(np.float64)[:, None], 0.0)
-            return a.reshape(9, NX, NY).transpose(1, 2, 0)[:, :, None, :]
+            return np.ascontiguousarray(a.reshape(9, NX, NY).transpose(1, 2, 0)[:, :VY, None, :])
 
-        m = self.macseg[p, 4 * o: 4 * (o + ps)].reshape(4, ps)[:, :s.N].cpu().numpy().reshape(4, NX, NY)
-        return {"f2": pops(p), "f": pops(p ^ 1), "rho": m[0][..., None], "total_rho": m[1],
-                "vel": np.stack([m[2], m[3]], -1)}
+        m = self.macseg[p, 4 * o: 4 * (o + ps)].reshape(4, ps)[:, :s.N].cpu().numpy().reshape(4, NX, NY)[:, :, :VY]
+        return {"f2": pops(p), "f": pops(p ^ 1), "rho": np.ascontiguousarray(m[0][..., None]),
+                "total_rho": np.ascontiguousarray(m[1]), "vel": np.stack([m[2], m[3]], -1)}
 
     def velocity(self, slot: int) -> np.ndarray:
         return self.fields(slot)["vel"]

@@ -223,3 +223,31 @@ def test_rule_rejections():
     with pytest.raises(UnsupportedBoundaryList, match="node rule"):
         compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("velocity_bb", col(0), 3, 0.01),
                                       BCSpec("bounceback", col(0))], nb)
+
+
+def test_row_padding_only_without_wrapping_pulls():
+    """A batch lattice is stored with rows padded to 32 nodes only when no
+    active node streams across the j boundary (checked by brute force over
+    the boundary columns here)."""
+    from taichi_lbm.batch import LatticeSpec, pad_rows
+    NX, NY = 9, 21
+    mask = np.ones((NX, NY), np.int32)
+    mask[:, 0] = mask[:, -1] = -1            # solid walls: the rows next to them bounce
+    nb = neighbor_table(mask)
+    fluid = np.argwhere(mask == 1)
+    wall = np.argwhere((mask == 1) & np.any(nb[..., 0] == -1, axis=2))
+    code, _ = compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid), BCSpec("bounceback", wall)], nb)
+    L = H.Lattice()
+    L.nx, L.ny, L.nc = NX, NY, 1
+    spec = LatticeSpec(NX, NY, code, [], L, np.ones((NX, NY)), np.zeros((NX, NY, 2)))
+    padded = pad_rows(spec)
+    assert padded.NY == 32 and padded.lattice.ny == 32 and padded.tag["view_ny"] == NY
+    c2 = padded.code.reshape(NX, 32)
+    assert np.array_equal(c2[:, :NY].reshape(-1), code) and not c2[:, NY:].any()
+    # the same lattice periodic across j (no walls): fluid nodes at j = 0 pull from j = NY-1
+    mask[:] = 1
+    nb = neighbor_table(mask)
+    fluid = np.argwhere(mask == 1)
+    code, _ = compile_codes(NX, NY, fluid, [BCSpec("periodic", fluid)], nb)
+    spec = LatticeSpec(NX, NY, code, [], L, np.ones((NX, NY)), np.zeros((NX, NY, 2)))
+    assert pad_rows(spec) is spec
