@@ -15,6 +15,7 @@ on the host for a bounded number of steps of the same lattice.
 """
 import argparse
 import contextlib
+import copy
 import io
 import json
 import os
@@ -120,7 +121,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     import torch
     from taichi_lbm import _hip as H
     from taichi_lbm import cases
-    from taichi_lbm.batch import BatchRunner, autotune_grid, autotune_tiles, spec_from_case
+    from taichi_lbm.batch import BatchRunner, autotune_grid, autotune_tiles, spec_from_case, stored_nodes
     rng = np.random.default_rng(seed)
     specs = []
     t0 = time.time()
@@ -160,21 +161,27 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         e1.record(stream)
         torch.cuda.synchronize()
         ms_b = e0.elapsed_time(e1) / (reps * graph_steps)
-        # plain path: one launch per lattice per step (lbm_run on each lattice's own buffers)
+        g.close()
+        runner.close()
+        # plain path: one launch per lattice per step (lbm_run on each lattice's
+        # own buffers), on the lattices as given (rows not padded)
+        plain = BatchRunner(specs, storage="f32", blocked=False, align_rows=False)
+        plain.run(2)
         lib = H.lib()
         s = stream.cuda_stream
-        par = runner.parity
-        slots = sorted(runner.slots)
+        par = plain.parity
+        slots = sorted(plain.slots)
+
+        def rules(l):
+            return ctypes.cast(ctypes.addressof(plain.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
+                               ctypes.POINTER(H.NeemRule))
         for l in slots:        # warm-up
-            H.check(lib.lbm_run(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
-                                ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
-                                            ctypes.POINTER(H.NeemRule)), 2, par, 0, s))
+            H.check(lib.lbm_run(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l), 2, par, 0, s))
         torch.cuda.synchronize()
         e0.record(stream)
         for l in slots:
-            H.check(lib.lbm_run(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
-                                ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
-                                            ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, s))
+            H.check(lib.lbm_run(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l),
+                                plain_steps, par, 0, s))
         e1.record(stream)
         torch.cuda.synchronize()
         ms_p = e0.elapsed_time(e1) / plain_steps
@@ -182,15 +189,12 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         out = ctypes.c_int32()
         e0.record(stream)
         for l in slots:
-            H.check(lib.lbm_run_blocked(ctypes.byref(runner.lats[l]), ctypes.byref(runner.flds[l]),
-                                        ctypes.cast(ctypes.addressof(runner.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
-                                                    ctypes.POINTER(H.NeemRule)), plain_steps, par, 0, 0, None,
-                                        ctypes.byref(out), s))
+            H.check(lib.lbm_run_blocked(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l),
+                                        plain_steps, par, 0, 0, None, ctypes.byref(out), s))
         e1.record(stream)
         torch.cuda.synchronize()
         ms_p2 = e0.elapsed_time(e1) / plain_steps
-        g.close()
-        runner.close()
+        plain.close()
         # accuracy at solve_steps: a fresh batch run to the end of the solve
         runner = BatchRunner(specs, storage="f32", grid_blocks=grid)
         runner.run(solve_steps)
@@ -204,7 +208,8 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         r64.close()
     sps_b = n_lattices / (ms_b * 1e-3 * solve_steps)
     sps_p = n_lattices / (ms_p * 1e-3 * solve_steps)
-    return {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid, "steps_per_solve": solve_steps,
+    return {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid,
+            "stored_nodes": sum(stored_nodes(sp) for sp in specs), "steps_per_solve": solve_steps,
             "ms_per_step_batched": round(ms_b, 4), "ms_per_step_plain": round(ms_p, 4),
             "mlups_batched": round(fluid / (ms_b * 1e-3) / 1e6, 1),
             "hbm_frac_batched": round(fluid * 72 / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -217,13 +222,16 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
             "max_abs_u_err_vs_fp64": err64}
 
 
-def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16", "bf16"), n_fp64=3):
+def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16", "bf16"), n_fp64=3,
+                short_steps=2000):
     """Poisson stream of 4:1 contraction-flow power-law solves (width ~ U{256..2048},
     height ~ U{64..512}, n ~ U[0.5, 1.5], TRT magic 1/4, u_in = 0.02, tau in
     [0.51, 2.0]), each run until ||u - u_(-tick)||/||u|| < tol or max_steps.
     Continuous batching (SolveServer) against the plain path (each solve alone,
     one after another in arrival order); shifted FP16/BF16 storage against
-    FP32; errors against the FP64 path on the first n_fp64 requests."""
+    FP32; errors against the FP64 path on the first n_fp64 requests, at the
+    step count they ran and at short_steps, beside the FP64 path's own
+    spread under a 1e-12 perturbation of the initial densities."""
     import numpy as np
     import torch
     from taichi_lbm import cases
@@ -277,6 +285,8 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
     out["speedup_solves_per_s"] = round(b["solves_per_s"] / out["plain_f32"]["solves_per_s"], 2)
     # precision: max |u - u_fp64| at the same step count, on the first requests
     err = {s: [] for s in storages}
+    err_short = {s: [] for s in storages}
+    sens = []
     for q in range(min(n_fp64, n_req)):
         nst = results["f32"][q][1]
         u64, _, _ = solve_alone(specs[q], nst, 0.0, tick, "f64")
@@ -287,7 +297,22 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
             else:   # a different stopping step: compare at the FP32 step count
                 u2, _, _ = solve_alone(specs[q], nst, 0.0, tick, s)
                 err[s].append(float(np.abs(u2 - u64).max()))
+        # the flow's own sensitivity: FP64 again from densities perturbed by
+        # 1e-12 (relative); an unsteady flow amplifies any difference, the
+        # storage format's rounding included
+        sp = copy.copy(specs[q])
+        sp.rho0 = specs[q].rho0 * (1.0 + 1e-12 * rng.standard_normal(specs[q].rho0.shape))
+        u64p, _, _ = solve_alone(sp, nst, 0.0, tick, "f64")
+        sens.append(float(np.abs(u64p - u64).max()))
+        # a short horizon, before any instability has grown
+        u64s, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, "f64")
+        for s in storages:
+            us, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, s)
+            err_short[s].append(float(np.abs(us - u64s).max()))
     out["max_abs_u_err_vs_fp64"] = {s: max(v) if v else None for s, v in err.items()}
+    out["fp64_sensitivity_1e-12"] = max(sens) if sens else None
+    out["max_abs_u_err_vs_fp64_short"] = {"steps": short_steps,
+                                          **{s: max(v) if v else None for s, v in err_short.items()}}
     return out
 
 
