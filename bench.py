@@ -316,6 +316,74 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
     return out
 
 
+def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
+    """The two other single-lattice configurations, timed with HIP events on
+    the launch stream:
+    - the 400 x 100 cylinder channel (BGK tau = 0.6, r = 10 at (80, 50),
+      bounce-back walls, VelocityBB inlet u = 0.05 over a bounce-back inlet
+      column, PressureNEEM outlet rho = 1), cyl_steps steps: us/step and MLUPS
+      (its 1.4 MB of populations stay in the caches, so no HBM fraction);
+    - two-component Shan-Chen separation, sc_N^2 periodic, MRT nu = 1/6,
+      G = [[0, 0.9], [0.9, 0]], rho_c = 1 + U(-0.01, 0.01) seeded per
+      component, sc_steps steps: ms/step, MLUPS, the algorithmic HBM fraction
+      (72 B x 2 components per node and step; psi traffic not credited) and
+      the FP64 mass of each component every sc_every steps against its
+      initial sum."""
+    import numpy as np
+    import torch
+    from taichi_lbm import cases
+    from taichi_lbm.stepper import Stepper
+    out = {}
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.cylinder(inlet="bb")
+        cases.initialise(case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"])
+    st.run(20)
+    torch.cuda.synchronize()
+    e0.record(s)
+    st.run(cyl_steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / cyl_steps
+    nf = len(case["fluid"].group.nodes)
+    red = st.reduce()
+    out["cylinder_400x100"] = {"steps": cyl_steps + 20, "us_per_step": round(ms * 1e3, 2),
+                               "mlups": round(nf / (ms * 1e-3) / 1e6, 1), "fluid_nodes": nf,
+                               "nonfinite": red["nonfinite"]}
+    del st, case
+    torch.cuda.empty_cache()
+
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.shan_chen(NX=sc_N, NY=sc_N, nc=2, gcross=0.9, seed=3)
+        cases.initialise(case)
+    m0 = case["rho"].astype(np.float64).sum(axis=(0, 1))
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
+    done, t_ms, drift = 0, 0.0, []
+    while done < sc_steps:
+        n = min(sc_every, sc_steps - done)
+        torch.cuda.synchronize()
+        e0.record(s)
+        st.run(n)
+        e1.record(s)
+        torch.cuda.synchronize()
+        t_ms += e0.elapsed_time(e1)
+        done += n
+        red = st.reduce()
+        drift.append([done] + [float(x) for x in np.abs(np.array(red["mass"]) - m0) / m0])
+    ms = t_ms / sc_steps
+    nodes = sc_N * sc_N
+    out[f"shan_chen_{sc_N}x{sc_N}x2"] = {
+        "steps": sc_steps, "ms_per_step": round(ms, 4), "mlups": round(nodes / (ms * 1e-3) / 1e6, 1),
+        "hbm_frac": round(nodes * 72 * 2 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "mass_drift_rel": drift, "nonfinite": red["nonfinite"]}
+    del st, case
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,11 +400,15 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--only-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--only-configs", action="store_true")
+    ap.add_argument("--configs-cyl-steps", type=int, default=10_000)
+    ap.add_argument("--configs-sc-steps", type=int, default=20_000)
     ap.add_argument("--headline-only", action="store_true",
                     help="only the 8192^2 MRT power-law timing (for profiling runs)")
     args = ap.parse_args()
     if args.headline_only:
-        args.no_cpu = args.no_trt = args.no_batched = args.no_serving = True
+        args.no_cpu = args.no_trt = args.no_batched = args.no_serving = args.no_configs = True
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
@@ -349,6 +421,9 @@ def main():
 
     if args.only_serving:
         print(json.dumps(serving_leg(args.serve_requests, 4.0, 5, 50_000, 1e-6, 200)), flush=True)
+        return
+    if args.only_configs:
+        print(json.dumps(configs_leg(args.configs_cyl_steps, sc_steps=args.configs_sc_steps)), flush=True)
         return
     if args.only_batched:
         print(json.dumps(batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)), flush=True)
@@ -399,6 +474,12 @@ def main():
         serving = serving_leg(args.serve_requests, 4.0, 5, 50_000, 1e-6, 200)
         log(f"serving: {serving}")
 
+    configs = None
+    if not args.no_configs:
+        torch.cuda.empty_cache()
+        configs = configs_leg(args.configs_cyl_steps, sc_steps=args.configs_sc_steps)
+        log(f"configs: {configs}")
+
     try:
         extra["copy_ceiling_gbs"] = round(copy_ceiling_gbs(), 1)
     except Exception as e:   # the bandwidth probe is informative only
@@ -435,6 +516,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "configs": configs,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)

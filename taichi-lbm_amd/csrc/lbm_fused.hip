@@ -11,9 +11,9 @@
 //            (:223-237), power-law shear_rate/relaxation_time
 //            (CollisionEngine.py:72-92, 157-178, 322-377) and the BGK / TRT /
 //            MRT collision (:96-110, 223-245, 424-451).
-// Shan-Chen runs add a first pass per step that produces psi from rho
-// (ExternalForce.py:70-75, 114-120); the collide pass then reads psi at the
-// 8 neighbours (ExternalForce.py:78-91, 124-145).
+// Shan-Chen steps (ExternalForce.py:70-145) run as one tiled launch that
+// stages psi of the tile and a one-node ring in LDS (k_sc below); FP64
+// storage keeps a psi pass of its own before the step.
 //
 // Memory layout: SoA planes, node (i, j) at i*NY + j, population plane
 // c*9 + k.  One thread owns one node; a wavefront covers 64 consecutive nodes
@@ -271,11 +271,29 @@ struct StepOut {
   R state[NC];                // new relaxation state (power law)
 };
 
+// Where a step reads Shan-Chen psi of the step at x + c_k (k = 0: x itself):
+// psi.at(c, k) for component c.  PsiPlanes: the psi plane set `set` in HBM,
+// periodic neighbours; PsiNone: lattices without Shan-Chen forces (never
+// read).  The LDS tile of the fused Shan-Chen kernel is a third source.
+template <typename R>
+struct PsiPlanes {
+  const Lat& a;
+  uint32_t set, i, j;
+  __device__ __forceinline__ R at(int c, int k) const {
+    const Nbr nb(i, j, a.nx, a.ny);
+    return ((const R*)a.psi[set])[(size_t)c * a.ps + (k == 0 ? nb.n : nb.src(opp(k)))];   // x - c_opp(k) = x + c_k
+  }
+};
+template <typename R>
+struct PsiNone {
+  __device__ __forceinline__ R at(int, int) const { return R(0); }
+};
+
 // S1..S6 of one step at node n = (i, j).  state_in is the per-node relaxation
 // state (BGK tau, TRT tau_sym, MRT 1/tau) for LBM_TAU_FIELD / _POWER_LAW.
-// Shan-Chen reads psi of this step at x + c_k from psi plane set `psi_set`.
-template <int COLL, int NC, typename R>
-__device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32_t n, uint32_t i, uint32_t j,
+// Shan-Chen reads psi of this step through `psi` (PsiPlanes and the like).
+template <int COLL, int NC, typename R, class PS>
+__device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t n, uint32_t i, uint32_t j,
                                           const R (&f)[NC][9], const R (&state_in)[NC], StepOut<NC, R>& o) {
   const uint32_t ps = a.ps;
   // S1 density (lbm.py:153-164) with the first moments, from the MRT moment
@@ -295,17 +313,15 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
 #pragma unroll
   for (int c = 0; c < NC; ++c) o.scx[c] = o.scy[c] = R(0);
   if (a.sc != LBM_SC_NONE) {
-    const Nbr nb(i, j, a.nx, a.ny);
-    const R* psi = (const R*)a.psi[psi_set];
     if (a.sc == LBM_SC_C1) {
       R Fx = R(0), Fy = R(0);
 #pragma unroll
       for (int k = 1; k < 9; ++k) {
-        const R v = psi[nb.src(opp(k))];  // x - c_opp(k) = x + c_k
+        const R v = psi.at(0, k);
         Fx += (wgt<R>(k) * (R)cx(k)) * v;
         Fy += (wgt<R>(k) * (R)cy(k)) * v;
       }
-      const R s = -(R)a.g_coh[0] * psi[n];
+      const R s = -(R)a.g_coh[0] * psi.at(0, 0);
       o.scx[0] = Fx * s;
       o.scy[0] = Fy * s;
     } else {
@@ -314,17 +330,16 @@ __device__ __forceinline__ void step_math(const Lat& a, uint32_t psi_set, uint32
         R Fx = R(0), Fy = R(0);
 #pragma unroll
         for (int k = 1; k < 9; ++k) {
-          const uint32_t xk = nb.src(opp(k));
 #pragma unroll
           for (int c2 = 0; c2 < NC; ++c2) {
             if (c2 == c1) continue;
-            const R v = psi[(size_t)c2 * ps + xk];
+            const R v = psi.at(c2, k);
             const R G = a.g_coh[c1 * LBM_MAX_NC + c2];
             Fx += (wgt<R>(k) * (R)cx(k)) * v * G;
             Fy += (wgt<R>(k) * (R)cy(k)) * v * G;
           }
         }
-        const R s = -psi[(size_t)c1 * ps + n];
+        const R s = -psi.at(c1, 0);
         o.scx[c1] = s * Fx;
         o.scy[c1] = s * Fy;
       }
@@ -497,7 +512,7 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
 #pragma unroll
   for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
   StepOut<NC, R> o;
-  step_math<COLL, NC, R>(a, p ^ 1u, n, i, j, f, st_in, o);
+  step_math<COLL, NC, R>(a, PsiPlanes<R>{a, p ^ 1u, i, j}, n, i, j, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[p ^ 1u];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -542,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
 //
 // Ring width: H = 1 streams; H = 2 when the lattice has node rules (a NEEM or
 // OpenNeumann node reads its interior neighbour's post-stream populations,
-// two nodes away).  Shan-Chen (psi needs its own pass) and FP64 storage run
+// two nodes away).  Shan-Chen (k_sc: one step per launch) and FP64 storage run
 // one step per launch.  The result of a launch goes to pop[parity^1] like a
 // one-step launch: a two-step launch flips the buffers once.
 // ---------------------------------------------------------------------------
@@ -566,9 +581,9 @@ constexpr int kDefaultTileCols = 32;
 // ring-extended tile fit one round of the block (block2_threads).  Narrow
 // tiles leave fewer lanes idle at a lattice's last tile column; wide ones
 // read longer rows.
-template <int H, int W>
+template <int H, int W, int NT_ = block2_threads(H, W)>
 struct TileShape {
-  static constexpr int NT = block2_threads(H, W);   // threads of the block
+  static constexpr int NT = NT_;            // threads of the block
   static constexpr int TI = NT / (W + 2 * H) - 2 * H;
   static constexpr int RI = TI + 2 * H;     // ring-extended rows
   static constexpr int RJ = W + 2 * H;      // ring-extended columns
@@ -623,6 +638,7 @@ struct Launch2 {
   const void* tau_in;         // single lattice, power law: state before step n
   void* tau_out;              // single lattice, power law: state after step n+1
   uint32_t tau_flip;          // batches, power law: read tau2, write tau (else the reverse)
+  int32_t write_psi;          // Shan-Chen: keep this step's psi in HBM (the last two steps of a run)
 };
 
 // Phase A inputs of one ring node: gathered populations and relaxation state.
@@ -722,7 +738,7 @@ __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typena
     }
   }
   StepOut<NC, R> o;
-  step_math<COLL, NC, R>(a, 0, n, gi, gj, in.f, in.st, o);
+  step_math<COLL, NC, R>(a, PsiNone<R>{}, n, gi, gj, in.f, in.st, o);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -813,7 +829,7 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
     for (int c = 0; c < NC; ++c)
       st_in[c] = pl ? L.st[c][x.q] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
     StepOut<NC, R> o;
-    step_math<COLL, NC, R>(a, 0, n, gi, gj, f, st_in, o);
+    step_math<COLL, NC, R>(a, PsiNone<R>{}, n, gi, gj, f, st_in, o);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -877,6 +893,133 @@ __global__ __launch_bounds__(block2_threads(ring_of(M), W)) void k_fused2(const 
   const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
   tile_two_steps<COLL, NC, S, M, W, BUF>(a, q, q.local_read != 0, ti * TileShape<H, W>::TI, tj * W, q.tau_in,
                                          q.tau_out, L, a.code, a.nx, a.ny);
+}
+
+// ---------------------------------------------------------------------------
+// Shan-Chen step with the psi halo in LDS (one step per launch).
+//
+// The force at x needs psi of this step at the 8 neighbours, and psi is a
+// function of the post-stream density (ExternalForce.py:114-120): a separate
+// pass would read every population twice per step.  Here a block owns a tile
+// of TI rows x 32 columns and one thread per node of the tile and a one-node
+// ring around it: every thread gathers its node (pull, bounce-back and node
+// rules, as the one-step kernel), sums the density and writes psi into LDS;
+// after one barrier the tile's threads take the force from LDS and finish
+// the step from the populations still in their registers.  The ring's
+// gathers are the only re-reads (mostly L2 hits: the ring is the next tile's
+// edge).  psi is written to HBM only on the last two steps of a run (the
+// host reads it, and the pressure observable uses the older one).  Nodes
+// outside the fluid group keep the psi the planes hold (their force terms
+// read it as the two-pass step does).  The arithmetic is the one-step
+// kernel's (same psi sum order, same force expressions): fields are
+// bit-identical to the two-pass path.
+// ---------------------------------------------------------------------------
+constexpr int kScCols = 32;
+
+template <int NT>
+using ScShape = TileShape<1, kScCols, NT>;
+
+// psi of component c at x + c_k from the LDS tile (slot q of x)
+template <typename R, class Shape>
+struct PsiTile {
+  const R* L;   // psi of component c at L[c * Shape::PN + slot]
+  uint32_t q;
+  __device__ __forceinline__ R at(int c, int k) const { return L[c * Shape::PN + q + Shape::back(k)]; }
+};
+
+// Eight waves per SIMD where the registers allow it without spilling (one
+// component, or two with MRT: 62 VGPRs; two-component BGK / TRT would spill)
+constexpr int sc_min_waves(int coll, int nc) { return (nc == 1 || coll == LBM_MRT) ? 8 : 1; }
+
+template <int COLL, int NC, int S, bool BUF, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves(COLL, NC)))) void k_sc(
+    const Lat a, const Launch2 q) {
+  using Shape = ScShape<NT>;
+  using T = typename Pop<S>::T;
+  using R = typename Pop<S>::R;
+  __shared__ R psiL[NC * Shape::PN];
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  const uint32_t i0 = ti * Shape::TI, j0 = tj * kScCols;
+  const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
+  const uint32_t r1 = threadIdx.x;
+  const uint32_t rr = r1 / (uint32_t)Shape::RJ, cc = r1 - rr * (uint32_t)Shape::RJ;
+  const uint32_t qs = Shape::slot(rr, cc);
+  const uint32_t pset = q.parity ^ 1u;   // the psi plane set of this step
+  const bool local = q.local_read != 0;
+  const T* __restrict__ src = (const T*)a.pop[q.parity];
+  const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+  R f[NC][9];
+  uint32_t code = 0, gi = 0, gj = 0;
+  if (r1 < (uint32_t)Shape::RN) {
+    ring_coords<1>(rr, cc, i0, j0, nx, ny, gi, gj);
+    const uint32_t n = gi * ny + gj;
+    code = a.code[n];
+    if (code & LBM_CODE_ACTIVE) {
+      if (local) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
+      } else {
+        if constexpr (BUF)
+          gather_buf<NC, S>(rsrc, ps, nx, ny, gi, gj, code, f);
+        else
+          gather<NC, S>(src, ps, Nbr(gi, gj, nx, ny), code & LBM_CODE_BOUNCE_MASK, f);
+        const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
+        if (e) {
+          const GlobalView<S> v{src, (const R*)a.mac[q.parity], a.code, ps, nx, ny};
+          apply_rule<NC, S>(a, v, a.neem[e - 1], v.node(gi, gj), n, f, nullptr);
+        }
+      }
+      // ShanChenForceC1/C2.update_psi: psi = psi(rho_c), rho_c summed k = 0..8
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s += f[c][k];
+        psiL[c * Shape::PN + qs] = sc_psi<R>(s, (R)a.psi_rho0[c]);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) psiL[c * Shape::PN + qs] = ((const R*)a.psi[pset])[(size_t)c * ps + n];
+    }
+  }
+  __syncthreads();
+  // the tile's own nodes (ring threads are done)
+  if (rr < 1 || rr > (uint32_t)Shape::TI || cc < 1 || cc > (uint32_t)kScCols) return;
+  if (i0 + rr - 1 >= nx || j0 + cc - 1 >= ny) return;   // past the lattice's last tile row / column
+  const uint32_t n = gi * ny + gj;                        // (inside the lattice gi, gj are unwrapped)
+  if (!(code & LBM_CODE_ACTIVE)) {
+    if (q.write_macro) {
+      R* mo = (R*)a.mac[pset];
+#pragma unroll
+      for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = R(0);
+    }
+    return;
+  }
+  R st_in[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
+  StepOut<NC, R> o;
+  step_math<COLL, NC, R>(a, PsiTile<R, Shape>{psiL, qs}, n, gi, gj, f, st_in, o);
+  T* __restrict__ dst = (T*)a.pop[pset];
+  const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if constexpr (BUF)
+        Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
+      else
+        Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+    }
+    if (a.tau_mode == LBM_TAU_POWER_LAW) ((R*)a.tau)[(size_t)c * ps + n] = o.state[c];
+    if (q.write_psi) ((R*)a.psi[pset])[(size_t)c * ps + n] = psiL[c * Shape::PN + qs];
+  }
+  store_macro<NC, R>(a, (R*)a.mac[pset], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
 // One tile of a batched two-step launch (32 bytes): what the tile's first
@@ -1288,6 +1431,13 @@ int tiles2_upload(lbm_batch* b, hipStream_t s) {
 }
 }  // namespace
 
+namespace lbm {
+// Shan-Chen runs through k_sc (defined below); false where that kernel is
+// not built for the lattice (then the two-pass path runs).
+bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
+            hipStream_t s, int* rc);
+}  // namespace lbm
+
 extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
                        int32_t nsteps, int32_t parity, int32_t first_local, void* stream) {
   using namespace lbm;
@@ -1297,6 +1447,7 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
   if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
   hipStream_t s = (hipStream_t)stream;
   const Lat a = make_lat(lat, fld, neem);
+  if (lat->shan_chen && run_sc(lat, a, nsteps, (uint32_t)parity, first_local, s, &rc)) return rc;
   Launch q = make_launch(a.N);
   KernelFn step = pick(lat->nc, lat->storage, lat->collision, MODE_STEP);
   KernelFn psik = pick(lat->nc, lat->storage, lat->collision, MODE_PSI);
@@ -1365,6 +1516,56 @@ uint32_t tile_rows(int h, int w) {
   return (uint32_t)(block2_threads(h, w) / (w + 2 * h) - 2 * h);
 }
 int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
+
+// ---- Shan-Chen ----------------------------------------------------------------
+constexpr int kScThreads = 512;   // 13 x 32 nodes per tile with the ring: 82 % of the threads own a tile node
+using ScFn = void (*)(const Lat, const Launch2);
+template <int NC, int S, bool BUF>
+ScFn pick_sc_coll(int coll) {
+  switch (coll) {
+    case LBM_BGK: return k_sc<LBM_BGK, NC, S, BUF, kScThreads>;
+    case LBM_TRT: return k_sc<LBM_TRT, NC, S, BUF, kScThreads>;
+    default: return k_sc<LBM_MRT, NC, S, BUF, kScThreads>;
+  }
+}
+// nullptr where k_sc is not built: FP64 storage (the high-precision path) and
+// shifted FP16 / BF16 storage (single lattices of the Python API store FP32)
+// run the two-pass step
+ScFn pick_sc(int nc, int store, int coll, bool buf) {
+  if (store != LBM_STORE_F32) return nullptr;
+  if (buf) return nc == 1 ? pick_sc_coll<1, LBM_STORE_F32, true>(coll) : pick_sc_coll<2, LBM_STORE_F32, true>(coll);
+  return nc == 1 ? pick_sc_coll<1, LBM_STORE_F32, false>(coll) : pick_sc_coll<2, LBM_STORE_F32, false>(coll);
+}
+
+bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
+            hipStream_t s, int* rc) {
+  ScFn fn = pick_sc(lat->nc, lat->storage, lat->collision, fits_buffer(lat));
+  if (!fn) return false;
+  using Shape = ScShape<kScThreads>;
+  Launch2 q;
+  memset(&q, 0, sizeof q);
+  q.tiles_i = (a.nx + Shape::TI - 1) / Shape::TI;
+  q.tiles_j = (a.ny + kScCols - 1) / kScCols;
+  const uint64_t nt = (uint64_t)q.tiles_i * q.tiles_j;
+  if (nt >= (1ull << 31)) {
+    *rc = fail("lattice too large for the Shan-Chen tiling");
+    return true;
+  }
+  q.xcd_remap = (nt % 8 == 0 && nt >= 64) ? 1 : 0;
+  uint32_t p = parity;
+  *rc = LBM_OK;
+  for (int t = 0; t < nsteps; ++t) {
+    q.parity = p;
+    q.step = t;
+    q.local_read = (t == 0 && first_local) ? 1 : 0;
+    q.write_macro = (t == nsteps - 1) ? 1 : 0;
+    q.write_psi = (t >= nsteps - 2) ? 1 : 0;
+    hipLaunchKernelGGL(fn, dim3((unsigned)nt), dim3(kScThreads), 0, s, a, q);
+    if ((*rc = hip_check(hipGetLastError(), "Shan-Chen launch"))) return true;
+    p ^= 1u;
+  }
+  return true;
+}
 }  // namespace lbm
 
 extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,

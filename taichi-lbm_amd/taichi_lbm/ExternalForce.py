@@ -101,6 +101,14 @@ class ShanChenModel:
             self._planes_valid = True
         return self._planes
 
+    def keep_previous(self, written):
+        """Before a one-step fused run that writes its psi into plane set
+        `written`: the pressure observable of that step reads the psi of the
+        step before from the other set, so if `written` holds it (the newest
+        psi), it is copied there first."""
+        if self._planes is not None and self._planes_idx == written:
+            self._planes[1 - written].copy_(self._planes[written])
+
     def after_run(self, idx):
         """The fused run left the newest psi in plane set idx."""
         self._planes_idx = idx

@@ -120,6 +120,10 @@ class Stepper:
             parity, first_local = 1 - st["f2_idx"], 1
         else:
             parity, first_local = st["f2_idx"], 0
+        if self.sc is not None and nsteps == 1:
+            # a launch writes its psi into plane set parity ^ 1; a run of two
+            # or more leaves the psi of its last two steps in both sets
+            self.sc.keep_previous(parity ^ 1)
         if self.blocked:
             tau_next = None
             if tau_mode == H.TAU_POWER_LAW:

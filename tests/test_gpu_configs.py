@@ -84,17 +84,22 @@ def test_porous_512_fused_vs_granular_gpu():
 @pytest.mark.parametrize("N,steps", [(256, 2000), (2048, 20000)])
 def test_two_component_shan_chen_mass_fp64(N, steps):
     """Two components, G_cross = 0.9, periodic, rho_c = 1 +- 1 %: the FP64 sums
-    of each component's mass, against their initial values.  Populations are
-    stored in FP32, so the sums drift by rounding; the bound is 2e-8 relative
-    per step (about a third of an FP32 ulp), the drift is printed.  2048^2 for
-    20k steps is the two-component separation configuration."""
+    of each component's mass, against their initial values.  The reference's
+    MRT returns to populations through M^-1 in FP32 (CollisionEngine.py:283-287),
+    whose first column sums to 1 + 2^-27 (9 x fl32(1/9)); with s_0 = 0 every
+    step then scales each node's mass by about 1 + 7.45e-9, so the sums grow
+    by that much per step plus rounding noise (the C++ restatement of the
+    reference drifts the same way).  Checked: growth, between 0.5x and 2x the
+    predicted bias.  2048^2 for 20k steps is the two-component separation
+    configuration."""
     case = _quiet(cases.shan_chen, NX=N, NY=N, nc=2, gcross=0.9, seed=3)
     _quiet(cases.initialise, case)
     st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
     m0 = case["rho"].astype(np.float64).sum(axis=(0, 1))
     st.run(steps)
     red = st.reduce()
-    drift = np.abs(np.array(red["mass"]) - m0) / m0
-    print(f"  N={N}: FP64 mass drift per component after {steps} steps = {drift}")
+    drift = (np.array(red["mass"]) - m0) / m0
+    bias = 2.0 ** -27 * steps
+    print(f"  N={N}: FP64 mass drift per component after {steps} steps = {drift} (M^-1 bias {bias:.3e})")
     assert red["nonfinite"] == 0
-    assert np.all(drift < 2e-8 * steps)
+    assert np.all(drift > 0.5 * bias) and np.all(drift < 2.0 * bias)

@@ -139,6 +139,28 @@ def test_shan_chen_fused_matches_oracle(nc):
 
 
 @pytest.mark.parametrize("nc", [1, 2])
+def test_shan_chen_tiles_and_split_runs(nc):
+    """The LDS-tiled Shan-Chen step on a lattice that leaves partial tiles in
+    both directions (70 x 45 against 13 x 32 tiles), run as 17 + 1 + 2 steps:
+    psi is kept in HBM only at the end of a run, and the pressure observable
+    reads the psi of the step before the last."""
+    print(f"case shan-chen tiles nc={nc}", flush=True)
+    case = cases.shan_chen(NX=70, NY=45, nc=nc, G=-5.0, gcross=0.9, seed=12)
+    cases.initialise(case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
+    sol = oracle_solver(case, np.float32)
+    done = 0
+    for n in (17, 1, 2):
+        st.run(n)
+        sol.run(n)
+        done += n
+        print(f"  after a run of {n} ({done} steps)", flush=True)
+        compare_state(case["lbf"], sol.lat, ("f", "f2", "rho", "vel", "SCforce", "pressure", "total_pressure"))
+        close("psi", case["lbf"].sc_field.psi_field.to_numpy(), sol.lat.psi)
+    assert done == 20
+
+
+@pytest.mark.parametrize("nc", [1, 2])
 def test_shan_chen_granular_matches_oracle(nc):
     case = cases.shan_chen(NX=24, NY=20, nc=nc, G=-5.0, gcross=0.9, seed=8)
     cases.initialise(case)
