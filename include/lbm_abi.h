@@ -158,7 +158,11 @@ const char* lbm_last_error(void);
  *   launch streams them.
  * The last launch also writes every node's rho/total_rho/u (and body force,
  * SC force, shear rate where enabled), as the reference's fields hold them
- * before the boundary pass.                                                */
+ * before the boundary pass.
+ * Shan-Chen: launch t computes psi of its step (FP32 storage: on chip, per
+ * tile) and leaves it for the active nodes in psi[1-p] only if it is one of
+ * the last two launches; nodes outside the fluid group keep the psi the
+ * planes hold (both planes should hold the same values there).            */
 int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
             int32_t nsteps, int32_t parity, int32_t first_local, void* stream);
 

@@ -8,7 +8,9 @@ graph).  Between ticks it
 * checks each solve's convergence -- the relative velocity change over the
   tick, ||u - u_prev||_2 / ||u||_2 over fluid nodes in FP64, against
   ``tol`` -- and its step budget, retires finished solves (reading their
-  velocity field) and frees their HBM;
+  velocity field) and frees their HBM; a solve whose velocity is no longer
+  finite (it diverged) is retired at once as failed, so it frees its slot
+  without disturbing the solves batched with it;
 * admits waiting requests in arrival order while their footprint fits the
   HBM budget and a slot is free, so work that arrives while others run
   joins the next tick.
@@ -42,6 +44,7 @@ class Request:
     steps: int = 0
     residual: float = float("inf")
     converged: bool = False
+    failed: bool = False            # diverged: non-finite velocity at a tick
     result: Optional[np.ndarray] = None
 
     @property
@@ -66,6 +69,7 @@ class ServeStats:
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
             "converged": int(sum(r.converged for r in self.completed)),
+            "failed": int(sum(r.failed for r in self.completed)),
             "ticks": self.ticks,
             "peak_resident": self.peak_resident,
             "peak_resident_bytes": self.peak_bytes,
@@ -149,11 +153,10 @@ class SolveServer:
                 now = clock() - t0
                 for slot, req in list(self.resident.items()):
                     req.steps += self.tick_steps
-                    d2, u2 = res[slot]
-                    req.residual = float(np.sqrt(d2 / u2)) if u2 > 0 else float("inf")
+                    req.residual, req.failed = _residual(*res[slot])
                     if req.residual < req.tol:
                         req.converged = True
-                    if req.converged or req.steps >= req.max_steps:
+                    if req.converged or req.failed or req.steps >= req.max_steps:
                         self._retire(slot, now)
                         stats.completed.append(req)
                 if on_tick:
@@ -168,19 +171,26 @@ class SolveServer:
         self.runner.close()
 
 
+def _residual(d2: float, u2: float):
+    """(||u - u_prev|| / ||u||, diverged) from the FP64 sums of a tick."""
+    if not (np.isfinite(d2) and np.isfinite(u2)):
+        return float("nan"), True
+    return (float(np.sqrt(d2 / u2)) if u2 > 0 else float("inf")), False
+
+
 def solve_alone(spec: LatticeSpec, max_steps: int, tol: float, tick_steps: int, storage: str = "f32",
                 grid_blocks: int = 0, blocked: bool = False):
     """The same request on its own: one lattice, the same tick / residual
-    schedule; by default the plain path (one step per launch), which gives
-    the same fields bit for bit as two steps per launch (blocked=True)."""
+    schedule (and the same stop on divergence); by default the plain path
+    (one step per launch), which gives the same fields bit for bit as two
+    steps per launch (blocked=True)."""
     r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks, blocked=blocked)
     steps, res = 0, float("inf")
     while True:
         r.run(tick_steps)
         steps += tick_steps
-        d2, u2 = r.residuals()[0]
-        res = float(np.sqrt(d2 / u2)) if u2 > 0 else float("inf")
-        if res < tol or steps >= max_steps:
+        res, failed = _residual(*r.residuals()[0])
+        if res < tol or failed or steps >= max_steps:
             break
     u = r.velocity(0)
     r.close()

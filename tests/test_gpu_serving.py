@@ -65,3 +65,28 @@ def test_shifted_storage_runs_in_the_server():
             err = np.abs(r.result - u64).max()
             print(f"  {storage} request {r.rid}: max |u - u_fp64| = {err:.2e}")
             assert err < (3e-4 if storage == "f16" else 3e-3)
+
+
+def test_diverged_solve_is_retired_without_disturbing_the_mix():
+    """The 400 x 100 cylinder with the reference's VelocityNEEM inlet as stated
+    blows up after about 170 steps (every restatement does); in a mix with a
+    healthy solve it is retired at the first tick whose velocity is not
+    finite, as failed, and the healthy solve still equals its run alone."""
+    with contextlib.redirect_stdout(io.StringIO()):
+        bad = spec_from_case(cases.cylinder(inlet="neem", name="diverging"))
+    good = _specs()[0]
+    reqs = [Request(0, 0.0, bad, max_steps=2000, tol=0.0), Request(1, 0.0, good, max_steps=2000, tol=0.0)]
+    srv = SolveServer(storage="f32", hbm_budget_bytes=1 << 28, max_slots=4, tick_steps=40)
+    stats = srv.serve(reqs)
+    srv.close()
+    done = {r.rid: r for r in stats.completed}
+    print(f"  diverging solve retired after {done[0].steps} steps (failed={done[0].failed})", flush=True)
+    assert done[0].failed and not done[0].converged
+    assert done[0].steps < 1000
+    assert not np.all(np.isfinite(done[0].result))
+    assert not done[1].failed and done[1].steps == 2000
+    assert stats.summary()["failed"] == 1
+    u, steps, res = solve_alone(good, 2000, 0.0, 40, "f32")
+    assert steps == 2000 and np.array_equal(u, done[1].result)
+    u, steps, res = solve_alone(bad, 2000, 0.0, 40, "f32")
+    assert steps == done[0].steps and np.isnan(res)
