@@ -3,6 +3,7 @@ infrastructure and CPU baseline only."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -10,6 +11,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "libcpu_ref.so"
+ASAN_LIB = HERE / "_build" / "libcpu_ref_asan.so"   # `make asan`; CPU_REF_LIB selects it
 
 
 class Params(C.Structure):
@@ -28,9 +30,9 @@ KIND = {"periodic": 1, "inside": 2, "bounceback": 3, "velocity_neem": 4, "pressu
         "open_neumann": 7}
 
 
-def build():
-    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
-    return LIB
+def build(target: str = "all"):
+    subprocess.run(["make", "-s", "-C", str(HERE), target], check=True)
+    return ASAN_LIB if target == "asan" else LIB
 
 
 _lib = None
@@ -39,9 +41,10 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not LIB.exists():
+        path = Path(os.environ.get("CPU_REF_LIB", str(LIB)))
+        if path == LIB and not LIB.exists():
             build()
-        _lib = C.CDLL(str(LIB))
+        _lib = C.CDLL(str(path))
         _lib.cref_run.restype = C.c_int
         _lib.cref_max_threads.restype = C.c_int
     return _lib

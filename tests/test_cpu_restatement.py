@@ -41,3 +41,43 @@ def test_reference_cylinder_boundaries(coll):
 def test_cylinder_velocity_bb_inlet():
     case = cases.cylinder(NX=48, NY=24, cx=12, cy=11, r=4, tau=0.6, u_in=0.05, collision="trt", inlet="bb")
     _compare(case, 30)
+
+
+_ASAN_RUN = r"""
+import contextlib, io, sys
+sys.path.insert(0, "taichi-lbm_amd"); sys.path.insert(0, ".")
+from oracle import cpu_ref
+from taichi_lbm import cases
+with contextlib.redirect_stdout(io.StringIO()):
+    runs = [(cases.cylinder(NX=48, NY=24, cx=12, cy=11, r=4, collision="mrt", tau_bulk=0.8), 20),
+            (cases.cylinder(NX=40, NY=20, cx=10, cy=10, r=3, inlet="bb"), 20),
+            (cases.cylinder_reference_bcs(NX=60, NY=24, tau=0.7, u_in=0.03, collision="bgk"), 20),
+            (cases.contraction(W=80, H=24, power_n=0.7), 20)]
+for case, n in runs:
+    cl = cpu_ref.from_case(case, nthreads=2)
+    cl.run(n)
+print("asan run done", flush=True)
+"""
+
+
+def test_restatement_is_clean_under_asan():
+    """The CPU restatement built with AddressSanitizer and UBSan (host code)
+    runs the cylinder, reference-boundary and power-law TRT contraction cases
+    without a report: every gather and table read stays in bounds.  (The
+    porous case is left out: its disk placement, plain Python, crawls under
+    the preloaded ASan allocator.)"""
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parent.parent
+    lib = cpu_ref.build("asan")
+    rt = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True, text=True).stdout.strip()
+    if not rt or not Path(rt).exists():
+        pytest.skip("no ASan runtime for the host compiler")
+    env = dict(os.environ, CPU_REF_LIB=str(lib), LD_PRELOAD=rt, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1", OMP_NUM_THREADS="2")
+    r = subprocess.run([sys.executable, "-c", _ASAN_RUN], cwd=root, env=env, capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0 and "asan run done" in r.stdout, (r.stdout[-2000:], r.stderr[-4000:])
+    assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
