@@ -964,17 +964,27 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
     ring_coords<1>(rr, cc, i0, j0, nx, ny, gi, gj);
     const uint32_t n = gi * ny + gj;
     code = a.code[n];
+    // the pulls issue without waiting for the code; the populations the code
+    // bounces are loaded again from the node itself afterwards
+    if (local) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
+    } else if constexpr (BUF) {
+      gather_buf<NC, S>(rsrc, ps, nx, ny, gi, gj, 0u, f);
+    } else {
+      gather<NC, S>(src, ps, Nbr(gi, gj, nx, ny), 0u, f);
+    }
     if (code & LBM_CODE_ACTIVE) {
-      if (local) {
+      if (!local) {
+        if (code & LBM_CODE_BOUNCE_MASK) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c)
+          for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
-      } else {
-        if constexpr (BUF)
-          gather_buf<NC, S>(rsrc, ps, nx, ny, gi, gj, code, f);
-        else
-          gather<NC, S>(src, ps, Nbr(gi, gj, nx, ny), code & LBM_CODE_BOUNCE_MASK, f);
+            for (int k = 1; k < 9; ++k)
+              if ((code >> (k - 1)) & 1u) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + opp(k)) * ps, n, opp(k));
+        }
         const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
         if (e) {
           const GlobalView<S> v{src, (const R*)a.mac[q.parity], a.code, ps, nx, ny};
