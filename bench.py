@@ -322,7 +322,10 @@ def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
     - the 400 x 100 cylinder channel (BGK tau = 0.6, r = 10 at (80, 50),
       bounce-back walls, VelocityBB inlet u = 0.05 over a bounce-back inlet
       column, PressureNEEM outlet rho = 1), cyl_steps steps: us/step and MLUPS
-      (its 1.4 MB of populations stay in the caches, so no HBM fraction);
+      (its 1.4 MB of populations stay in the caches, so no HBM fraction),
+      through the Stepper (one launch per two steps from the host) and as a
+      one-lattice batch replayed from a HIP graph (about the same: the
+      launches are bound by their own latency on the GPU, not by the host);
     - two-component Shan-Chen separation, sc_N^2 periodic, MRT nu = 1/6,
       G = [[0, 0.9], [0.9, 0]], rho_c = 1 + U(-0.01, 0.01) seeded per
       component, sc_steps steps: ms/step, MLUPS, the algorithmic HBM fraction
@@ -354,6 +357,32 @@ def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
                                "mlups": round(nf / (ms * 1e-3) / 1e6, 1), "fluid_nodes": nf,
                                "nonfinite": red["nonfinite"]}
     del st, case
+    # the same lattice as a one-lattice batch replayed from a HIP graph (no
+    # per-launch host cost), on a stream of its own (graphs are not captured
+    # on the legacy default stream)
+    from taichi_lbm.batch import BatchRunner, spec_from_case
+    with contextlib.redirect_stdout(io.StringIO()):
+        spec = spec_from_case(cases.cylinder(inlet="bb"))
+    gs = torch.cuda.Stream()
+    with torch.cuda.stream(gs):
+        r = BatchRunner([spec], storage="f32")
+        r.run(20)
+        g = r.capture(200)
+        g.replay()
+        torch.cuda.synchronize()
+        reps = max(1, cyl_steps // 200)
+        e0.record(gs)
+        for _ in range(reps):
+            g.replay()
+        e1.record(gs)
+        torch.cuda.synchronize()
+        msg = e0.elapsed_time(e1) / (reps * 200)
+        g.close()
+        u = r.velocity(0)
+        r.close()
+    out["cylinder_400x100"].update({"graph_us_per_step": round(msg * 1e3, 2),
+                                    "graph_mlups": round(nf / (msg * 1e-3) / 1e6, 1),
+                                    "graph_steps": reps * 200 + 220, "graph_finite": bool(np.isfinite(u).all())})
     torch.cuda.empty_cache()
 
     with contextlib.redirect_stdout(io.StringIO()):
@@ -361,9 +390,10 @@ def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
         cases.initialise(case)
     m0 = case["rho"].astype(np.float64).sum(axis=(0, 1))
     st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
-    done, t_ms, drift = 0, 0.0, []
-    while done < sc_steps:
-        n = min(sc_every, sc_steps - done)
+    st.run(20)                 # warm-up: the first run also compiles the node codes on the host
+    done, t_ms, drift = 20, 0.0, []
+    while done < sc_steps + 20:
+        n = min(sc_every, sc_steps + 20 - done)
         torch.cuda.synchronize()
         e0.record(s)
         st.run(n)
@@ -376,7 +406,7 @@ def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
     ms = t_ms / sc_steps
     nodes = sc_N * sc_N
     out[f"shan_chen_{sc_N}x{sc_N}x2"] = {
-        "steps": sc_steps, "ms_per_step": round(ms, 4), "mlups": round(nodes / (ms * 1e-3) / 1e6, 1),
+        "steps": sc_steps + 20, "timed_steps": sc_steps, "ms_per_step": round(ms, 4), "mlups": round(nodes / (ms * 1e-3) / 1e6, 1),
         "hbm_frac": round(nodes * 72 * 2 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "mass_drift_rel": drift, "nonfinite": red["nonfinite"]}
     del st, case
