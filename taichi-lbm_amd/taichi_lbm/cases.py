@@ -220,17 +220,22 @@ def stepper(case):
 
 
 def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt", nu=1.0 / 6.0, rho_mean=None,
-              amp=None, name="shan_chen"):
+              amp=None, name="shan_chen", disk=None):
     """Periodic Shan-Chen lattice.  nc=1: ShanChenForceC1 with cohesion G, rho =
     rho_mean + amp*U(0,1) (cases/Bubble or two-phase separation.py:174);
     nc=2: ShanChenForceC2 with G = [[0, gcross], [gcross, 0]] and each rho_c =
-    1 + amp*U(-1, 1), seeded per component.  psi = SC_psi(rho0=1)."""
+    1 + amp*U(-1, 1), seeded per component.  psi = SC_psi(rho0=1).
+    disk=(cx, cy, r): a solid disk (Mask_cricle_identify) with a
+    BounceBackWall group around it; psi stays 0 on its nodes (update_psi
+    writes the group only)."""
     from .EOS import SC_psi
     from .ExternalForce import ShanChenForceC1, ShanChenForceC2
     from .SCBoundaryCondition import BoundaryEngineSC, PeriodicPsi_BT, PeriodicPsi_LR
     lbf = LBField(name, NX, NY, nc)
     bce = BoundaryEngine()
     clf = BoundaryClassifier((NX, NY))
+    if disk is not None:
+        bce.Mask_cricle_identify(lbf, *disk)
     m = lbf.mask.arr
     fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
     fluid.precompute(clf)
@@ -238,6 +243,11 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
     stream_bc = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
     stream_bc.precompute(clf)
     bce.add_boundary_condition("stream", stream_bc)
+    if disk is not None:
+        wmap = solid_neighbour_map(m)
+        wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+        wall.precompute(clf)
+        bce.add_boundary_condition("wall", wall)
     lbf.neighbor_classify()
     coll = make_collision(collision, fluid.group, NX, NY, nc, [nu] * nc, nu_bulk=[nu] * nc)
     rng = np.random.default_rng(seed)
@@ -262,13 +272,14 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
     bsc.add_boundary_condition("periodicBT", PeriodicPsi_BT())
     psi = np.zeros((NX + 2, NY + 2, nc), np.float32)
     psi[1:-1, 1:-1, :] = SC_psi(1.0).get_psi(rho)
+    psi[1:-1, 1:-1, :][m == -1] = 0.0
     psi = np.pad(psi[1:-1, 1:-1, :], ((1, 1), (1, 1), (0, 0)), mode="wrap")
     macro = MacroscopicEngine(fluid.group)
     vel = np.zeros((NX, NY, 2), np.float32)
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
                 psi=psi, bsc=bsc, sc_spec=dict(kind="c1" if nc == 1 else "c2", g_coh=g_spec, rho0=(1.0,) * nc),
                 spec=dict(kind="shan_chen", NX=NX, NY=NY, nc=nc, G=G, gcross=gcross, seed=seed,
-                          collision=collision, nu=nu))
+                          collision=collision, nu=nu, disk=disk))
 
 
 

@@ -161,6 +161,23 @@ def test_shan_chen_tiles_and_split_runs(nc):
 
 
 @pytest.mark.parametrize("nc", [1, 2])
+def test_shan_chen_with_solid_disk(nc):
+    """A solid disk with a BounceBackWall ring inside a periodic Shan-Chen
+    lattice: the tiled step's bounce-back populations (loaded again after the
+    early pulls) and psi = 0 on the solid nodes, against the oracle."""
+    print(f"case shan-chen disk nc={nc}", flush=True)
+    case = cases.shan_chen(NX=60, NY=44, nc=nc, G=-5.0, gcross=0.9, seed=14, disk=(30, 22, 7))
+    cases.initialise(case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], boundary_engine_sc=case["bsc"])
+    st.run(12)
+    st.run(13)
+    sol = oracle_solver(case, np.float32)
+    sol.run(25)
+    compare_state(case["lbf"], sol.lat, ("f", "f2", "rho", "vel", "total_rho", "SCforce", "pressure"))
+    close("psi", case["lbf"].sc_field.psi_field.to_numpy(), sol.lat.psi)
+
+
+@pytest.mark.parametrize("nc", [1, 2])
 def test_shan_chen_granular_matches_oracle(nc):
     case = cases.shan_chen(NX=24, NY=20, nc=nc, G=-5.0, gcross=0.9, seed=8)
     cases.initialise(case)
