@@ -87,8 +87,10 @@ def copy_ceiling_gbs(nbytes=4 << 30, reps=10):
     return 2 * nbytes / sec / 1e9
 
 
-def cpu_baseline(case, steps):
-    """The reference-algorithm CPU restatement on the same lattice."""
+def cpu_baseline(case, steps, one_thread_steps=1):
+    """The reference-algorithm CPU restatement on the same lattice: `steps`
+    steps on up to 16 threads (the value), then a short single-thread sample;
+    the host's CPU model and available CPUs are recorded."""
     from oracle import cpu_ref
     nthreads = min(16, os.cpu_count() or 1)
     cl = cpu_ref.from_case(case, nthreads=nthreads)
@@ -97,12 +99,25 @@ def cpu_baseline(case, steps):
     cl.run(steps)
     dt = time.perf_counter() - t
     n_fluid = len(case["fluid"].group.nodes)
+    # the same lattice continued on one thread, a short sample
+    cl.p.nthreads = 1
+    t = time.perf_counter()
+    cl.run(one_thread_steps)
+    dt1 = time.perf_counter() - t
+    model = ""
+    try:
+        model = next((l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(n_fluid * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": nthreads,
             "kind": "port",
             "sample": f"{steps} steps (after 1 warm-up step) of the same {case['lbf'].NX}x{case['lbf'].NY} "
                       f"MRT power-law lattice; C++/OpenMP restatement of the reference's unfused kernel "
                       f"sequence (AoS layout, group-index loops), not Taichi",
-            "seconds": round(dt, 2)}
+            "seconds": round(dt, 2),
+            "one_thread": {"value": round(n_fluid * one_thread_steps / dt1 / 1e6, 2), "unit": "MLUPS",
+                           "steps": one_thread_steps, "seconds": round(dt1, 2)},
+            "host": {"cpu_model": model, "cpus_available": len(os.sched_getaffinity(0))}}
 
 
 def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100, n_fp64=16):
