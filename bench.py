@@ -49,7 +49,10 @@ def time_gpu(case, steps, warmup):
     from taichi_lbm.stepper import Stepper
     with contextlib.redirect_stdout(sys.stderr):
         cases.initialise(case)
-    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True)
+    # tile width tuned on the warm-up steps (16 + 4 of them; a shorter warm-up
+    # keeps the library's default width, so the timed steps never tune)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True,
+                 tile_cols="auto" if warmup >= 20 else 0)
     st.run(warmup)
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()      # lbm_run launches on this stream
@@ -62,6 +65,8 @@ def time_gpu(case, steps, warmup):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
     red = st.reduce()
+    red["tile_cols"] = st.tile_cols or 32
+    red["tile_timings_ms"] = {w: round(v, 4) for w, v in st.tune_timings.items()}
     return ms, red
 
 
@@ -483,6 +488,7 @@ def main():
     mlups = n_fluid / (ms * 1e-3) / 1e6
     achieved = n_fluid * BYTES_POWER_LAW / (ms * 1e-3) / 1e9
     extra = {"mlups_all_nodes": round(N / (ms * 1e-3) / 1e6, 1), "fluid_fraction": round(n_fluid / N, 4),
+             "tile_cols": red["tile_cols"], "tile_timings_ms": red["tile_timings_ms"],
              "n_disks": case["spec"]["n_disks"], "mass_fp64": red["mass"][0], "nonfinite": red["nonfinite"]}
 
     if not args.no_trt:
@@ -499,6 +505,7 @@ def main():
         ms_t, red_t = time_gpu(tcase, args.steps, args.warmup)
         log(f"TRT: {ms_t:.4f} ms/step")
         extra["trt_ms_per_step"] = round(ms_t, 5)
+        extra["trt_tile_cols"] = red_t["tile_cols"]
         extra["trt_mlups"] = round(n_fluid / (ms_t * 1e-3) / 1e6, 1)
         extra["trt_hbm_frac"] = round(n_fluid * BYTES_POWER_LAW / (ms_t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         tcase["coll"]._state = None
@@ -557,7 +564,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_fluid_node": BYTES_POWER_LAW, "steps_per_launch": steps_per_launch,
-                     "kernel": "k_fused2 (two steps per launch, LDS-staged)"},
+                     "kernel": f"k_fused2 (two steps per launch, LDS-staged, {red['tile_cols']}-column tiles)"},
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

@@ -43,10 +43,16 @@ class Stepper:
                  boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0):
         """blocked: run two steps per launch where the lattice allows it
         (lbm_run_blocked; bit-identical to one step per launch), on tiles of
-        tile_cols = 16, 32 or 64 columns (0: the library's default)."""
+        tile_cols = 16, 32 or 64 columns (0: the library's default; "auto":
+        the first run long enough times TUNE_STEPS steps with each of
+        TUNE_CANDIDATES, after a two-step launch of that width, and keeps the
+        fastest -- the fields do not depend on the width, so those steps are
+        ordinary steps of the run)."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
-        self.tile_cols = int(tile_cols)
+        self._tune = tile_cols == "auto"
+        self.tile_cols = 0 if self._tune else int(tile_cols)
+        self.tune_timings = {}
         self._tau_next = None
         self.macro = macroscopic_engine
         self.coll = collision_engine
@@ -105,11 +111,36 @@ class Stepper:
         F.code = H.ptr(self.code)
         return L, F
 
+    TUNE_CANDIDATES = (32, 64)
+    TUNE_STEPS = 8            # timed steps per candidate (even: two-step launches only)
+
     def run(self, nsteps: int):
-        """Advance ``nsteps`` steps (asynchronous on the current stream)."""
+        """Advance ``nsteps`` steps (asynchronous on the current stream; a
+        tuning run with tile_cols="auto" synchronises)."""
         nsteps = int(nsteps)
         if nsteps <= 0:
             return
+        per = 2 + self.TUNE_STEPS
+        if self._tune and self.blocked and nsteps >= len(self.TUNE_CANDIDATES) * per:
+            self._tune = False
+            t = torch()
+            s = t.cuda.current_stream()
+            for w in self.TUNE_CANDIDATES:
+                self.tile_cols = w
+                e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+                self._run(2)                       # one launch of this width first (loads its code)
+                e0.record(s)
+                self._run(self.TUNE_STEPS)
+                e1.record(s)
+                t.cuda.synchronize()
+                self.tune_timings[w] = e0.elapsed_time(e1) / self.TUNE_STEPS
+            self.tile_cols = min(self.tune_timings, key=self.tune_timings.get)
+            nsteps -= len(self.TUNE_CANDIDATES) * per
+            if nsteps <= 0:
+                return
+        self._run(nsteps)
+
+    def _run(self, nsteps: int):
         lbf = self.lbf
         if self._compiled_version != lbf.geometry_version:
             self._compile()

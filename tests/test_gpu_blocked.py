@@ -140,3 +140,20 @@ def test_pulls_from_outside_the_fluid_group(coll, tile_cols):
     sb.run(11)
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho"))
     assert np.all(np.isfinite(a["lbf"].f2.to_numpy()))
+
+
+def test_autotuned_tile_width_changes_nothing():
+    """tile_cols="auto": the first long run times both widths on ordinary
+    steps of the run and keeps the faster; the fields equal the one-step
+    path's bit for bit (power law, so the state buffers alternate too)."""
+    print("case porous 200x136 mrt power law, autotuned width", flush=True)
+    build = lambda: cases.porous(NX=200, NY=136, seed=5, collision="mrt", rmin=3.0, rmax=8.0, g=(2e-5, 1e-6))
+    a, b, sa, sb = _pair(build, True, "auto")
+    sa.run(27)
+    sb.run(27)
+    assert set(sa.tune_timings) == set(Stepper.TUNE_CANDIDATES) and sa.tile_cols in Stepper.TUNE_CANDIDATES
+    _same(a, b, ("f", "f2", "rho", "vel", "total_rho", "shear_rate", "body_force"))
+    assert np.array_equal(a["coll"].diag.to_numpy(), b["coll"].diag.to_numpy())
+    sa.run(6)
+    sb.run(6)
+    _same(a, b, ("f", "f2", "rho", "vel"))
