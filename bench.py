@@ -49,10 +49,10 @@ def time_gpu(case, steps, warmup):
     from taichi_lbm.stepper import Stepper
     with contextlib.redirect_stdout(sys.stderr):
         cases.initialise(case)
-    # tile width tuned on the warm-up steps (16 + 4 of them; a shorter warm-up
+    # tile width tuned on the warm-up steps (56 of them; a shorter warm-up
     # keeps the library's default width, so the timed steps never tune)
     st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True,
-                 tile_cols="auto" if warmup >= 20 else 0)
+                 tile_cols="auto" if warmup >= Stepper.tune_steps() else 0)
     st.run(warmup)
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()      # lbm_run launches on this stream
@@ -438,7 +438,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-steps", type=int, default=24)

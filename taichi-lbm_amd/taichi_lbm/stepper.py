@@ -44,10 +44,10 @@ class Stepper:
         """blocked: run two steps per launch where the lattice allows it
         (lbm_run_blocked; bit-identical to one step per launch), on tiles of
         tile_cols = 16, 32 or 64 columns (0: the library's default; "auto":
-        the first run long enough times TUNE_STEPS steps with each of
-        TUNE_CANDIDATES, after a two-step launch of that width, and keeps the
-        fastest -- the fields do not depend on the width, so those steps are
-        ordinary steps of the run)."""
+        the first run of at least tune_steps() steps times TUNE_STEPS steps
+        with each of TUNE_CANDIDATES, twice, and keeps the fastest -- the
+        fields do not depend on the width, so those steps are ordinary steps
+        of the run)."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
         self._tune = tile_cols == "auto"
@@ -112,7 +112,12 @@ class Stepper:
         return L, F
 
     TUNE_CANDIDATES = (32, 64)
-    TUNE_STEPS = 8            # timed steps per candidate (even: two-step launches only)
+    TUNE_STEPS = 12           # timed steps per sample (even: two-step launches only)
+
+    @classmethod
+    def tune_steps(cls) -> int:
+        """Steps a tuning run takes (tile_cols="auto")."""
+        return 2 * len(cls.TUNE_CANDIDATES) * (2 + cls.TUNE_STEPS)
 
     def run(self, nsteps: int):
         """Advance ``nsteps`` steps (asynchronous on the current stream; a
@@ -120,22 +125,26 @@ class Stepper:
         nsteps = int(nsteps)
         if nsteps <= 0:
             return
-        per = 2 + self.TUNE_STEPS
-        if self._tune and self.blocked and nsteps >= len(self.TUNE_CANDIDATES) * per:
+        if self._tune and self.blocked and nsteps >= self.tune_steps():
+            # each width twice, in the order A B B A (the first runs of a
+            # lattice are slower: first touches of its buffers); each sample
+            # after one launch of that width (which also loads its code)
             self._tune = False
             t = torch()
             s = t.cuda.current_stream()
-            for w in self.TUNE_CANDIDATES:
+            samples = {w: [] for w in self.TUNE_CANDIDATES}
+            for w in self.TUNE_CANDIDATES + tuple(reversed(self.TUNE_CANDIDATES)):
                 self.tile_cols = w
                 e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
-                self._run(2)                       # one launch of this width first (loads its code)
+                self._run(2)
                 e0.record(s)
                 self._run(self.TUNE_STEPS)
                 e1.record(s)
                 t.cuda.synchronize()
-                self.tune_timings[w] = e0.elapsed_time(e1) / self.TUNE_STEPS
+                samples[w].append(e0.elapsed_time(e1) / self.TUNE_STEPS)
+            self.tune_timings = {w: min(v) for w, v in samples.items()}
             self.tile_cols = min(self.tune_timings, key=self.tune_timings.get)
-            nsteps -= len(self.TUNE_CANDIDATES) * per
+            nsteps -= self.tune_steps()
             if nsteps <= 0:
                 return
         self._run(nsteps)

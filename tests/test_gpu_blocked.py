@@ -149,8 +149,9 @@ def test_autotuned_tile_width_changes_nothing():
     print("case porous 200x136 mrt power law, autotuned width", flush=True)
     build = lambda: cases.porous(NX=200, NY=136, seed=5, collision="mrt", rmin=3.0, rmax=8.0, g=(2e-5, 1e-6))
     a, b, sa, sb = _pair(build, True, "auto")
-    sa.run(27)
-    sb.run(27)
+    n = Stepper.tune_steps() + 3
+    sa.run(n)
+    sb.run(n)
     assert set(sa.tune_timings) == set(Stepper.TUNE_CANDIDATES) and sa.tile_cols in Stepper.TUNE_CANDIDATES
     _same(a, b, ("f", "f2", "rho", "vel", "total_rho", "shear_rate", "body_force"))
     assert np.array_equal(a["coll"].diag.to_numpy(), b["coll"].diag.to_numpy())
