@@ -78,7 +78,7 @@ class ServeStats:
 
 class SolveServer:
     def __init__(self, storage: str = "f32", hbm_budget_bytes: int = 8 << 30, max_slots: int = 256,
-                 tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True):
+                 tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True, tile_cols: int = 0):
         if tick_steps % 4:
             raise ValueError("tick_steps must be a multiple of 4 (graph replays of two-step launches keep one "
                              "parity schedule)")
@@ -91,7 +91,8 @@ class SolveServer:
         # is captured again whenever the resident set changes (the two-step
         # tiling, its table and mode follow the resident lattices)
         self.runner = BatchRunner([], storage=storage, grid_blocks=grid_blocks,
-                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=True)
+                                  capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=True,
+                                  tile_cols=tile_cols)
         self.changed = False
         self.tick_steps = int(tick_steps)
         self.keep_results = keep_results
