@@ -111,8 +111,10 @@ class Stepper:
         F.code = H.ptr(self.code)
         return L, F
 
-    TUNE_CANDIDATES = (32, 64)
+    TUNE_CANDIDATES = (32, 64)    # the first is the library's default width
     TUNE_STEPS = 12           # timed steps per sample (even: two-step launches only)
+    TUNE_MARGIN = 0.02        # another width must beat the default by this much (short samples
+                              # read a few percent slow; closer calls keep the default)
 
     @classmethod
     def tune_steps(cls) -> int:
@@ -143,7 +145,11 @@ class Stepper:
                 t.cuda.synchronize()
                 samples[w].append(e0.elapsed_time(e1) / self.TUNE_STEPS)
             self.tune_timings = {w: min(v) for w, v in samples.items()}
-            self.tile_cols = min(self.tune_timings, key=self.tune_timings.get)
+            best = min(self.tune_timings, key=self.tune_timings.get)
+            default = self.TUNE_CANDIDATES[0]
+            if self.tune_timings[best] > (1.0 - self.TUNE_MARGIN) * self.tune_timings[default]:
+                best = default
+            self.tile_cols = best
             nsteps -= self.tune_steps()
             if nsteps <= 0:
                 return
