@@ -542,8 +542,11 @@ def main():
     if pmc.exists():
         try:
             rec = json.loads(pmc.read_text())
-            traffic = rec.get("mrt_step_bytes_per_launch")
-            steps_per_launch = rec.get("steps_per_launch", steps_per_launch)
+            # the record's counters belong to one kernel (tile width): use them
+            # only if that is the kernel this run timed
+            if f", {red['tile_cols']}, true>" in rec.get("kernel", ""):
+                traffic = rec.get("mrt_step_bytes_per_launch")
+                steps_per_launch = rec.get("steps_per_launch", steps_per_launch)
         except Exception:
             traffic = None
 
