@@ -326,8 +326,8 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
         sens.append(float(np.abs(u64p - u64).max()))
         # a short horizon, before any instability has grown
         u64s, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, "f64")
-        for s in storages:
-            us, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, s)
+        for s in storages:   # two steps per launch, as the server runs them
+            us, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, s, blocked=True)
             err_short[s].append(float(np.abs(us - u64s).max()))
     out["max_abs_u_err_vs_fp64"] = {s: max(v) if v else None for s, v in err.items()}
     out["fp64_sensitivity_1e-12"] = max(sens) if sens else None
