@@ -315,7 +315,7 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
             if ns == nst:
                 err[s].append(float(np.abs(u - u64).max()))
             else:   # a different stopping step: compare at the FP32 step count
-                u2, _, _ = solve_alone(specs[q], nst, 0.0, tick, s)
+                u2, _, _ = solve_alone(specs[q], nst, 0.0, tick, s, blocked=True)
                 err[s].append(float(np.abs(u2 - u64).max()))
         # the flow's own sensitivity: FP64 again from densities perturbed by
         # 1e-12 (relative); an unsteady flow amplifies any difference, the
