@@ -460,7 +460,7 @@ def main():
     if args.headline_only:
         args.no_cpu = args.no_trt = args.no_batched = args.no_serving = args.no_configs = True
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+        print(f"bench.py runs on one GPU: there is no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
     import torch
