@@ -446,6 +446,9 @@ def main():
     ap.add_argument("--no-trt", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-lattices", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=1000)          # timed graph-replayed steps
+    ap.add_argument("--batch-plain-steps", type=int, default=100)     # timed steps of the plain paths
+    ap.add_argument("--batch-solve-steps", type=int, default=10_000)  # steps of the accuracy run
     ap.add_argument("--only-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--only-serving", action="store_true")
@@ -476,7 +479,7 @@ def main():
         print(json.dumps(configs_leg(args.configs_cyl_steps, sc_steps=args.configs_sc_steps)), flush=True)
         return
     if args.only_batched:
-        print(json.dumps(batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)), flush=True)
+        print(json.dumps(batched_leg(args.batch_lattices, 4, args.batch_steps, args.batch_plain_steps, args.batch_solve_steps)), flush=True)
         return
     t0 = time.time()
     case = build_case(args.size, "mrt", args.seed)
@@ -517,7 +520,7 @@ def main():
         case["lbf"]._st = None
         case["coll"]._state = None
         torch.cuda.empty_cache()
-        batched = batched_leg(args.batch_lattices, 4, 1000, 100, 10_000)
+        batched = batched_leg(args.batch_lattices, 4, args.batch_steps, args.batch_plain_steps, args.batch_solve_steps)
         log(f"batched: {batched}")
 
     serving = None
