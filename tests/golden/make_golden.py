@@ -1,0 +1,113 @@
+"""Writes the golden fixtures of the D2Q9 step (tests/golden/*.npz).
+
+Each fixture is one small lattice built with the drop-in case builders
+(taichi_lbm.cases: the reference's harnesses `cases/Circular cylinder.py`,
+`cases/Power Law Fluid.py`, `cases/Bubble or two-phase separation.py`),
+advanced n steps by the oracle in the reference's unfused order
+(`lbm_solve`: density, pressure, force_density, velocity, [shear_rate,
+relaxation_time,] apply, boundary conditions):
+
+* inputs: mask, initial rho / vel (and psi for Shan-Chen);
+* out32_*: the FP32 oracle (the reference's arithmetic in float32, the
+  parameters as the reference computes them);
+* out64_*: the FP64 oracle given the FP32-rounded parameters the HIP library
+  receives, i.e. the exact solution of the discrete problem both FP32 paths
+  integrate (populations too for one-component lattices without Shan-Chen,
+  the ones the FP64-storage batch path runs).
+
+The oracle's own invariants and closed forms are pinned in
+tests/test_oracle_kat.py; these fixtures freeze its outputs so that a later
+change to the oracle, the case builders or the HIP path is compared against
+the same numbers.  Regenerate with
+
+    python tests/golden/make_golden.py
+
+(CPU only, a few seconds).
+"""
+from __future__ import annotations
+
+import contextlib
+import io
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+for p in (ROOT, ROOT / "taichi-lbm_amd", ROOT / "tests"):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+# name -> (builder, kwargs, steps).  Sizes keep each fixture under ~160 KB.
+CASES = {
+    "cylinder_bgk_48x24": ("cylinder", dict(NX=48, NY=24, cx=12, cy=11, r=4, tau=0.6, u_in=0.05,
+                                            collision="bgk"), 30),
+    "cylinder_mrt_48x24": ("cylinder", dict(NX=48, NY=24, cx=12, cy=11, r=4, tau=0.6, u_in=0.05,
+                                            collision="mrt", tau_bulk=0.8), 30),
+    "cylinder_trt_vbb_48x24": ("cylinder", dict(NX=48, NY=24, cx=12, cy=11, r=4, tau=0.6, u_in=0.05,
+                                                collision="trt", inlet="bb"), 30),
+    "cylinder_refbcs_bgk_60x24": ("cylinder_reference_bcs", dict(NX=60, NY=24, tau=0.7, u_in=0.03,
+                                                                 collision="bgk"), 30),
+    "porous_powerlaw_mrt_48x40": ("porous", dict(NX=48, NY=40, seed=2, collision="mrt", rmin=3.0, rmax=6.0,
+                                                 g=(2e-5, 1e-6)), 20),
+    "porous_powerlaw_trt_48x40": ("porous", dict(NX=48, NY=40, seed=4, collision="trt", rmin=3.0, rmax=6.0,
+                                                 g=(2e-5, 0.0)), 20),
+    "contraction_powerlaw_80x24": ("contraction", dict(W=80, H=24, power_n=0.7), 20),
+    "shan_chen_c1_32x32": ("shan_chen", dict(NX=32, NY=32, nc=1, G=-5.0, seed=7), 20),
+    "shan_chen_c2_32x24": ("shan_chen", dict(NX=32, NY=24, nc=2, gcross=0.9, seed=9), 20),
+}
+
+
+def build(name):
+    """The case of fixture `name` (host side only; no device needed)."""
+    from taichi_lbm import cases
+    builder, kw, steps = CASES[name]
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = getattr(cases, builder)(**kw)
+    return case, steps
+
+
+def run_oracle(case, steps, dtype, params32):
+    from oracle_bridge import oracle_solver
+    sol = oracle_solver(case, dtype, params32=params32)
+    sol.run(steps)
+    return sol.lat
+
+
+def make(name):
+    case, steps = build(name)
+    lbf = case["lbf"]
+    out = {
+        "meta": np.array(json.dumps({"case": name, "builder": CASES[name][0], "kwargs": CASES[name][1],
+                                     "steps": steps, "nc": int(lbf.nc),
+                                     "power_law": bool(case.get("power_law", False))})),
+        "in_mask": lbf.mask.arr.astype(np.int8),
+        "in_rho": np.asarray(case["rho"], np.float32),
+        "in_vel": np.asarray(case["vel"], np.float32),
+    }
+    if case.get("psi") is not None:
+        out["in_psi"] = np.asarray(case["psi"], np.float32)
+    l32 = run_oracle(case, steps, np.float32, params32=False)
+    out["out32_f"] = l32.f.astype(np.float32)
+    out["out32_rho"] = l32.rho.astype(np.float32)
+    out["out32_vel"] = l32.vel.astype(np.float32)
+    l64 = run_oracle(case, steps, np.float64, params32=True)
+    out["out64_rho"] = l64.rho.astype(np.float64)
+    out["out64_vel"] = l64.vel.astype(np.float64)
+    if lbf.nc == 1 and case.get("sc_spec") is None:
+        out["out64_f"] = l64.f.astype(np.float64)   # for the FP64-storage batch path
+    return out
+
+
+def main():
+    for name in CASES:
+        arrays = make(name)
+        path = HERE / f"{name}.npz"
+        np.savez_compressed(path, **arrays)
+        print(f"wrote {path.name} ({path.stat().st_size // 1024} KB)")
+
+
+if __name__ == "__main__":
+    main()
