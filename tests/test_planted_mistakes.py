@@ -1,0 +1,99 @@
+"""Mistakes planted in the oracle, on the CPU, must each fail an independent
+pin: a closed form, special value or invariant of tests/test_oracle_kat.py
+or the C++ restatement (oracle/cpu_ref.cpp, written separately from the
+NumPy oracle).  Where a planted change
+moves the fields of a golden case beyond the FP32 parity bounds, the
+committed vectors and the C++ restatement must flag it too; the sign of the
+Guo cross term does not (9 (c.F)(c.u) with F = 2e-5 and u = 4e-4 changes f
+by about 1e-8 per step, under the FP32 bound of 4e-6), so only its moment
+pin catches it.
+
+Each mutation is a slip a restatement of the reference could plausibly make:
+a dropped term of the equilibrium (CollisionEngine.py:14-18), a sign of the
+Guo forcing term (CollisionEngine.py:20-25), the diagonal opposites of the
+bounce map (BoundaryCondition.py:242), an untransposed MRT inverse
+(CollisionEngine.py:283-287), the other common pseudopotential (EOS.py:40-47).
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import test_oracle_kat as kat
+from oracle import lbm_oracle as O
+from tolerances import check
+
+def _feq_without_quadratic(vel, rho, dt):
+    cf = O.C.astype(dt)
+    w = O.weights(dt)
+    eu = cf[None, :, 0] * vel[:, 0:1] + cf[None, :, 1] * vel[:, 1:2]
+    uv = vel[:, 0:1] ** 2 + vel[:, 1:2] ** 2
+    return w[None, :] * rho[:, None] * (1 + 3 * eu - 1.5 * uv)
+
+
+def _guo_with_flipped_cross_term(F, vel, dt):
+    cf = O.C.astype(dt)
+    w = O.weights(dt)
+    cF = cf[None, :, 0] * F[:, 0:1] + cf[None, :, 1] * F[:, 1:2]
+    cu = cf[None, :, 0] * vel[:, 0:1] + cf[None, :, 1] * vel[:, 1:2]
+    uF = vel[:, 0:1] * F[:, 0:1] + vel[:, 1:2] * F[:, 1:2]
+    return w[None, :] * (3 * cF - 9 * cF * cu - 3 * uF)
+
+
+def _psi_exponential_form(rho, rho0):
+    return rho0 * np.exp(-rho0 / rho)
+
+
+# name -> (attribute of the oracle module, planted value, KAT pins, golden case
+#          that exercises it, golden vectors resolve it, C++ restatement resolves it)
+MUTATIONS = {
+    "feq_quadratic_term_dropped": ("f_eq", _feq_without_quadratic, [kat.test_feq_moments],
+                                   "cylinder_bgk_48x24", True, True),
+    "guo_cross_term_sign": ("f_force", _guo_with_flipped_cross_term, [kat.test_guo_force_moments],
+                            "porous_powerlaw_trt_48x40", False, False),
+    "diagonal_opposites_swapped": ("OPP", np.array([0, 3, 4, 1, 2, 8, 7, 6, 5], dtype=np.int64),
+                                   [kat.test_constants], "cylinder_bgk_48x24", True, True),
+    "mrt_inverse_not_transposed": ("M_INV", O.M / O.M_DIAG[:, None],
+                                   [kat.test_mrt_transform_identities, kat.test_mrt_equals_bgk_at_unit_rates],
+                                   "cylinder_mrt_48x24", True, True),
+    "psi_exponential_form": ("sc_psi", _psi_exponential_form, [kat.test_sc_psi_and_periodic_halo],
+                             "shan_chen_c1_32x32", True, False),   # no Shan-Chen in the C++ restatement
+}
+
+
+def _fails(fn):
+    try:
+        fn()
+    except AssertionError:
+        return True
+    return False
+
+
+def _golden(name):
+    z = np.load(Path(__file__).resolve().parent / "golden" / f"{name}.npz")
+    return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS))
+def test_planted_mistake_is_caught(name, monkeypatch):
+    attr, value, pins, case_name, by_golden, by_cpp = MUTATIONS[name]
+    from golden.make_golden import build, run_oracle
+    case, steps = build(case_name)
+    # the unmutated oracle agrees with the golden vectors (control)
+    fx = _golden(case_name)
+    good = run_oracle(case, steps, np.float32, params32=False)
+    check("f", good.f, fx["out32_f"], log=False)
+
+    monkeypatch.setattr(O, attr, value)
+    caught = [p.__name__ for p in pins if _fails(p)]
+    print(f"  {name}: closed-form pins that fail: {caught}")
+    assert caught, f"{name}: no closed-form pin fails"
+
+    case, steps = build(case_name)
+    bad = run_oracle(case, steps, np.float32, params32=False)
+    assert _fails(lambda: check("f", bad.f, fx["out32_f"], log=False)) == by_golden, name
+    if case["lbf"].nc == 1 and case.get("sc_spec") is None:
+        from oracle import cpu_ref
+        cl = cpu_ref.from_case(case, nthreads=2)
+        cl.run(steps)
+        assert _fails(lambda: check("f", bad.f, cl.f, log=False)) == by_cpp, name
