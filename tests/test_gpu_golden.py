@@ -98,3 +98,42 @@ def test_fp64_batch_against_fixture(name):
     print(f"  max |f - f64| = {e_f:.3e}, max |u - u64| = {e_vel:.3e}")
     assert e_f <= 1e-12
     assert e_vel <= 1e-7
+
+
+def test_fp32_batch_of_all_fixtures():
+    """Every one-component fixture lattice in one batch (uneven sizes, BGK,
+    TRT and MRT, power law and not, two steps per launch): each lattice
+    against its own FP32 vectors."""
+    from golden.make_golden import build
+    from taichi_lbm.batch import BatchRunner, spec_from_case
+    names = [n for n in NAMES if not n.startswith("shan_chen")]
+    built = [build(n) for n in names]
+    steps = {s for _, s in built}
+    for n_steps in sorted(steps):
+        group = [(n, c) for n, (c, s) in zip(names, built) if s == n_steps]
+        print(f"golden fp32 batch of {len(group)} lattices, {n_steps} steps", flush=True)
+        runner = BatchRunner([spec_from_case(c) for _, c in group], storage="f32")
+        runner.run(n_steps)
+        for q, (name, _) in enumerate(group):
+            fx = _load(name)
+            out = runner.fields(q)
+            print(f" {name}")
+            check("f", out["f"], fx["out32_f"])
+            check("vel", out["vel"], fx["out32_vel"])
+
+
+@pytest.mark.parametrize("storage,bound", [("f16", 2e-4), ("bf16", 2e-3)])
+def test_shifted_storage_against_fixture(storage, bound):
+    """FP16 / BF16 shifted population storage (f - w stored, arithmetic in
+    FP32): velocity within the same bound against the FP64 vectors as the
+    batch tests allow after 200 steps."""
+    from golden.make_golden import build
+    from taichi_lbm.batch import BatchRunner, spec_from_case
+    for name in [n for n in NAMES if not n.startswith("shan_chen")]:
+        fx = _load(name)
+        case, steps = build(name)
+        runner = BatchRunner([spec_from_case(case)], storage=storage)
+        runner.run(steps)
+        du = np.abs(runner.fields(0)["vel"] - fx["out64_vel"]).max()
+        print(f"  {storage} {name}: max |u - u_fp64| = {du:.2e}", flush=True)
+        assert du < bound
