@@ -20,7 +20,11 @@ NEEM/Zou-He identities, the Shan-Chen stencil on uniform/linear psi and the
 reference's own tabulated Shan-Chen coexistence densities
 (cases/Bubble or two-phase separation.py:48-71).  Nothing from the reference
 could be executed here (Taichi is not installed), so no function is pinned by a
-reference run.
+reference run: parity unpinned for Taichi's own FP32 rounding trajectory
+(fast-math contraction, summation order) and for its ti.random() stream.
+tests/test_planted_mistakes.py plants plausible slips here and checks that a
+pin fails for each; tests/golden/*.npz freeze this module's outputs on nine
+small lattices (written by tests/golden/make_golden.py).
 """
 from __future__ import annotations
 
