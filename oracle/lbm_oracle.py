@@ -23,8 +23,8 @@ could be executed here (Taichi is not installed), so no function is pinned by a
 reference run: parity unpinned for Taichi's own FP32 rounding trajectory
 (fast-math contraction, summation order) and for its ti.random() stream.
 tests/test_planted_mistakes.py plants plausible slips here and checks that a
-pin fails for each; tests/golden/*.npz freeze this module's outputs on nine
-small lattices (written by tests/golden/make_golden.py).
+pin fails for each; tests/golden/*.npz freeze this module's outputs on eleven
+lattices (written by tests/golden/make_golden.py).
 """
 from __future__ import annotations
 

@@ -2,6 +2,9 @@
 the case builders, the NumPy oracle and the C++ restatement against the
 committed vectors, on the CPU.
 
+The two full-size 400 x 100 cylinder fixtures keep every second node (FP32)
+and every fourth (FP64) in each direction, and no populations.
+
 Tolerances: the builders' inputs bit for bit; the FP32 oracle within the FP32
 parity bounds of tests/tolerances.py (bit-identical on the machine that wrote
 the fixtures; summation order inside NumPy may differ elsewhere); the FP64
@@ -31,6 +34,13 @@ def _build(name):
     return build(name)
 
 
+def _s(fx, key):
+    """The nodes of a full field that fixture array `key` keeps."""
+    from golden.make_golden import sub
+    k = sub(fx, key)
+    return (slice(None, None, k), slice(None, None, k))
+
+
 def test_every_case_has_a_fixture():
     from golden.make_golden import CASES
     assert sorted(CASES) == NAMES
@@ -55,23 +65,26 @@ def test_oracle_reproduces_fixture(name):
     fx = _load(name)
     case, steps = _build(name)
     l32 = run_oracle(case, steps, np.float32, params32=False)
-    check("f", l32.f, fx["out32_f"])
-    check("rho", l32.rho, fx["out32_rho"])
-    check("vel", l32.vel, fx["out32_vel"])
+    if "out32_f" in fx:
+        check("f", l32.f, fx["out32_f"])
+    check("rho", l32.rho[_s(fx, "out32")], fx["out32_rho"])
+    check("vel", l32.vel[_s(fx, "out32")], fx["out32_vel"])
     l64 = run_oracle(case, steps, np.float64, params32=True)
-    assert np.abs(l64.rho - fx["out64_rho"]).max() <= 1e-12
-    assert np.abs(l64.vel - fx["out64_vel"]).max() <= 1e-12
+    assert np.abs(l64.rho[_s(fx, "out64")] - fx["out64_rho"]).max() <= 1e-12
+    assert np.abs(l64.vel[_s(fx, "out64")] - fx["out64_vel"]).max() <= 1e-12
     if "out64_f" in fx:
         assert np.abs(l64.f - fx["out64_f"]).max() <= 1e-12
 
 
 @pytest.mark.parametrize("name", NAMES)
 def test_fp32_fixture_is_the_fp64_problem(name):
+    from golden.make_golden import sub
     fx = _load(name)
-    err = np.abs(fx["out32_vel"].astype(np.float64) - fx["out64_vel"]).max()
+    r = sub(fx, "out64") // sub(fx, "out32")
+    err = np.abs(fx["out32_vel"][::r, ::r].astype(np.float64) - fx["out64_vel"]).max()
     print(f"  {name}: max |u32 - u64| = {err:.3e}")
     assert err <= 2e-6
-    assert np.all(np.isfinite(fx["out32_f"]))
+    assert np.all(np.isfinite(fx["out32_vel"])) and np.all(np.isfinite(fx["out32_rho"]))
     assert np.abs(fx["out32_vel"]).max() > 1e-5       # the flow moved
 
 
@@ -83,6 +96,7 @@ def test_cpu_restatement_against_fixture(name):
     case, steps = _build(name)
     cl = cpu_ref.from_case(case, nthreads=2)
     cl.run(steps)
-    check("f", cl.f, fx["out32_f"])
-    check("rho", cl.rho, fx["out32_rho"])
-    check("vel", cl.vel, fx["out32_vel"])
+    if "out32_f" in fx:
+        check("f", cl.f, fx["out32_f"])
+    check("rho", cl.rho[_s(fx, "out32")], fx["out32_rho"])
+    check("vel", cl.vel[_s(fx, "out32")], fx["out32_vel"])

@@ -10,7 +10,8 @@
   populations within 1e-12 absolute: both integrate the same discrete
   problem in double precision, so only rounding-order differences remain
   (below 1e-13 against the oracle); its velocity is written back in FP32 and
-  is checked within 1e-7.
+  is checked within 1e-7 (the full-size fixtures keep ρ and u only, read
+  back in FP64: within 1e-12).
 """
 import contextlib
 import io
@@ -33,6 +34,12 @@ def _load(name):
     return {k: z[k] for k in z.files}
 
 
+def _s(fx, key):
+    from golden.make_golden import sub
+    k = sub(fx, key)
+    return (slice(None, None, k), slice(None, None, k))
+
+
 def _device_case(name):
     from golden.make_golden import build
     from taichi_lbm import cases
@@ -50,11 +57,12 @@ def test_fused_against_fixture(name):
     case, steps = _device_case(name)
     cases.stepper(case).run(steps)
     lbf = case["lbf"]
-    check("f", lbf.f.to_numpy(), fx["out32_f"])
-    check("rho", lbf.rho.to_numpy(), fx["out32_rho"])
+    if "out32_f" in fx:
+        check("f", lbf.f.to_numpy(), fx["out32_f"])
+    check("rho", lbf.rho.to_numpy()[_s(fx, "out32")], fx["out32_rho"])
     vel = lbf.vel.to_numpy()
-    check("vel", vel, fx["out32_vel"])
-    err64 = np.abs(vel.astype(np.float64) - fx["out64_vel"]).max()
+    check("vel", vel[_s(fx, "out32")], fx["out32_vel"])
+    err64 = np.abs(vel[_s(fx, "out64")].astype(np.float64) - fx["out64_vel"]).max()
     print(f"  max |u - u_fp64| = {err64:.3e}")
     assert err64 < 1e-5
 
@@ -93,11 +101,16 @@ def test_fp64_batch_against_fixture(name):
     runner = BatchRunner([spec_from_case(case)], storage="f64")
     runner.run(steps)
     out = runner.fields(0)
-    e_f = np.abs(out["f"] - fx["out64_f"]).max()
-    e_vel = np.abs(out["vel"] - fx["out64_vel"]).max()
-    print(f"  max |f - f64| = {e_f:.3e}, max |u - u64| = {e_vel:.3e}")
-    assert e_f <= 1e-12
-    assert e_vel <= 1e-7
+    e_vel = np.abs(out["vel"][_s(fx, "out64")] - fx["out64_vel"]).max()
+    if "out64_f" in fx:
+        e_f = np.abs(out["f"] - fx["out64_f"]).max()
+        print(f"  max |f - f64| = {e_f:.3e}, max |u - u64| = {e_vel:.3e}")
+        assert e_f <= 1e-12
+        assert e_vel <= 1e-7
+    else:   # full-size fixtures keep rho and u only (FP64 in the FP64-storage batch)
+        e_rho = np.abs(out["rho"][_s(fx, "out64")] - fx["out64_rho"]).max()
+        print(f"  max |rho - rho64| = {e_rho:.3e}, max |u - u64| = {e_vel:.3e}")
+        assert e_rho <= 1e-12 and e_vel <= 1e-12
 
 
 def test_fp32_batch_of_all_fixtures():
@@ -118,8 +131,9 @@ def test_fp32_batch_of_all_fixtures():
             fx = _load(name)
             out = runner.fields(q)
             print(f" {name}")
-            check("f", out["f"], fx["out32_f"])
-            check("vel", out["vel"], fx["out32_vel"])
+            if "out32_f" in fx:
+                check("f", out["f"], fx["out32_f"])
+            check("vel", out["vel"][_s(fx, "out32")], fx["out32_vel"])
 
 
 @pytest.mark.parametrize("storage,bound", [("f16", 2e-4), ("bf16", 2e-3)])
@@ -134,6 +148,6 @@ def test_shifted_storage_against_fixture(storage, bound):
         case, steps = build(name)
         runner = BatchRunner([spec_from_case(case)], storage=storage)
         runner.run(steps)
-        du = np.abs(runner.fields(0)["vel"] - fx["out64_vel"]).max()
+        du = np.abs(runner.fields(0)["vel"][_s(fx, "out64")] - fx["out64_vel"]).max()
         print(f"  {storage} {name}: max |u - u_fp64| = {du:.2e}", flush=True)
         assert du < bound
