@@ -7,11 +7,12 @@
 * The reference's own unfused call sequence through the granular kernels,
   against the same vectors.
 * The FP64-storage batch path (one-component lattices) against the FP64
-  populations within 1e-12 absolute: both integrate the same discrete
-  problem in double precision, so only rounding-order differences remain
-  (below 1e-13 against the oracle); its velocity is written back in FP32 and
-  is checked within 1e-7 (the full-size fixtures keep ρ and u only, read
-  back in FP64: within 1e-12).
+  populations and velocities within 1e-12 absolute (the full-size fixtures
+  keep ρ and u only): both integrate the same discrete problem in double
+  precision, so only rounding-order differences remain (below 2e-15
+  measured).
+* FP16 / BF16 shifted storage against the FP64 velocities, with the bounds
+  the batch tests allow after 200 steps.
 """
 import contextlib
 import io
@@ -105,8 +106,7 @@ def test_fp64_batch_against_fixture(name):
     if "out64_f" in fx:
         e_f = np.abs(out["f"] - fx["out64_f"]).max()
         print(f"  max |f - f64| = {e_f:.3e}, max |u - u64| = {e_vel:.3e}")
-        assert e_f <= 1e-12
-        assert e_vel <= 1e-7
+        assert e_f <= 1e-12 and e_vel <= 1e-12
     else:   # full-size fixtures keep rho and u only (FP64 in the FP64-storage batch)
         e_rho = np.abs(out["rho"][_s(fx, "out64")] - fx["out64_rho"]).max()
         print(f"  max |rho - rho64| = {e_rho:.3e}, max |u - u64| = {e_vel:.3e}")
