@@ -1,7 +1,9 @@
-"""Parity on the full 8192 x 8192 porous lattice the headline figure is measured
-on (MRT, power law n = 0.7, tau clamped to [0.51, 2.0], gravity 1e-6, seeded
-disks of radius 8-32 to solid fraction 0.3; `cases/Power Law Fluid.py`'s
-step order):
+"""Parity at full configuration size.
+
+The 8192 x 8192 porous lattice the headline figure is measured on (MRT, and
+TRT with magic 1/4; power law n = 0.7, tau clamped to [0.51, 2.0], gravity
+1e-6, seeded disks of radius 8-32 to solid fraction 0.3; `cases/Power Law
+Fluid.py`'s step order):
 
 * 10 fused steps on the GPU (two per launch) against 10 steps of the C++
   restatement of the reference's unfused kernels (oracle/cpu_ref.cpp) on the
@@ -11,6 +13,12 @@ step order):
 * FP64 mass (the sum of rho over the fluid group) is conserved to FP32
   rounding (bounce-back walls and periodic edges move mass, never create it):
   relative drift below 1e-6 over the 10 steps.
+
+The 2048 x 2048 two-component Shan-Chen separation (MRT, G_cross = 0.9,
+densities 1 +- 1 % seeded per component; `cases/Bubble or two-phase
+separation.py`'s step order): 2 steps of the LDS-tiled Shan-Chen kernel
+against 2 steps of the FP32 NumPy oracle (about 12 s a step on the CPU),
+every field within the FP32 parity bounds.
 """
 import contextlib
 import io
@@ -26,10 +34,10 @@ pytestmark = pytest.mark.gpu
 N, STEPS = 8192, 10
 
 
-def _case():
+def _case(collision):
     from taichi_lbm import cases
     with contextlib.redirect_stdout(io.StringIO()):
-        return cases.porous(NX=N, NY=N, seed=1, collision="mrt")
+        return cases.porous(NX=N, NY=N, seed=1, collision=collision)
 
 
 def _gpu_run(case, tile_cols):
@@ -42,10 +50,11 @@ def _gpu_run(case, tile_cols):
     return st
 
 
-def test_headline_lattice_matches_cpu_restatement():
+@pytest.mark.parametrize("collision", ["mrt", "trt"])
+def test_headline_lattice_matches_cpu_restatement(collision):
     from oracle import cpu_ref
-    print(f"case porous {N}x{N} MRT power law, {STEPS} steps", flush=True)
-    case = _case()
+    print(f"case porous {N}x{N} {collision} power law, {STEPS} steps", flush=True)
+    case = _case(collision)
     threads = min(16, os.cpu_count() or 1)
     cl = cpu_ref.from_case(case, nthreads=threads)
     cl.run(STEPS)
@@ -64,6 +73,26 @@ def test_headline_lattice_matches_cpu_restatement():
     assert abs(mass - m_init) / m_init < 1e-6
     del st, cl
     # the same lattice on 32-column tiles
-    case32 = _case()
+    case32 = _case(collision)
     _gpu_run(case32, 32)
     assert np.array_equal(case32["lbf"].f.to_numpy(), f_w64)
+
+
+def test_shan_chen_2048_two_components_matches_oracle():
+    from oracle_bridge import oracle_solver
+    from taichi_lbm import cases
+    n = 2
+    print(f"case shan-chen 2048x2048x2 MRT, {n} steps", flush=True)
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.shan_chen(NX=2048, NY=2048, nc=2, gcross=0.9, seed=3)
+    sol = oracle_solver(case, np.float32)
+    sol.run(n)
+    with contextlib.redirect_stdout(io.StringIO()):
+        cases.initialise(case)
+    st = cases.stepper(case)
+    st.run(n)
+    lbf = case["lbf"]
+    for name in ("f", "rho", "vel", "total_rho", "SCforce", "body_force"):
+        check(name, getattr(lbf, name).to_numpy(), getattr(sol.lat, name))
+    check("psi", lbf.sc_field.psi_field.to_numpy(), sol.lat.psi)
+    assert st.reduce()["nonfinite"] == 0
