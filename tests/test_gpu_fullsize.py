@@ -19,6 +19,9 @@ densities 1 +- 1 % seeded per component; `cases/Bubble or two-phase
 separation.py`'s step order): 2 steps of the LDS-tiled Shan-Chen kernel
 against 2 steps of the FP32 NumPy oracle (about 12 s a step on the CPU),
 every field within the FP32 parity bounds.
+
+The 512 batched cylinder channels: a sample of lattices against the same
+lattices run alone (bit for bit) and against the C++ restatement.
 """
 import contextlib
 import io
@@ -96,3 +99,49 @@ def test_shan_chen_2048_two_components_matches_oracle():
         check(name, getattr(lbf, name).to_numpy(), getattr(sol.lat, name))
     check("psi", lbf.sc_field.psi_field.to_numpy(), sol.lat.psi)
     assert st.reduce()["nonfinite"] == 0
+
+
+def test_512_batched_channels_sample():
+    """The 512-lattice batch of independent cylinder channels (NX ~ U{128..1024},
+    NY ~ U{64..256}, r ~ U[4, NY/8], u ~ U[0.01, 0.08], tau ~ U[0.52, 0.8], BGK
+    or TRT by a seeded coin; VelocityBB inlet, PressureNEEM outlet), 20 steps
+    at two per launch: a seeded sample of 8 lattices (the largest among them)
+    equals each lattice run alone through the Stepper bit for bit, and 3 of
+    them agree with the C++ restatement within the FP32 parity bounds."""
+    from oracle import cpu_ref
+    from taichi_lbm import cases
+    from taichi_lbm.batch import BatchRunner, spec_from_case
+    from taichi_lbm.stepper import Stepper
+    n, count = 20, 512
+    print(f"case {count} batched channels, {n} steps", flush=True)
+    rng = np.random.default_rng(1)
+    kws = []
+    for q in range(count):
+        NX, NY = int(rng.integers(128, 1025)), int(rng.integers(64, 257))
+        r, u = float(rng.uniform(4, NY / 8)), float(rng.uniform(0.01, 0.08))
+        tau, coll = float(rng.uniform(0.52, 0.8)), ("bgk" if rng.random() < 0.5 else "trt")
+        kws.append(dict(NX=NX, NY=NY, cx=max(NX / 5, r + 4), cy=NY / 2, r=r, tau=tau, u_in=u,
+                        collision=coll, name=f"b{q}", inlet="bb"))
+
+    def build(q):
+        with contextlib.redirect_stdout(io.StringIO()):
+            return cases.cylinder(**kws[q])
+
+    runner = BatchRunner([spec_from_case(build(q)) for q in range(count)], storage="f32")
+    runner.run(n)
+    largest = max(range(count), key=lambda q: kws[q]["NX"] * kws[q]["NY"])
+    sample = sorted({largest, *np.random.default_rng(2).choice(count, 7, replace=False).tolist()})
+    for i, q in enumerate(sample):
+        print(f"  lattice {q}: {kws[q]['NX']} x {kws[q]['NY']} {kws[q]['collision']}", flush=True)
+        got = runner.fields(q)
+        c = build(q)
+        if i < 3:
+            cl = cpu_ref.from_case(c, nthreads=min(16, os.cpu_count() or 1))
+            cl.run(n)
+            for name in ("f", "rho", "vel"):
+                check(name, got[name], getattr(cl, name))
+        with contextlib.redirect_stdout(io.StringIO()):
+            cases.initialise(c)
+        Stepper(c["lbf"], c["macro"], c["coll"], c["bce"]).run(n)
+        for name in ("f", "f2", "rho", "vel"):
+            assert np.array_equal(got[name].astype(np.float32), getattr(c["lbf"], name).to_numpy()), (q, name)
