@@ -22,6 +22,9 @@ every field within the FP32 parity bounds.
 
 The 512 batched cylinder channels: a sample of lattices against the same
 lattices run alone (bit for bit) and against the C++ restatement.
+
+Served contraction solves at the serving workload's request sizes: each
+result equal to the same solve run alone, bit for bit.
 """
 import contextlib
 import io
@@ -145,3 +148,32 @@ def test_512_batched_channels_sample():
         Stepper(c["lbf"], c["macro"], c["coll"], c["bce"]).run(n)
         for name in ("f", "f2", "rho", "vel"):
             assert np.array_equal(got[name].astype(np.float32), getattr(c["lbf"], name).to_numpy()), (q, name)
+
+
+def test_served_contraction_solves_at_full_request_sizes():
+    """Contraction-flow power-law requests at the request sizes of the serving
+    workload (widths 256-2048, heights 64-512, 4:1 contraction, n in
+    [0.5, 1.5]), arriving while others run: each result equals the same solve
+    run alone bit for bit."""
+    from taichi_lbm.batch import spec_from_case
+    from taichi_lbm.serving import Request, SolveServer, solve_alone
+    from taichi_lbm import cases
+    sizes = [(2048, 512, 0.7), (256, 64, 1.5), (1024, 256, 0.5), (640, 160, 1.0), (1536, 384, 1.2)]
+    print(f"case {len(sizes)} served contraction solves, 2048 x 512 the largest", flush=True)
+    specs = []
+    with contextlib.redirect_stdout(io.StringIO()):
+        for q, (W, Hh, n) in enumerate(sizes):
+            specs.append(spec_from_case(cases.contraction(W=W, H=Hh, power_n=n, name=f"r{q}")))
+    arrivals = [0.0, 0.0, 0.01, 0.02, 0.03]
+    max_steps = 600
+    reqs = [Request(q, arrivals[q], s, max_steps=max_steps, tol=1e-12) for q, s in enumerate(specs)]
+    srv = SolveServer(storage="f32", hbm_budget_bytes=1 << 32, max_slots=8, tick_steps=100)
+    stats = srv.serve(reqs)
+    srv.close()
+    assert len(stats.completed) == len(sizes)
+    for r in stats.completed:
+        u, steps, res = solve_alone(specs[r.rid], max_steps, 1e-12, 100, "f32")
+        print(f"  request {r.rid} {sizes[r.rid][:2]}: steps {r.steps} vs {steps}", flush=True)
+        assert steps == r.steps
+        assert np.all(np.isfinite(u))
+        assert np.array_equal(u, r.result)
