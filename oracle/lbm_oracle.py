@@ -16,9 +16,15 @@ Pinning (see tests/test_oracle_*.py): every function is checked against closed
 forms or invariants -- equilibrium moments, the exact MRT transform identities,
 MRT/TRT <-> BGK equivalences, the rest-state fixed point, mass and momentum
 budgets, Poiseuille (Newtonian and power-law) profiles, Taylor-Green decay,
-NEEM/Zou-He identities, the Shan-Chen stencil on uniform/linear psi and the
-reference's own tabulated Shan-Chen coexistence densities
-(cases/Bubble or two-phase separation.py:48-71).  Nothing from the reference
+NEEM/Zou-He identities, the Shan-Chen stencil on uniform/linear psi (one and
+two components: F_c1 = -psi_c1 G[c1,c2] grad psi_c2 / 3, and the cross forces
+summing to zero over a periodic lattice), the reference's own tabulated
+Shan-Chen coexistence densities (cases/Bubble or two-phase separation.py:48-71;
+pressure1 is equal at the tabulated gas and liquid densities), pressure0 as
+the rest equilibrium's momentum flux, the pressure2 uniform-state form and its
+single-component limit, the nc = 2 force split (shares sum to g; the total
+momentum gains N g per step) and barycentric velocity, and the table stream
+against a plain shift.  Nothing from the reference
 could be executed here (Taichi is not installed), so no function is pinned by a
 reference run: parity unpinned for Taichi's own FP32 rounding trajectory
 (fast-math contraction, summation order) and for its ti.random() stream.

@@ -507,3 +507,240 @@ def test_open_neumann_copies_unknowns_and_recomputes_moments():
     assert np.allclose(lat.rho[4, :, 0], lat.f[4, :, 0, :].sum(-1))
     m = lat.f[4, :, 0, :] @ O.C.astype(f64)
     assert np.allclose(lat.vel[4], m / lat.rho[4, :, 0][:, None])
+
+
+# ---------------------------------------------------------------------------
+# Two-component Shan-Chen force, pressure observables, the nc = 2 force split
+# and barycentric velocity, the table stream (InsideBoundary)
+# ---------------------------------------------------------------------------
+G2 = np.array([[0.37, 0.9], [1.3, -0.6]])   # asymmetric, non-zero diagonal: a G index slip changes F
+
+
+def _linear_psi(NX, NY, coef):
+    """psi_c(x, y) = p_c + a_c x + b_c y on the lattice and its halo (offset -1)."""
+    I = np.arange(-1, NX + 1)[:, None]
+    J = np.arange(-1, NY + 1)[None, :]
+    return np.stack([p + a * I + b * J for p, a, b in coef], axis=-1).astype(f64)
+
+
+def test_sc_c2_linear_psi_closed_form():
+    # sum_k w_k c_k psi(x + c_k) = grad psi / 3 for a linear psi (sum w c = 0,
+    # sum w c c = I/3), so ExternalForce.py:96-117 gives, fluid everywhere,
+    # F_c1 = -psi_c1(x) sum_{c2 != c1} G[c1, c2] grad psi_c2 / 3 for each ordered pair
+    NX, NY = 8, 7
+    lat, grp = _lat_all_fluid(NX, NY, nc=2)
+    coef = [(0.4, 0.011, -0.007), (0.6, -0.005, 0.013)]
+    lat.psi = _linear_psi(NX, NY, coef)
+    sc = O.ShanChen("c2", grp, G2, (1.0, 1.0))
+    sc.cau_scforce(lat)
+    for c1 in range(2):
+        c2 = 1 - c1
+        grad = np.array(coef[c2][1:])
+        exact = -lat.psi[1:-1, 1:-1, c1][..., None] * G2[c1, c2] * grad / 3
+        assert np.allclose(lat.SCforce[:, :, c1, :], exact, rtol=1e-12, atol=1e-18), c1
+
+
+def test_sc_c2_total_force_vanishes_periodic():
+    # with a symmetric G the cross forces are pairwise equal and opposite:
+    # sum_x sum_c F_c = 0 on a periodic lattice, though each component's sum is not
+    NX, NY = 9, 6
+    lat, grp = _lat_all_fluid(NX, NY, nc=2)
+    rng = np.random.default_rng(21)
+    lat.psi = np.zeros((NX + 2, NY + 2, 2))
+    lat.psi[1:-1, 1:-1, :] = rng.uniform(0.2, 0.9, (NX, NY, 2))
+    O.periodic_psi(lat)
+    G = np.array([[0.3, 0.9], [0.9, -0.4]])
+    O.ShanChen("c2", grp, G, (1.0, 1.0)).cau_scforce(lat)
+    per = lat.SCforce.sum(axis=(0, 1))            # (nc, 2)
+    assert np.abs(per[0]).max() > 1e-3
+    assert np.abs(per.sum(axis=0)).max() < 1e-15
+
+
+def _uniform(NX, NY, rhos, sc=None, G=None):
+    lat, grp = _lat_all_fluid(NX, NY, nc=len(rhos))
+    for c, r in enumerate(rhos):
+        lat.rho[..., c] = r
+    lat.psi = np.zeros((NX + 2, NY + 2, len(rhos)))
+    if sc is not None:
+        sc.update_psi(lat)
+    return lat, grp
+
+
+def test_pressure0_is_the_rest_momentum_flux():
+    # p0 = rho/3 (lbm.py:166-173) is the xx and yy momentum flux of the rest equilibrium
+    lat, grp = _uniform(3, 2, [1.37])
+    O.pressure(lat, grp, None)
+    fe = O.f_eq(np.zeros((1, 2)), np.array([1.37]), f64)[0]
+    cf = O.C.astype(f64)
+    assert np.allclose(lat.pressure[..., 0], fe @ (cf[:, 0] ** 2), rtol=1e-15)
+    assert np.allclose(lat.pressure[..., 0], fe @ (cf[:, 1] ** 2), rtol=1e-15)
+    assert np.array_equal(lat.total_pressure, lat.pressure[..., 0])
+
+
+def test_pressure1_equal_at_reference_coexistence_densities():
+    # pressure1 (lbm.py:175-182) evaluated on uniform states at the reference's own
+    # tabulated coexistence densities (cases/Bubble or two-phase separation.py:48-68,
+    # the eps = 1 construction of p = rho/3 + G psi^2/6): gas and liquid pressures agree
+    for G, (rg, rl) in REF_TABLE.items():
+        p = []
+        for r in (rg, rl):
+            lat, grp = _lat_all_fluid(2, 2)
+            lat.rho[..., 0] = r
+            lat.psi = np.zeros((4, 4, 1))
+            sc = O.ShanChen("c1", grp, G, (1.0,))
+            sc.update_psi(lat)
+            O.pressure(lat, grp, sc)
+            psi = 1 - math.exp(-r)
+            assert np.allclose(lat.pressure[..., 0], (r + 0.5 * G * psi ** 2) / 3, rtol=1e-14)
+            assert np.array_equal(lat.total_pressure, lat.pressure[..., 0])
+            p.append(lat.total_pressure[0, 0])
+        assert abs(p[0] / p[1] - 1) < 1e-9, (G, p)
+
+
+def test_pressure1_reads_the_psi_of_its_own_node():
+    # update_psi writes psi at offset (-1, -1); pressure must pair node (i, j)'s
+    # density with that node's psi: on a non-uniform state p(i, j) depends on rho(i, j) alone
+    NX, NY, G = 5, 4, -5.0
+    lat, grp = _lat_all_fluid(NX, NY)
+    rng = np.random.default_rng(3)
+    lat.rho[..., 0] = rng.uniform(0.2, 2.0, (NX, NY))
+    lat.psi = np.zeros((NX + 2, NY + 2, 1))
+    sc = O.ShanChen("c1", grp, G, (1.0,))
+    sc.update_psi(lat)
+    O.pressure(lat, grp, sc)
+    r = lat.rho[..., 0]
+    assert np.allclose(lat.pressure[..., 0], (r + 0.5 * G * (1 - np.exp(-r)) ** 2) / 3, rtol=1e-14)
+
+
+def test_pressure2_uniform_states():
+    # lbm.py:185-202: per component (rho_c + G_cc psi_c^2 / 2)/3; total = sum of those
+    # + G_12 psi_1 psi_2 / 6 (each unordered pair once)
+    rhos = [0.83, 1.21]
+    G = np.array([[0.2, 0.9], [0.9, -0.3]])
+    sc = O.ShanChen("c2", None, G, (1.0, 1.0))
+    lat, grp = _uniform(3, 2, rhos)
+    sc.group = grp
+    sc.update_psi(lat)
+    O.pressure(lat, grp, sc)
+    psi = [1 - math.exp(-r) for r in rhos]
+    for c in range(2):
+        assert np.allclose(lat.pressure[..., c], (rhos[c] + 0.5 * G[c, c] * psi[c] ** 2) / 3, rtol=1e-14)
+    tot = sum((rhos[c] + 0.5 * G[c, c] * psi[c] ** 2) / 3 for c in range(2)) + G[0, 1] * psi[0] * psi[1] / 6
+    assert np.allclose(lat.total_pressure, tot, rtol=1e-14)
+    # no cross coupling: the sum of the two single-component pressure1 observables
+    G0 = np.diag(np.diag(G))
+    lat0, _ = _uniform(3, 2, rhos)
+    sc0 = O.ShanChen("c2", grp, G0, (1.0, 1.0))
+    sc0.update_psi(lat0)
+    O.pressure(lat0, grp, sc0)
+    single = 0.0
+    for c in range(2):
+        l1, g1 = _uniform(3, 2, [rhos[c]])
+        s1 = O.ShanChen("c1", g1, G[c, c], (1.0,))
+        s1.update_psi(l1)
+        O.pressure(l1, g1, s1)
+        single = single + l1.total_pressure
+    assert np.allclose(lat0.total_pressure, single, rtol=1e-14)
+    # relabelling the components (G transposed by the same permutation) changes nothing
+    latS, _ = _uniform(3, 2, rhos[::-1])
+    scS = O.ShanChen("c2", grp, G[::-1, ::-1], (1.0, 1.0))
+    scS.update_psi(latS)
+    O.pressure(latS, grp, scS)
+    assert np.allclose(latS.total_pressure, lat.total_pressure, rtol=1e-14)
+
+
+def test_two_component_force_split():
+    # lbm.py:215-220: F_c = SCforce_c + rho_c / rho_tot g; the components' shares sum to g
+    NX, NY = 4, 3
+    lat, grp = _lat_all_fluid(NX, NY, nc=2, g=(2e-5, -3e-5))
+    rng = np.random.default_rng(8)
+    lat.rho[...] = rng.uniform(0.1, 1.5, (NX, NY, 2))
+    lat.total_rho[...] = lat.rho.sum(-1)
+    lat.SCforce[...] = 0.0
+    O.force_density(lat, grp)
+    assert np.allclose(lat.body_force.sum(axis=2), lat.g, rtol=1e-13, atol=0)
+    share = lat.body_force[..., 0] / lat.g[0]
+    assert np.allclose(share[..., 0], lat.rho[..., 0] / lat.total_rho, rtol=1e-13)
+
+
+def test_two_component_barycentric_velocity():
+    # lbm.py:223-237: u = (sum_c j_c + F_tot / 2) / rho_tot with j_c the momentum of
+    # component c: components at equilibrium with velocities u_c give the
+    # density-weighted mean, shifted by half the total force over rho_tot
+    NX, NY = 3, 2
+    lat, grp = _lat_all_fluid(NX, NY, nc=2)
+    rho = np.array([0.7, 1.4])
+    u = np.array([[0.03, -0.01], [-0.02, 0.04]])
+    for c in range(2):
+        lat.f[:, :, c, :] = O.f_eq(u[c][None], rho[c:c + 1], f64)[0]
+    F = np.array([[1e-4, 2e-4], [-3e-4, 5e-5]])
+    lat.body_force[...] = F[None, None]
+    O.density(lat, grp)
+    O.velocity(lat, grp)
+    exact = (rho[:, None] * u).sum(0) / rho.sum() + F.sum(0) / (2 * rho.sum())
+    assert np.allclose(lat.vel, exact, rtol=1e-13, atol=1e-18)
+
+
+def _sc2_solver(g, G, NX=8, NY=6, seed=12):
+    lat, grp = _lat_all_fluid(NX, NY, nc=2, g=g)
+    rng = np.random.default_rng(seed)
+    lat.rho[...] = 1.0 + 0.05 * rng.standard_normal((NX, NY, 2))
+    lat.psi = np.zeros((NX + 2, NY + 2, 2))
+    O.init_lbm(lat, grp)
+    col = O.Collision("mrt", grp, O.FluidModel("newtonian", nu=1 / 6), [1 / 6, 1 / 6], nu_bulk=[1 / 6, 1 / 6])
+    col.init_tau(lat)
+    sc = O.ShanChen("c2", grp, G, (1.0, 1.0))
+    return O.Solver(lat, grp, col, [O.BC("fluid", grp), O.BC("periodic", grp)], sc=sc)
+
+
+def test_two_component_momentum_budget_and_mass():
+    # periodic, symmetric G: the cross forces cancel in the sum, so the total
+    # momentum sum_x sum_c sum_k f c_k gains exactly N g per step; each
+    # component's mass is conserved
+    g = (4e-5, -1e-5)
+    s = _sc2_solver(g, np.array([[0.0, 0.9], [0.9, 0.0]]))
+    cf = O.C.astype(f64)
+    p0 = np.einsum("xyck,ka->a", s.lat.f, cf)
+    m0 = s.lat.f.sum(axis=(0, 1, 3))
+    n = 12
+    s.run(n)
+    assert np.abs(s.lat.SCforce).max() > 1e-4        # the coupling is active
+    p1 = np.einsum("xyck,ka->a", s.lat.f, cf)
+    N = s.lat.NX * s.lat.NY
+    assert np.allclose(p1 - p0, n * N * np.array(g), rtol=1e-9, atol=1e-14)
+    assert np.allclose(s.lat.f.sum(axis=(0, 1, 3)), m0, rtol=1e-13)
+
+
+def test_inside_boundary_table_stream():
+    # InsideBoundary (BoundaryCondition.py:435-449): f(x, k) = f2(neighbor[x][k], k).
+    # Where the table holds an in-domain fluid source x - c_k this is the plain
+    # shift of f2 by c_k (np.roll, independent of the table); where it holds -1
+    # the reference reads out of bounds, which the oracle marks NaN; a
+    # BounceBackWall over those nodes afterwards (the reference cases' order)
+    # leaves no NaN and reflects f2(x, opp k) there
+    NX, NY = 7, 5
+    lat = O.Lattice(NX, NY, 2, dtype=f64)
+    O.mask_rectangle_identify(lat, 2, 4, 0, 3)       # solid (3, 1), (3, 2)
+    O.neighbor_classify(lat)
+    rng = np.random.default_rng(17)
+    lat.f2[...] = rng.standard_normal(lat.f2.shape)
+    fluid = np.argwhere(lat.mask == 1)
+    O.BC("inside", fluid).apply(lat)
+    I, J = fluid[:, 0], fluid[:, 1]
+    solid = lat.mask == -1
+    for k in range(9):
+        si, sj = I - O.C[k, 0], J - O.C[k, 1]
+        inside = (si >= 0) & (si < NX) & (sj >= 0) & (sj < NY)
+        ok = inside.copy()
+        ok[inside] &= ~solid[si[inside], sj[inside]]
+        rolled = np.roll(lat.f2[..., k], shift=tuple(O.C[k]), axis=(0, 1))[I, J]
+        got = lat.f[I, J, :, k]
+        assert np.array_equal(got[ok], rolled[ok]), k
+        assert np.isnan(got[~ok]).all(), k
+    wall = np.argwhere((lat.mask == 1) & np.any(lat.neighbor[:, :, :, 0] == -1, axis=2))
+    O.BC("bounceback", wall).apply(lat)
+    assert np.isfinite(lat.f[I, J]).all()
+    w0 = wall[0]
+    for k in range(9):
+        if lat.neighbor[w0[0], w0[1], k, 0] == -1:
+            assert np.array_equal(lat.f[w0[0], w0[1], :, k], lat.f2[w0[0], w0[1], :, O.OPP[k]])

@@ -97,3 +97,101 @@ def test_planted_mistake_is_caught(name, monkeypatch):
         cl = cpu_ref.from_case(case, nthreads=2)
         cl.run(steps)
         assert _fails(lambda: check("f", bad.f, cl.f, log=False)) == by_cpp, name
+
+
+# ---------------------------------------------------------------------------
+# Slips in the two-component Shan-Chen force (ExternalForce.py:96-117), the
+# pressure observables (lbm.py:166-212), the nc = 2 force split and
+# barycentric velocity (lbm.py:215-237) and the table stream
+# (BoundaryCondition.py:435-449).  Each is planted as a wrapper around the
+# oracle's own function and must fail one of the pins listed beside it.
+# ---------------------------------------------------------------------------
+_orig_pressure = O.pressure
+_orig_force = O.ShanChen.cau_scforce
+_orig_force_density = O.force_density
+_orig_bc_apply = O.BC.apply
+
+
+def _pressure_cross_term_unscaled(lat, group, sc=None):
+    _orig_pressure(lat, group, sc)
+    if sc is not None and lat.nc == 2:       # p_ij in place of p_ij / 6
+        I, J = O.group_ij(group)
+        G = np.asarray(sc.g_coh, lat.dt)
+        lat.total_pressure[I, J] += 5 * G[0, 1] * lat.psi[I + 1, J + 1, 0] * lat.psi[I + 1, J + 1, 1] / 6
+
+
+def _pressure1_without_half(lat, group, sc=None):
+    _orig_pressure(lat, group, sc)
+    if sc is not None and lat.nc == 1:       # (rho + G psi^2)/3 in place of (rho + G psi^2 / 2)/3
+        I, J = O.group_ij(group)
+        lat.pressure[I, J, 0] += 0.5 * float(sc.g_coh) * lat.psi[I + 1, J + 1, 0] ** 2 / 3
+        lat.total_pressure[I, J] = lat.pressure[I, J, 0]
+
+
+def _force_self_coupling(self, lat):
+    if self.kind != "c2":
+        return _orig_force(self, lat)
+    G = np.asarray(self.g_coh)
+    keep = self.g_coh
+    self.g_coh = np.array([[G[0, 0], G[0, 0]], [G[1, 1], G[1, 1]]])   # G[c1, c1] where G[c1, c2] belongs
+    try:
+        _orig_force(self, lat)
+    finally:
+        self.g_coh = keep
+
+
+def _force_sign_flipped(self, lat):
+    _orig_force(self, lat)
+    if self.kind == "c2":
+        lat.SCforce *= -1                    # +psi_c1(x) in place of -psi_c1(x)
+
+
+def _force_split_by_total_density(lat, group):
+    I, J = O.group_ij(group)
+    for c in range(lat.nc):                  # rho_tot / rho_tot in place of rho_c / rho_tot
+        lat.body_force[I, J, c, :] = lat.SCforce[I, J, c, :] + lat.g[None, :]
+
+
+def _velocity_of_first_component(lat, group):
+    lat.vel.fill(0)
+    I, J = O.group_ij(group)
+    cf = O.C.astype(lat.dt)
+    j = lat.f[I, J, 0, :] @ cf + 0.5 * lat.body_force[I, J, 0, :]   # component 0 over its own density
+    lat.vel[I, J] = j / lat.rho[I, J, 0][:, None]
+
+
+def _inside_wraps_periodically(self, lat):
+    if self.kind != "inside":
+        return _orig_bc_apply(self, lat)
+    keep = self.kind
+    self.kind = "periodic"                  # pulls across the edges and from solids in place of the table
+    try:
+        _orig_bc_apply(self, lat)
+    finally:
+        self.kind = keep
+
+
+# name -> (object, attribute, planted value, pins that must fail)
+MUTATIONS_SC = {
+    "pressure2_cross_term_unscaled": (O, "pressure", _pressure_cross_term_unscaled, [kat.test_pressure2_uniform_states]),
+    "pressure1_half_dropped": (O, "pressure", _pressure1_without_half,
+                               [kat.test_pressure1_equal_at_reference_coexistence_densities]),
+    "sc_c2_self_coupling": (O.ShanChen, "cau_scforce", _force_self_coupling, [kat.test_sc_c2_linear_psi_closed_form]),
+    "sc_c2_sign_flipped": (O.ShanChen, "cau_scforce", _force_sign_flipped, [kat.test_sc_c2_linear_psi_closed_form]),
+    "force_split_by_total_density": (O, "force_density", _force_split_by_total_density,
+                                     [kat.test_two_component_force_split, kat.test_two_component_momentum_budget_and_mass]),
+    "velocity_of_first_component": (O, "velocity", _velocity_of_first_component,
+                                    [kat.test_two_component_barycentric_velocity]),
+    "inside_wraps_periodically": (O.BC, "apply", _inside_wraps_periodically, [kat.test_inside_boundary_table_stream]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS_SC))
+def test_planted_multicomponent_mistake_is_caught(name, monkeypatch):
+    obj, attr, value, pins = MUTATIONS_SC[name]
+    for p in pins:                           # control: the unmutated oracle passes every pin
+        p()
+    monkeypatch.setattr(obj, attr, value)
+    failed = [p.__name__ for p in pins if _fails(p)]
+    print(f"  {name}: pins that fail: {failed}")
+    assert failed == [p.__name__ for p in pins], f"{name}: a pin does not catch it"
