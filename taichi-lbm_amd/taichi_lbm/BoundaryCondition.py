@@ -103,6 +103,44 @@ class BoundaryClassifier:
         return MaskAndGroup(self.NX, self.NY, np.argwhere(hit).astype(np.int32))
 
 
+_C = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
+
+
+def check_granular_bounds(kind, nodes, NX, NY, direction=None, table=None, name="boundary"):
+    """Raises LBMError unless every address a granular boundary kernel forms
+    from this group lies inside the NX x NY lattice: the group's nodes; for
+    the neighbour-table kinds (BounceBackWall, InsideBoundary) every table
+    entry of those nodes is (-1, -1) or a node of the lattice; for the node
+    rules that read x - c_d (VelocityNEEM, PressureNEEM, OpenNeumann) that
+    interior neighbour.  The reference reads out of range silently
+    (BoundaryCondition.py:154-166, :215-226, :321-334, :446-449); the kernels
+    trust these addresses, so they are checked here before any launch."""
+    nodes = np.asarray(nodes, np.int64).reshape(-1, 2)
+    if len(nodes) == 0:
+        return
+    I, J = nodes[:, 0], nodes[:, 1]
+    if I.min() < 0 or I.max() >= NX or J.min() < 0 or J.max() >= NY:
+        raise H.LBMError(f"{name}: group node outside the {NX} x {NY} lattice")
+    if kind in (H.BC_INSIDE, H.BC_BOUNCEBACK):
+        nb = np.asarray(table)[I, J].reshape(-1, 9, 2).astype(np.int64)
+        none = (nb[..., 0] == -1) & (nb[..., 1] == -1)
+        inside = (nb[..., 0] >= 0) & (nb[..., 0] < NX) & (nb[..., 1] >= 0) & (nb[..., 1] < NY)
+        bad = ~(none | inside)
+        if bad.any():
+            m, k = np.argwhere(bad)[0]
+            raise H.LBMError(f"{name}: neighbour table entry {tuple(nb[m, k])} of node {tuple(nodes[m])}, "
+                             f"population {k}, is neither -1 nor a node of the lattice")
+    elif kind in (H.BC_VELOCITY_NEEM, H.BC_PRESSURE_NEEM, H.BC_OPEN_NEUMANN):
+        if direction not in (1, 2, 3, 4):
+            raise H.LBMError(f"{name}: direction {direction} (1..4)")
+        I2, J2 = I - _C[direction, 0], J - _C[direction, 1]
+        out = (I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)
+        if out.any():
+            m = int(np.argmax(out))
+            raise H.LBMError(f"{name}: node {tuple(nodes[m])} reads its interior neighbour "
+                             f"{(int(I2[m]), int(J2[m]))} outside the lattice (direction {direction})")
+
+
 class BoundaryCondition:
     """BoundaryCondition.py:41-54."""
 
@@ -122,9 +160,16 @@ class BoundaryCondition:
         return BCSpec(self.kind, self.group.nodes, name=name)
 
     def _granular(self, lb_field, kind, rule=None):
+        need_nb = kind in (H.BC_INSIDE, H.BC_BOUNCEBACK)
+        key = (kind, id(self.group), len(self.group.nodes), lb_field.geometry_version,
+               rule.direction if rule is not None else 0)
+        if getattr(self, "_checked", None) != key:
+            check_granular_bounds(kind, self.group.nodes, lb_field.NX, lb_field.NY,
+                                  rule.direction if rule is not None else None,
+                                  lb_field.neighbor.arr if need_nb else None, type(self).__name__)
+            self._checked = key
         lb_field.materialize()
         g = group_device(self.group)
-        need_nb = kind in (H.BC_INSIDE, H.BC_BOUNCEBACK)
         H.check(H.lib().lbm_g_boundary(ctypes.byref(lattice_struct(lb_field)),
                                        ctypes.byref(gfields(lb_field, neighbor=need_nb)),
                                        kind, ctypes.byref(rule) if rule is not None else None, g[0], g[1],

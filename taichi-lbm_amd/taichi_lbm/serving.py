@@ -27,7 +27,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from .batch import BatchRunner, LatticeSpec, footprint_bytes
+from .batch import BatchRunner, LatticeSpec, footprint_bytes, plane_stride, stored_nodes
 from ._device import torch
 
 
@@ -45,6 +45,7 @@ class Request:
     residual: float = float("inf")
     converged: bool = False
     failed: bool = False            # diverged: non-finite velocity at a tick
+    rejected: Optional[str] = None  # why it can never be admitted (then failed, never run)
     result: Optional[np.ndarray] = None
 
     @property
@@ -70,6 +71,7 @@ class ServeStats:
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
             "converged": int(sum(r.converged for r in self.completed)),
             "failed": int(sum(r.failed for r in self.completed)),
+            "rejected": int(sum(r.rejected is not None for r in self.completed)),
             "ticks": self.ticks,
             "peak_resident": self.peak_resident,
             "peak_resident_bytes": self.peak_bytes,
@@ -120,8 +122,18 @@ class SolveServer:
               on_tick: Optional[Callable] = None) -> ServeStats:
         """Runs until every request has finished; arrivals are honoured in real time."""
         t = torch()
-        pending = sorted(requests, key=lambda r: r.arrival)
         stats = ServeStats()
+        pending = []
+        for req in sorted(requests, key=lambda r: r.arrival):
+            # a request that cannot fit even an empty server would hold the
+            # arrival-order queue forever: it is finished at once as failed
+            why = never_fits(req.spec, self.storage, self.budget, self.capacity_nodes, self.runner.max_slots,
+                             self.runner.align_rows)
+            if why is None:
+                pending.append(req)
+            else:
+                req.rejected, req.failed, req.finished = why, True, req.arrival
+                stats.completed.append(req)
         stream = t.cuda.Stream()
         t0 = clock()
         with t.cuda.stream(stream):
@@ -170,6 +182,21 @@ class SolveServer:
             self.graph.close()
             self.graph = None
         self.runner.close()
+
+
+def never_fits(spec: LatticeSpec, storage: str, budget_bytes: int, capacity_nodes: int, max_slots: int = 1,
+               align_rows: bool = True) -> Optional[str]:
+    """Why a request can never be admitted by a server with this HBM budget
+    and arena (None if an otherwise empty server admits it)."""
+    need = footprint_bytes(spec, storage)
+    if need > budget_bytes:
+        return f"footprint {need} B exceeds the HBM budget of {budget_bytes} B"
+    nodes = plane_stride(stored_nodes(spec) if align_rows else spec.N)
+    if nodes > capacity_nodes:
+        return f"{nodes} stored nodes exceed the arena of {capacity_nodes} nodes"
+    if max_slots < 1:
+        return "the server has no slots"
+    return None
 
 
 def _residual(d2: float, u2: float):
