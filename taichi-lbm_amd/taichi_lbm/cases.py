@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from .BoundaryCondition import (BounceBackWall, BoundaryClassifier, BoundaryEngine, BoundarySpec, FluidBoundary,
-                                PeriodicAllBoundary, PressureNEEM, VelocityNEEM)
+                                InsideBoundary, PeriodicAllBoundary, PressureNEEM, VelocityNEEM)
 from .CollisionEngine import BGKCollision, MRTCollision, TRTCollision
 from .FluidModels import NewtonianFluid, PowerLawFluid
 from .lbm import LBField, MacroscopicEngine
@@ -283,8 +283,36 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
 
 
 
+def reference_inside_map(mask: np.ndarray) -> np.ndarray:
+    """The InsideBoundary group of cases/contraction flow.py:96-108 and
+    cases/Power Law Fluid.py:75-87: fluid nodes with at least one neighbour
+    x - c_k strictly inside the lattice (0 < i < NX-1, 0 < j < NY-1) that
+    is fluid."""
+    from ._codes import shifted
+    NX, NY = mask.shape
+    interior_fluid = np.zeros((NX, NY), bool)
+    interior_fluid[1:NX - 1, 1:NY - 1] = mask[1:NX - 1, 1:NY - 1] == 1
+    out = np.zeros((NX, NY), bool)
+    for k in range(9):
+        out |= shifted(interior_fluid, int(C[k, 0]), int(C[k, 1]), False)
+    return out & (mask == 1)
+
+
+def edge_or_solid_map(mask: np.ndarray) -> np.ndarray:
+    """The BounceBackWall group of cases/Power Law Fluid.py:89-99 (and of
+    cases/contraction flow.py:111-123, whose predicate reads mask[-1, j] and
+    mask[NX, j] out of range at the inlet and outlet columns; here a
+    neighbour beyond any lattice edge counts as a wall): fluid nodes with a
+    neighbour x - c_k outside the lattice or solid."""
+    NX, NY = mask.shape
+    out = solid_neighbour_map(mask, y_walls=True)
+    edge = np.zeros((NX, NY), bool)
+    edge[[0, NX - 1], :] = True
+    return out | (edge & (mask == 1))
+
+
 def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt", magic=0.25, tau_min=0.51,
-                tau_max=2.0, name="contraction", inlet="bb"):
+                tau_max=2.0, name="contraction", inlet="bb", stream="periodic"):
     """4:1 planar contraction scaled from cases/contraction flow.py:16-33,80-81
     (a 40 x 8 channel with solid blocks over x in (25, 35) leaving the gap
     y in (3, 5)): VelocityBB inlet over a bounce-back inlet column (d=3,
@@ -292,10 +320,13 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     inlet instead, which diverges like the cylinder's), PressureNEEM outlet
     (d=1, rho=1), bounce-back walls, power-law fluid with m chosen so that the
     apparent viscosity is nu_ref at the inlet shear rate u_in / (H/2), tau
-    clamped to [tau_min, tau_max].  The reference's table stream
-    (InsideBoundary) reads outside the field at the inlet column; the
-    equivalent periodic pull is used, the bounce-back column then replacing
-    what would come from outside."""
+    clamped to [tau_min, tau_max].  stream="inside" keeps the reference's
+    boundary list (cases/contraction flow.py:96-149): InsideBoundary over its
+    interior-neighbour group (table stream), then BounceBackWall over the
+    fluid nodes next to a solid or beyond a lattice edge, VelocityBB,
+    PressureNEEM; every -1 table entry the table stream would read is then
+    bounced.  stream="periodic" streams with PeriodicAllBoundary instead and
+    bounces the inlet column: the same fields."""
     NX, NY = int(W), int(H)
     lbf = LBField(name, NX, NY, 1)
     bce = BoundaryEngine()
@@ -307,12 +338,19 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
     fluid.precompute(clf)
     bce.add_boundary_condition("fluid", fluid)
-    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
-    periodic.precompute(clf)
-    bce.add_boundary_condition("periodic", periodic)
-    wmap = solid_neighbour_map(m, y_walls=True)
-    if inlet == "bb":
-        wmap[0, :] |= m[0, :] == 1
+    if stream == "inside":
+        imap = reference_inside_map(m)
+        inside = InsideBoundary(BoundarySpec(lambda I, J: imap[I, J]))
+        inside.precompute(clf)
+        bce.add_boundary_condition("inside", inside)
+        wmap = edge_or_solid_map(m)
+    else:
+        periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+        periodic.precompute(clf)
+        bce.add_boundary_condition("periodic", periodic)
+        wmap = solid_neighbour_map(m, y_walls=True)
+        if inlet == "bb":
+            wmap[0, :] |= m[0, :] == 1
     wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
     wall.precompute(clf)
     bce.add_boundary_condition("wall", wall)
@@ -337,7 +375,46 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     macro, rho, vel = _finish(lbf, bce, fluid, coll)
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
                 spec=dict(kind="contraction", NX=NX, NY=NY, power_n=power_n, power_m=power_m, u_in=u_in,
-                          nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max, inlet=inlet))
+                          nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max, inlet=inlet,
+                          stream=stream))
+
+
+def cavity(N=256, power_n=1.0, Re=10.0, u_lid=5e-3, collision="mrt", magic=0.25, name="cavity"):
+    """Power-law lid-driven cavity of cases/Power Law Fluid.py:14-141: an
+    N x N lattice without solids, InsideBoundary over the interior-neighbour
+    group, BounceBackWall over the nodes next to a lattice edge, a VelocityBB
+    lid (direction 2, u = (u_lid, 0)) over the top row j = N-1; power-law
+    consistency K = u_lid^(2-n) (N-1)^n / Re (:40-41, with the reference's
+    lattice velocity Ma cs = u_lid), shearviscosity = bulkviscosity = [K],
+    magic 1/4, rho = 1, u = 0, default tau clamps."""
+    from .BoundaryCondition import VelocityBB
+    NX = NY = int(N)
+    lbf = LBField(name, NX, NY, 1)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    m = lbf.mask.arr
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: m[I, J] == 1))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    imap = reference_inside_map(m)
+    inside = InsideBoundary(BoundarySpec(lambda I, J: imap[I, J]))
+    inside.precompute(clf)
+    bce.add_boundary_condition("inside", inside)
+    wmap = edge_or_solid_map(m)
+    wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
+    wall.precompute(clf)
+    bce.add_boundary_condition("wall", wall)
+    lid = VelocityBB(BoundarySpec(lambda I, J: (J == NY - 1) & (I >= 0)), (u_lid, 0.0), 2)
+    lid.precompute(clf)
+    bce.add_boundary_condition("inlet", lid)
+    lbf.neighbor_classify()
+    power_K = u_lid ** (2.0 - power_n) * (NX - 1.0) ** power_n / Re
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [power_K], PowerLawFluid(power_K, power_n), magic=magic,
+                          nu_bulk=[power_K])
+    macro, rho, vel = _finish(lbf, bce, fluid, coll)
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
+                spec=dict(kind="cavity", NX=NX, NY=NY, power_n=power_n, power_K=power_K, Re=Re, u_lid=u_lid,
+                          collision=collision))
 
 
 def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt", tau_bulk=None,

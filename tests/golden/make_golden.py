@@ -55,6 +55,12 @@ CASES = {
     "porous_powerlaw_trt_48x40": ("porous", dict(NX=48, NY=40, seed=4, collision="trt", rmin=3.0, rmax=6.0,
                                                  g=(2e-5, 0.0)), 20),
     "contraction_powerlaw_80x24": ("contraction", dict(W=80, H=24, power_n=0.7), 20),
+    # the reference's table-stream lists: InsideBoundary + BounceBackWall + VelocityBB (+ PressureNEEM)
+    # (cases/contraction flow.py:96-149; cases/Power Law Fluid.py:64-117, the lid moving along +x)
+    "contraction_inside_trt_80x24": ("contraction", dict(W=80, H=24, power_n=0.7, stream="inside"), 20),
+    # (n = 1 through the power-law path, as the reference runs it: with n != 1 a cavity started at rest
+    # takes tau from the FP32 rounding noise of its near-zero shear rates, see tests/test_inside_cases.py)
+    "cavity_mrt_40x40": ("cavity", dict(N=40, power_n=1.0, u_lid=0.02), 30),
     "shan_chen_c1_32x32": ("shan_chen", dict(NX=32, NY=32, nc=1, G=-5.0, seed=7), 20),
     "shan_chen_c2_32x24": ("shan_chen", dict(NX=32, NY=24, nc=2, gcross=0.9, seed=9), 20),
     # the reference's own cylinder channel at full size (cases/Circular cylinder.py: 400 x 100, BGK
@@ -62,9 +68,11 @@ CASES = {
     # VelocityNEEM inlet, which diverges after about 150 steps, so its vectors stop at 60
     "cylinder_400x100_vbb": ("cylinder", dict(inlet="bb"), 100),
     "cylinder_400x100_neem": ("cylinder", dict(inlet="neem"), 60),
+    # the reference's own power-law cavity at full size (256 x 256, n = 1, Re = 10, U = 5e-3, MRT)
+    "cavity_256_reference": ("cavity", dict(), 100),
 }
 # full-size lattices keep every SUB32-th (FP32) / SUB64-th (FP64) node in each direction
-SUBSAMPLED = {"cylinder_400x100_vbb", "cylinder_400x100_neem"}
+SUBSAMPLED = {"cylinder_400x100_vbb", "cylinder_400x100_neem", "cavity_256_reference"}
 SUB32, SUB64 = 2, 4
 
 
@@ -119,8 +127,8 @@ def sub(fx, key):
     return SUB64 if key.startswith("out64") else SUB32
 
 
-def main():
-    for name in CASES:
+def main(names=None):
+    for name in names or CASES:
         arrays = make(name)
         path = HERE / f"{name}.npz"
         np.savez_compressed(path, **arrays)
@@ -128,4 +136,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -103,3 +103,47 @@ def test_two_component_shan_chen_mass_fp64(N, steps):
     print(f"  N={N}: FP64 mass drift per component after {steps} steps = {drift} (M^-1 bias {bias:.3e})")
     assert red["nonfinite"] == 0
     assert np.all(drift > 0.5 * bias) and np.all(drift < 2.0 * bias)
+
+
+def test_power_law_cavity_256_reference():
+    """The reference's power-law lid-driven cavity at its own size
+    (cases/Power Law Fluid.py: 256 x 256, InsideBoundary table stream,
+    BounceBackWall at the edges, VelocityBB lid d = 2, MRT, n = 1 through the
+    power-law path, Re = 10), 4000 steps fused: HIP = C++ restatement, and
+    the FP32 field's distance to the FP64 path."""
+    n = 4000
+    print(f"case cavity 256x256 mrt power law, {n} steps", flush=True)
+    case = _quiet(cases.cavity)
+    cl = cpu_ref.from_case(case, nthreads=16)
+    cl.run(n)
+    _quiet(cases.initialise, case)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True)
+    st.run(n)
+    lbf = case["lbf"]
+    for name in ("f", "f2", "rho", "vel", "total_rho"):
+        check(name, getattr(lbf, name).to_numpy(), getattr(cl, name), growth=10.0)
+    vel = lbf.vel.to_numpy()
+    assert vel[:, -1, 0].max() > 2e-3          # the lid drives the top row
+    r64 = BatchRunner([spec_from_case(_quiet(cases.cavity))], storage="f64")
+    r64.run(n)
+    err = np.abs(vel - r64.fields(0)["vel"]).max()
+    print(f"  max |u_fp32 - u_fp64| after {n} steps = {err:.2e}")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("tile_cols", [0, 64])
+def test_contraction_table_stream_equals_periodic_stream(tile_cols):
+    """The reference's table-stream list (InsideBoundary + BounceBackWall at
+    solids and edges + VelocityBB + PressureNEEM) and the periodic-stream
+    list give the same power-law contraction fields bit for bit on the GPU
+    (every -1 entry of the table is bounced)."""
+    print(f"case contraction 320x96 n=0.6 inside vs periodic, 300 steps, tiles {tile_cols}", flush=True)
+    runs = []
+    for stream in ("inside", "periodic"):
+        case = _quiet(cases.contraction, W=320, H=96, power_n=0.6, stream=stream)
+        _quiet(cases.initialise, case)
+        Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True, tile_cols=tile_cols).run(300)
+        runs.append({k: getattr(case["lbf"], k).to_numpy() for k in ("f", "rho", "vel", "shear_rate")})
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+    assert np.isfinite(runs[0]["f"]).all()

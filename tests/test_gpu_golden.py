@@ -68,10 +68,13 @@ def test_fused_against_fixture(name):
     assert err64 < 1e-5
 
 
-@pytest.mark.parametrize("name", ["cylinder_bgk_48x24", "cylinder_refbcs_bgk_60x24", "porous_powerlaw_mrt_48x40"])
+@pytest.mark.parametrize("name", ["cylinder_bgk_48x24", "cylinder_refbcs_bgk_60x24", "porous_powerlaw_mrt_48x40",
+                                  "contraction_inside_trt_80x24", "cavity_mrt_40x40"])
 def test_granular_sequence_against_fixture(name):
     """lbm_solve's order (cases/Circular cylinder.py:182-193, cases/Power Law
-    Fluid.py:160-173) through the one-kernel-per-method entry points."""
+    Fluid.py:160-173) through the one-kernel-per-method entry points; the
+    table-stream lists run InsideBoundary (BoundaryCondition.py:435-449)
+    and a VelocityBB with direction 3 (inlet) and 2 (lid)."""
     print(f"golden granular {name}", flush=True)
     fx = _load(name)
     meta = json.loads(str(fx["meta"]))
