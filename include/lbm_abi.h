@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 9
+#define LBM_ABI_VERSION 10
 
 /* status codes */
 #define LBM_OK 0
@@ -184,6 +184,22 @@ int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* 
 int lbm_run_blocked(const lbm_lattice* lattice, const lbm_fields* fields, const lbm_neem_rule* neem,
                     int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols, void* tau_next,
                     int32_t* out_parity, void* stream);
+/* lbm_run_blocked without the tau_next argument, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The second relaxation-state buffer of two-step
+ * power-law runs is a fixed workspace of 2^22 floats (16 MiB) in device
+ * memory that every GPU the library runs on holds until the process ends,
+ * so nc * plane_stride must not exceed 4194304 (a 2048 x 2048 lattice with
+ * one component, 2048 x 1024 with two); larger power-law lattices are
+ * refused with LBM_E_INVALID.  The call makes no hipMalloc, hipFree or other
+ * allocating call.  All calls on one GPU share its workspace and nothing in
+ * it needs to start at zero: calls that overlap on one GPU (different
+ * streams or host threads) race and must be serialised by the caller.
+ * Results are those of lbm_run_blocked bit for bit.                       */
+int lbm_run_blocked_noalloc(const lbm_lattice* lattice, const lbm_fields* fields, const lbm_neem_rule* neem,
+                            int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
+                            int32_t* out_parity, void* stream);
 /* Rows of a two-step tile with `ring` = 1 or 2 and `tile_cols` = 16, 32 or
  * 64 columns (0: the library's default width).                            */
 int lbm_tile_rows(int32_t ring, int32_t tile_cols);

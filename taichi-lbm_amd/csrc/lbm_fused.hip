@@ -34,6 +34,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1587,10 +1588,12 @@ bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parit
 }
 }  // namespace lbm
 
-extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
-                               int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
-                               void* tau_next, int32_t* out_parity, void* stream) {
-  using namespace lbm;
+namespace lbm {
+// lbm_run_blocked with the second relaxation-state buffer given (a caller's
+// buffer, or the fixed workspace of lbm_run_blocked_noalloc).  Launches only.
+int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem, int32_t nsteps,
+                int32_t parity, int32_t first_local, int32_t tile_cols, void* tau_next, int32_t* out_parity,
+                void* stream) {
   int rc = validate(lat, fld, neem);
   if (rc) return rc;
   if (nsteps < 0) return fail("nsteps %d < 0", nsteps);
@@ -1658,6 +1661,66 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
   }
   *out_parity = (int32_t)p;
   return LBM_OK;
+}
+
+// Workspace of lbm_run_blocked_noalloc: the second relaxation-state buffer of
+// two-step power-law runs (nc planes of plane_stride floats), a fixed device
+// array instead of a caller's buffer.  Every GPU holds its own copy, at its
+// own address, until the process ends; the name is a host-side stand-in, so
+// host code uses only the per-device address from hipGetSymbolAddress.
+// Nothing in it has to start at zero: a launch writes the state of every
+// active node of its tiles before the next launch reads it, and the final
+// copy-back reads active nodes only.
+constexpr size_t kNoAllocStateMax = size_t(1) << 22;   // floats (16 MiB)
+static __device__ __align__(256) float scratch[kNoAllocStateMax];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+// Device address of the current device's workspace, looked up once per
+// device (with that device current).
+int scratch_of_current_device(void** out) {
+  int dev = -1;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) return fail("device %d beyond the %d the workspace table holds", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if ((rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))) return rc;
+    if (!ptr) return fail("hipGetSymbolAddress returned no address for the workspace");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+}  // namespace lbm
+
+extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
+                               int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
+                               void* tau_next, int32_t* out_parity, void* stream) {
+  return lbm::run_blocked(lat, fld, neem, nsteps, parity, first_local, tile_cols, tau_next, out_parity, stream);
+}
+
+extern "C" int lbm_run_blocked_noalloc(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
+                                       int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
+                                       int32_t* out_parity, void* stream) {
+  using namespace lbm;
+  int rc = validate(lat, fld, neem);
+  if (rc) return rc;
+  void* ws = nullptr;
+  // the workspace is used where lbm_run_blocked would use tau_next: two-step
+  // power-law runs (FP64 storage and Shan-Chen run one step per launch)
+  if (lat->tau_mode == LBM_TAU_POWER_LAW && lat->storage != LBM_STORE_F64 && lat->shan_chen == LBM_SC_NONE) {
+    const uint64_t need = (uint64_t)lat->nc * (uint64_t)lat->plane_stride;
+    if (need > kNoAllocStateMax)
+      return fail("lbm_run_blocked_noalloc: the power-law state of %d component(s) x %lld nodes exceeds the "
+                  "workspace of %llu values; use lbm_run_blocked with a tau_next buffer",
+                  lat->nc, (long long)lat->plane_stride, (unsigned long long)kNoAllocStateMax);
+    if ((rc = scratch_of_current_device(&ws))) return rc;
+  }
+  return run_blocked(lat, fld, neem, nsteps, parity, first_local, tile_cols, ws, out_parity, stream);
 }
 
 extern "C" int lbm_finish(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,

@@ -26,7 +26,7 @@ CODE_NEEM_MASK = 0x3C00
 CODE_PULLED = 0x4000      # inactive node pulled by an active one (plain pull)
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 
 class NeemRule(C.Structure):
@@ -95,6 +95,7 @@ def lib():
         "lbm_last_error": ([], C.c_char_p),
         "lbm_run": ([LP, FP, NP, I32, I32, I32, P], C.c_int),
         "lbm_run_blocked": ([LP, FP, NP, I32, I32, I32, I32, P, C.POINTER(C.c_int32), P], C.c_int),
+        "lbm_run_blocked_noalloc": ([LP, FP, NP, I32, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
         "lbm_tile_rows": ([I32, I32], C.c_int),
         "lbm_finish": ([LP, FP, NP, I32, P], C.c_int),
         "lbm_reduce_macro": ([LP, P, P, D, P], C.c_int),
@@ -144,7 +145,8 @@ def lib():
 
 def exported_symbols():
     """Names of every entry point the header declares (checked by the tests)."""
-    return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_run_blocked", "lbm_finish", "lbm_reduce_macro",
+    return ["lbm_abi_version", "lbm_last_error", "lbm_run", "lbm_run_blocked", "lbm_run_blocked_noalloc",
+            "lbm_finish", "lbm_reduce_macro",
             "lbm_residual", "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",

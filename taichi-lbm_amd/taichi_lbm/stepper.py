@@ -40,16 +40,21 @@ def _same_nodes(g1, g2, NY) -> bool:
 
 class Stepper:
     def __init__(self, lb_field, macroscopic_engine, collision_engine, boundary_engine, power_law=False,
-                 boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0):
+                 boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0,
+                 no_alloc=False):
         """blocked: run two steps per launch where the lattice allows it
         (lbm_run_blocked; bit-identical to one step per launch), on tiles of
         tile_cols = 16, 32 or 64 columns (0: the library's default; "auto":
         the first run of at least tune_steps() steps times TUNE_STEPS steps
         with each of TUNE_CANDIDATES, twice, and keeps the fastest -- the
         fields do not depend on the width, so those steps are ordinary steps
-        of the run)."""
+        of the run).  no_alloc: two-step runs go through
+        lbm_run_blocked_noalloc, which keeps the power-law state's second
+        buffer in the library's fixed device workspace (nothing is allocated
+        per run; lattices up to 2^22 state values)."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
+        self.no_alloc = bool(no_alloc)
         self._tune = tile_cols == "auto"
         self.tile_cols = 0 if self._tune else int(tile_cols)
         self.tune_timings = {}
@@ -170,7 +175,13 @@ class Stepper:
             # a launch writes its psi into plane set parity ^ 1; a run of two
             # or more leaves the psi of its last two steps in both sets
             self.sc.keep_previous(parity ^ 1)
-        if self.blocked:
+        if self.blocked and self.no_alloc:
+            out = ctypes.c_int32(-1)
+            H.check(H.lib().lbm_run_blocked_noalloc(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity,
+                                                    first_local, self.tile_cols, ctypes.byref(out), stream()),
+                    "lbm_run_blocked_noalloc")
+            st["f2_idx"] = int(out.value)
+        elif self.blocked:
             tau_next = None
             if tau_mode == H.TAU_POWER_LAW:
                 state = self.coll.state_tensor()

@@ -74,3 +74,22 @@ def test_invalid_arguments_fail_cleanly():
     assert rc == -1 and b"missing" in lib.lbm_last_error()
     with pytest.raises(H.LBMError):
         H.check(rc, "lbm_run")
+
+
+def test_no_alloc_entry_point_refuses_a_state_beyond_its_workspace():
+    # the size check comes before any HIP call (nothing runs on a device)
+    lib = H.lib()
+    L = H.Lattice()
+    L.nx, L.ny, L.nc = 4096, 2048, 1
+    L.plane_stride = 4096 * 2048
+    L.tau_mode = H.TAU_POWER_LAW
+    L.tau_min, L.tau_max = 0.51, 2.0
+    F = H.Fields()
+    for name in ("tau", "code"):
+        setattr(F, name, 256)
+    F.f[0] = F.f[1] = F.macro[0] = F.macro[1] = 256
+    out = ctypes.c_int32(-1)
+    rc = lib.lbm_run_blocked_noalloc(ctypes.byref(L), ctypes.byref(F), None, 2, 0, 1, 0, ctypes.byref(out), None)
+    assert rc == -1
+    assert b"exceeds the workspace" in lib.lbm_last_error()
+    assert out.value == -1
