@@ -127,17 +127,25 @@ def cpu_baseline(case, steps, one_thread_steps=3):
             "host": {"cpu_model": model, "cpus_available": len(os.sched_getaffinity(0))}}
 
 
-def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100, n_fp64=16):
+def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=100, n_fp64=16,
+                storages=("f32", "f16", "bf16"), bar=1e-4):
     """512 independent cylinder channels (NX ~ U{128..1024}, NY ~ U{64..256},
     r ~ U[4, NY/8] at (max(NX/5, r + 4), NY/2), u_in ~ U[0.01, 0.08], tau ~ U[0.52, 0.8],
     BGK or TRT (magic 1/4) by a seeded coin; bounce-back walls, VelocityBB
-    inlet, PressureNEEM outlet), FP32: batched launches (two steps per launch
-    over the tiles of every lattice) replayed from a graph, against the plain
-    path (one lbm_run launch per lattice per step) and, for reference, the
-    plain path with two steps per launch (lbm_run_blocked per lattice).  Afterwards every lattice is run to solve_steps in the
-    batch and the non-finite ones are counted; a sample is rerun in FP64 and
-    the largest velocity difference reported (the plain and batched paths
-    give bit-identical fields, so one number describes both)."""
+    inlet, PressureNEEM outlet).
+
+    Batched: two steps per launch over the tiles of every lattice, replayed
+    from a graph, with FP32 populations and with shifted FP16 / BF16 storage
+    (f - w in 16 bits, FP32 arithmetic).  Plain FP32 path, one lattice per
+    launch, four ways: one step per launch (lbm_run) and two steps per launch
+    (lbm_run_blocked), each launched from the host and replayed from a graph
+    that holds every lattice's launches.  Accuracy: every lattice is run to
+    solve_steps in each batched format, the non-finite ones are counted, and
+    a sample of n_fp64 finite lattices is rerun in FP64; the largest velocity
+    difference is reported per format (the plain and batched FP32 paths give
+    bit-identical fields, so the FP32 number describes both).  The speedup at
+    matched accuracy compares the fastest batched format whose error meets
+    the bar with the fastest plain path."""
     import ctypes
     import numpy as np
     import torch
@@ -159,89 +167,121 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     fluid = sum(int(((s.code & H.CODE_ACTIVE) != 0).sum()) for s in specs)
     log(f"batched setup {time.time() - t0:.1f}s: {n_lattices} lattices, {nodes} nodes")
     stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        runner = BatchRunner(specs, storage="f32")
-        runner.run(2)
-        mode = runner.mode
-        tile_cols = 0
-        if mode == 0:          # the persistent one-step grid is tuned; two-step launches have a block per tile
-            grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
-            log(f"autotuned grid {grid}: {timings}")
-        else:                  # two-step launches: the tile width is tuned
-            grid = 0
-            tile_cols, timings = autotune_tiles(runner, candidates=(16, 32, 64), steps=10)
-            log(f"autotuned tile columns {tile_cols}: {timings}")
-            runner.run(10)
-        g = runner.capture(graph_steps)
-        g.replay()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps):
         torch.cuda.synchronize()
-        reps = max(1, steps // graph_steps)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for _ in range(reps):
-            g.replay()
+            fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        ms_b = e0.elapsed_time(e1) / (reps * graph_steps)
-        g.close()
-        runner.close()
-        # plain path: one launch per lattice per step (lbm_run on each lattice's
-        # own buffers), on the lattices as given (rows not padded)
+        return e0.elapsed_time(e1)
+
+    out = {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid,
+           "stored_nodes": sum(stored_nodes(sp) for sp in specs), "steps_per_solve": solve_steps,
+           "graph_steps": graph_steps}
+    ms_b, tile_cols, grid, mode = {}, 0, 0, None
+    with torch.cuda.stream(stream):
+        for storage in storages:
+            runner = BatchRunner(specs, storage=storage, grid_blocks=grid)
+            runner.run(2)
+            mode = runner.mode if mode is None else mode
+            if storage == storages[0]:
+                if runner.mode == 0:    # the persistent one-step grid is tuned; two-step launches have a block per tile
+                    grid, timings = autotune_grid(runner, candidates=(2, 4, 8, 16), steps=10)
+                    log(f"autotuned grid {grid}: {timings}")
+                else:                   # two-step launches: the tile width is tuned
+                    tile_cols, timings = autotune_tiles(runner, candidates=(16, 32, 64), steps=10)
+                    log(f"autotuned tile columns {tile_cols}: {timings}")
+            elif tile_cols:
+                runner.set_tile_cols(tile_cols)
+                runner.fit_tables()
+            runner.run(10)
+            g = runner.capture(graph_steps)
+            g.replay()
+            reps = max(1, steps // graph_steps)
+            ms_b[storage] = timed(g.replay, reps) / (reps * graph_steps)
+            log(f"batched {storage}: {ms_b[storage]:.4f} ms/step")
+            g.close()
+            runner.close()
+        out["timed_steps_batched"] = max(1, steps // graph_steps) * graph_steps
+        # the plain FP32 path: each lattice on its own buffers, as given (rows not padded)
         plain = BatchRunner(specs, storage="f32", blocked=False, align_rows=False)
         plain.run(2)
         lib = H.lib()
         s = stream.cuda_stream
         par = plain.parity
         slots = sorted(plain.slots)
+        outp = ctypes.c_int32()
 
         def rules(l):
             return ctypes.cast(ctypes.addressof(plain.neem) + l * H.MAX_NEEM * ctypes.sizeof(H.NeemRule),
                                ctypes.POINTER(H.NeemRule))
-        for l in slots:        # warm-up
-            H.check(lib.lbm_run(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l), 2, par, 0, s))
-        torch.cuda.synchronize()
-        e0.record(stream)
-        for l in slots:
-            H.check(lib.lbm_run(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l),
-                                plain_steps, par, 0, s))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_p = e0.elapsed_time(e1) / plain_steps
-        # the same, each lattice with two steps per launch (lbm_run_blocked)
-        out = ctypes.c_int32()
-        e0.record(stream)
-        for l in slots:
-            H.check(lib.lbm_run_blocked(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l),
-                                        plain_steps, par, 0, 0, None, ctypes.byref(out), s))
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_p2 = e0.elapsed_time(e1) / plain_steps
+
+        def one_step(n):        # lbm_run per lattice: n launches each
+            for l in slots:
+                H.check(lib.lbm_run(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l), n, par, 0, s))
+
+        def two_step(n):        # lbm_run_blocked per lattice: n / 2 launches each
+            for l in slots:
+                H.check(lib.lbm_run_blocked(ctypes.byref(plain.lats[l]), ctypes.byref(plain.flds[l]), rules(l),
+                                            n, par, 0, 0, None, ctypes.byref(outp), s))
+        ms_p = {}
+        for name, fn in (("one_step", one_step), ("two_step", two_step)):
+            fn(4)                                               # warm-up
+            ms_p[name] = timed(lambda: fn(plain_steps), 1) / plain_steps
+            # the same launches captured once per 4 steps (the parity comes back
+            # to where it started) and replayed
+            cg = torch.cuda.CUDAGraph()
+            torch.cuda.synchronize()
+            with torch.cuda.graph(cg, stream=stream):
+                fn(4)
+            cg.replay()
+            reps = max(1, plain_steps // 4)
+            ms_p[name + "_graph"] = timed(cg.replay, reps) / (reps * 4)
+            del cg
+            log(f"plain {name}: {ms_p[name]:.4f} ms/step host-launched, {ms_p[name + '_graph']:.4f} from a graph")
         plain.close()
-        # accuracy at solve_steps: a fresh batch run to the end of the solve
-        runner = BatchRunner(specs, storage="f32", grid_blocks=grid)
-        runner.run(solve_steps)
-        u32 = [runner.velocity(l) for l in range(n_lattices)]
-        runner.close()
-        finite = [bool(np.isfinite(u).all()) for u in u32]
-        sample = [l for l in range(n_lattices) if finite[l]][:n_fp64]
+        # accuracy at solve_steps, per batched format, against FP64 on a sample
+        vel = {}
+        for storage in storages:
+            runner = BatchRunner(specs, storage=storage, grid_blocks=grid)
+            runner.run(solve_steps)
+            vel[storage] = [runner.velocity(l) for l in range(n_lattices)]
+            runner.close()
+        finite = {st: [bool(np.isfinite(u).all()) for u in v] for st, v in vel.items()}
+        sample = [l for l in range(n_lattices) if all(finite[st][l] for st in storages)][:n_fp64]
         r64 = BatchRunner([specs[l] for l in sample], storage="f64", grid_blocks=grid)
         r64.run(solve_steps)
-        err64 = max(float(np.abs(u32[l] - r64.velocity(q)).max()) for q, l in enumerate(sample)) if sample else None
+        u64 = [r64.velocity(q) for q in range(len(sample))]
         r64.close()
-    sps_b = n_lattices / (ms_b * 1e-3 * solve_steps)
-    sps_p = n_lattices / (ms_p * 1e-3 * solve_steps)
-    return {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid,
-            "stored_nodes": sum(stored_nodes(sp) for sp in specs), "steps_per_solve": solve_steps,
-            "ms_per_step_batched": round(ms_b, 4), "ms_per_step_plain": round(ms_p, 4),
-            "mlups_batched": round(fluid / (ms_b * 1e-3) / 1e6, 1),
-            "hbm_frac_batched": round(fluid * 72 / (ms_b * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "solves_per_s_batched": round(sps_b, 3), "solves_per_s_plain": round(sps_p, 3),
-            "speedup": round(sps_b / sps_p, 2), "launch_mode": mode,
-            "ms_per_step_plain_two_step": round(ms_p2, 4),
-            "speedup_vs_plain_two_step": round(ms_p2 / ms_b, 2), "grid_blocks": grid, "tile_cols": tile_cols, "graph_steps": graph_steps,
-            "timed_steps_batched": reps * graph_steps, "timed_steps_plain": plain_steps,
-            "finite_after_solve": int(sum(finite)), "fp64_sample": len(sample),
-            "max_abs_u_err_vs_fp64": err64}
+    err = {st: (max(float(np.abs(vel[st][l] - u64[q]).max()) for q, l in enumerate(sample)) if sample else None)
+           for st in storages}
+    umax = max(float(np.abs(u).max()) for u in u64) if u64 else None
+    sps = lambda ms: n_lattices / (ms * 1e-3 * solve_steps)
+    best_plain = min(ms_p, key=ms_p.get)
+    ok = [st for st in storages if err[st] is not None and err[st] <= bar]
+    best_b = min(ok, key=lambda st: ms_b[st]) if ok else None
+    out.update({
+        "ms_per_step_batched": {st: round(v, 4) for st, v in ms_b.items()},
+        "ms_per_step_plain": {k: round(v, 4) for k, v in ms_p.items()},
+        "timed_steps_plain": plain_steps,
+        "mlups_batched": {st: round(fluid / (v * 1e-3) / 1e6, 1) for st, v in ms_b.items()},
+        "hbm_frac_batched": {st: round(fluid * 72 * (0.5 if st in ("f16", "bf16") else 1.0) / (v * 1e-3) / 1e9
+                                       / HBM_PEAK_GBS, 4) for st, v in ms_b.items()},
+        "solves_per_s_batched": {st: round(sps(v), 3) for st, v in ms_b.items()},
+        "solves_per_s_plain": {k: round(sps(v), 3) for k, v in ms_p.items()},
+        "speedup": round(ms_p["one_step"] / ms_b["f32"], 2),
+        "speedup_vs_best_plain_f32": round(ms_p[best_plain] / ms_b["f32"], 2),
+        "best_plain": best_plain,
+        "max_abs_u_err_vs_fp64": err, "max_abs_u_fp64": umax, "accuracy_bar": bar,
+        "matched_accuracy": {"batched": best_b, "plain": best_plain,
+                             "speedup": round(ms_p[best_plain] / ms_b[best_b], 2) if best_b else None},
+        "launch_mode": mode, "grid_blocks": grid, "tile_cols": tile_cols,
+        "finite_after_solve": {st: int(sum(v)) for st, v in finite.items()}, "fp64_sample": len(sample),
+    })
+    return out
 
 
 def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16", "bf16"), n_fp64=3,
