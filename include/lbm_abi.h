@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 10
+#define LBM_ABI_VERSION 11
 
 /* status codes */
 #define LBM_OK 0
@@ -54,6 +54,18 @@ enum {
 /* Shan-Chen variants (ExternalForce.py:55-145) */
 enum { LBM_SC_NONE = 0, LBM_SC_C1 = 1, LBM_SC_C2 = 2 };
 
+/* Pseudopotentials, EOS.py: SC_psi (:40-47) psi = rho0 (1 - exp(-rho/rho0));
+ * the cubic equations of state (:5-95) psi = sqrt(|6/G (p(rho, T) - rho/3)|)
+ * with T the node's temperature (lb_field.T) and eos = (a, b, R, G, omega,
+ * T_cr):
+ *   vdW  p = rho R T/(1 - b rho) - a rho^2                             (:50-58)
+ *   RK   p = rho R T/(1 - b rho) - a rho^2 / (sqrt(T) (1 + b rho))      (:61-69)
+ *   PR   p = rho R T/(1 - b rho) - a alpha(T) rho^2/(1 + 2 b rho - (b rho)^2),
+ *        alpha = (1 + (0.37464 + 1.54226 omega - 0.269922 omega^2)
+ *                     (1 - sqrt(T/T_cr)))^2                            (:72-83)
+ *   CS   p = rho R T (1 + x + x^2 - x^3)/(1 - x)^3 - a rho^2, x = b rho/4 (:86-95) */
+enum { LBM_PSI_SC = 0, LBM_PSI_VDW = 1, LBM_PSI_RK = 2, LBM_PSI_PR = 3, LBM_PSI_CS = 4 };
+
 /* node code: one uint16 per node, compiled on the host from the ordered
  * boundary-condition list (BoundaryCondition.py:459-483).               */
 #define LBM_CODE_BOUNCE_MASK 0x00FFu /* bit k-1 set: population k takes f2(x, opp k)   */
@@ -61,6 +73,9 @@ enum { LBM_SC_NONE = 0, LBM_SC_C1 = 1, LBM_SC_C2 = 2 };
 #define LBM_CODE_KEEP_MACRO 0x0200u  /* rho/u of this node are read by a node rule      */
 #define LBM_CODE_NEEM_SHIFT 10       /* bits 10..13: node rule index + 1 (0 = none)     */
 #define LBM_CODE_NEEM_MASK 0x3C00u
+#define LBM_CODE_SC_SOLID 0x8000u    /* fluid_strategy is false here: a Shan-Chen      *
+                                      * neighbour contributing adhesion, not cohesion  *
+                                      * (ExternalForce.py:105-116)                      */
 #define LBM_CODE_PULLED 0x4000u      /* node outside the fluid group whose populations  *
                                       * an active node pulls (a plain, not bounced,    *
                                       * population reads its unchanged f2)             */
@@ -71,7 +86,10 @@ enum {
   LBM_NEEM_VELOCITY = 1,     /* VelocityNEEM.apply, BoundaryCondition.py:149-166 */
   LBM_NEEM_PRESSURE = 2,     /* PressureNEEM.apply, :209-226                     */
   LBM_RULE_VELOCITY_BB = 3,  /* VelocityBB.apply, :85-102 (adds 6 w_k rho c_k.u) */
-  LBM_RULE_OPEN_NEUMANN = 4  /* OpenNeumann.apply, :321-334 (unknowns := f(x_nb)) */
+  LBM_RULE_OPEN_NEUMANN = 4, /* OpenNeumann.apply, :321-334 (unknowns := f(x_nb)) */
+  LBM_RULE_PRESSURE_ABC = 5, /* PressureABC.apply, :190-205 (unknowns := -f + 2 w rho_value (...), *
+                              * u_w = 1.5 u(x) - 0.5 u(x_nb))                                      */
+  LBM_RULE_VELOCITY_EQ = 6   /* VelocityEquilibrium.apply, :135-145 (all 9 := feq(u_w, rho_w))     */
 };
 
 /* Where the relaxation time comes from.  The per-node state (lbm_fields.tau,
@@ -88,12 +106,14 @@ enum {
 #define LBM_MAX_NC 2
 
 /* One node rule: VelocityNEEM (BoundaryCondition.py:149-166), PressureNEEM
- * (:209-226), VelocityBB (:85-102) or OpenNeumann (:321-334).            */
+ * (:209-226), VelocityBB (:85-102), OpenNeumann (:321-334), PressureABC
+ * (:177-205) or VelocityEquilibrium (:132-145).                           */
 typedef struct lbm_neem_rule {
   int32_t kind;      /* LBM_NEEM_VELOCITY | LBM_NEEM_PRESSURE | LBM_RULE_*   */
   int32_t direction; /* outward population index d in 1..4; x_nb = x - c_d */
   float ux, uy;      /* velocity_value (velocity NEEM, VelocityBB)          */
-  float rho;         /* rho_value (pressure rule; written to total_rho only)*/
+  float rho;         /* rho_value (pressure rules; PressureNEEM writes it to *
+                      * total_rho only, PressureABC also imposes it)        */
   int32_t pad;
 } lbm_neem_rule;
 
@@ -119,7 +139,12 @@ typedef struct lbm_lattice {
   float gx, gy;                /* gravity_force (force density, lbm.py:220)   */
   float g_coh[LBM_MAX_NC * LBM_MAX_NC]; /* C1: g_coh[0]; C2: matrix row-major */
   float psi_rho0[LBM_MAX_NC];  /* SC_psi rho0 per component (EOS.py:40-47)    */
-  int32_t pad0, pad1;
+  int32_t psi_kind[LBM_MAX_NC]; /* LBM_PSI_*: the psi object of each component */
+  float eos[LBM_MAX_NC][6];    /* cubic EOS (EOS.py:5-95): a, b, R, G, omega, T_cr */
+  float g_adh[LBM_MAX_NC];     /* ShanChenForceC2 gadh (ExternalForce.py:105-116) */
+  int32_t sc_adhesion;         /* 1: some node is not fluid for the Shan-Chen   *
+                                * force (LBM_CODE_SC_SOLID / lbm_gfields.sc_fluid) */
+  int32_t pad0;
 } lbm_lattice;
 
 /* Device buffers of one lattice (see the plane layout above). */
@@ -137,6 +162,7 @@ typedef struct lbm_fields {
   void* tau2;           /* batches: a second relaxation-state buffer (as    */
                         /* tau) that lets power-law lattices take two steps */
                         /* per launch; NULL: one step per launch            */
+  const float* temperature; /* 1 plane (lb_field.T): cubic-EOS psi only     */
 } lbm_fields;
 
 /* ---- version / errors ---------------------------------------------------*/
@@ -318,6 +344,9 @@ typedef struct lbm_gfields {
   float* psi;          /* nc planes of (NX+2) x (NY+2), psi(-1,-1) at offset 0 */
   const int32_t* neighbor; /* (NX, NY, 9, 2) int32, lbm.py:50 layout */
   float* tau;          /* nc planes: BGK tau / TRT tau_sym / MRT diag[7] */
+  const uint8_t* sc_fluid; /* (NX+2) x (NY+2) halo grid, offset -1: 1 where the *
+                            * ShanChenForceC2 fluid_strategy is true (read when *
+                            * lbm_lattice.sc_adhesion is set)                   */
 } lbm_gfields;
 
 /* LBField.init_LBM, lbm.py:119-125 */
@@ -345,7 +374,9 @@ enum {
   LBM_BC_VELOCITY_NEEM = 4,  /* VelocityNEEM.apply, :154-166 */
   LBM_BC_PRESSURE_NEEM = 5,  /* PressureNEEM.apply, :215-226 */
   LBM_BC_VELOCITY_BB = 6,    /* VelocityBB.apply, :85-102 (additive, rule.ux/uy, direction) */
-  LBM_BC_OPEN_NEUMANN = 7    /* OpenNeumann.apply, :321-334 (copy unknowns, then density and velocity) */
+  LBM_BC_OPEN_NEUMANN = 7,   /* OpenNeumann.apply, :321-334 (copy unknowns, then density and velocity) */
+  LBM_BC_PRESSURE_ABC = 8,   /* PressureABC.apply, :190-205 (rule.rho, direction)               */
+  LBM_BC_VELOCITY_EQ = 9     /* VelocityEquilibrium.apply, :135-145 (rule.ux/uy, direction)     */
 };
 int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                    const int32_t* group, int32_t count, void* stream);
@@ -356,6 +387,21 @@ int lbm_g_sc_update_psi(const lbm_lattice* lat, const lbm_gfields* g, const int3
 int lbm_g_sc_periodic_psi(const lbm_lattice* lat, const lbm_gfields* g, int32_t which, void* stream);
 /* ShanChenForceC1/C2.cau_SCforce, ExternalForce.py:78-91, 124-145 */
 int lbm_g_sc_force(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32_t count, void* stream);
+
+/* psi boundaries other than the periodic pair, SCBoundaryCondition.py:9-123.
+ * The group lists halo-grid nodes (i, j) in [-1, NX] x [-1, NY]; the
+ * interior neighbour is x - c_direction.                                   */
+enum {
+  LBM_PSI_BC_NEUMANN = 1,       /* OpenNeumannPsi :19-33: psi_c(x) := psi_c(x - c_d)          */
+  LBM_PSI_BC_CONVECTIVE1 = 2,   /* OpenConvective1orderPsi :36-58                              */
+  LBM_PSI_BC_CONVECTIVE2 = 3,   /* OpenConvective2orderPsi :61-89                              */
+  LBM_PSI_BC_EXTRAPOLATION = 4, /* OpenExtrapolationPsi :92-107: 2 psi(x - c_d) - psi(x - 2c_d) */
+  LBM_PSI_BC_FIXED = 5          /* FixedPsi :110-123: psi_0(x) := value (component 0 only)     */
+};
+/* The convective kinds average the velocity of the group's interior
+ * neighbours (one block reduces the group, so groups are boundary lines). */
+int lbm_g_sc_psi_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, int32_t direction,
+                          float value, const int32_t* group, int32_t count, void* stream);
 
 /* ---- memory-bandwidth probe -----------------------------------------------*/
 /* Plain fp32 copy of n elements (the achievable-bandwidth reference used to

@@ -35,7 +35,7 @@ typedef struct cref_params {
 } cref_params;
 
 typedef struct cref_bc {
-  int32_t kind;  // 1 periodic, 2 inside, 3 bounceback, 4 velocity NEEM, 5 pressure NEEM,
+  int32_t kind;  // 1 periodic, 2 inside, 3 bounceback, 4 velocity NEEM, 5 pressure NEEM, 8 pressure ABC, 9 velocity equilibrium,
                  // 6 VelocityBB, 7 OpenNeumann
   int32_t direction;
   float ux, uy, rho;
@@ -272,10 +272,41 @@ void step(Ctx& C, const cref_bc* bcs, int nbc) {
             C.F(i, j, c)[k] += 6 * W[k] * rw * cv;
           }
         }
+      } else if (bc.kind == 9) {
+        // VelocityEquilibrium, BoundaryCondition.py:132-145
+        C.vel[2 * n] = bc.ux;
+        C.vel[2 * n + 1] = bc.uy;
+        for (int c = 0; c < nc; ++c) {
+          float* q = C.F(i, j, c);
+          float rw = 0.0f;
+          if (d == 2) rw = 1.0f / (1.0f + bc.uy) * (q[0] + q[1] + q[3] + 2 * (q[2] + q[5] + q[6]));
+          if (d == 4) rw = 1.0f / (1.0f - bc.uy) * (q[0] + q[1] + q[3] + 2 * (q[4] + q[7] + q[8]));
+          if (d == 1) rw = 1.0f / (1.0f + bc.ux) * (q[0] + q[2] + q[4] + 2 * (q[1] + q[5] + q[8]));
+          if (d == 3) rw = 1.0f / (1.0f - bc.ux) * (q[0] + q[2] + q[4] + 2 * (q[3] + q[6] + q[7]));
+          C.rho[n * nc + c] = rw;
+          feq(bc.ux, bc.uy, rw, q);
+        }
       } else {
         const int i2 = i - CX[d], j2 = j - CY[d];
         const size_t n2 = C.node(i2, j2);
-        if (bc.kind == 7) {
+        if (bc.kind == 8) {
+          // PressureABC, BoundaryCondition.py:177-205: u_w extrapolated from the
+          // velocity pass, unknowns := -f + 2 w rho_value (1 + 4.5 (c.u_w)^2 - 1.5 u_w^2)
+          const float uxw = 1.5f * C.vel[2 * n] - 0.5f * C.vel[2 * n2];
+          const float uyw = 1.5f * C.vel[2 * n + 1] - 0.5f * C.vel[2 * n2 + 1];
+          C.vel[2 * n] = uxw;
+          C.vel[2 * n + 1] = uyw;
+          C.trho[n] = bc.rho;
+          const float uw2 = uxw * uxw + uyw * uyw;
+          for (int c = 0; c < nc; ++c) {
+            float* q = C.F(i, j, c);
+            for (int u = 0; u < 3; ++u) {
+              const int k = UNK[d][u];
+              const float cu = (float)CX[k] * uxw + (float)CY[k] * uyw;
+              q[k] = -q[k] + 2 * W[k] * bc.rho * (1.0f + 4.5f * cu * cu - 1.5f * uw2);
+            }
+          }
+        } else if (bc.kind == 7) {
           // OpenNeumann, :327-334, then OpenBoundary.density / vel, :274-298
           float t = 0.0f, vx = 0.0f, vy = 0.0f;
           for (int c = 0; c < nc; ++c) {

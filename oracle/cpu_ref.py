@@ -27,7 +27,7 @@ class BC(C.Structure):
 
 
 KIND = {"periodic": 1, "inside": 2, "bounceback": 3, "velocity_neem": 4, "pressure_neem": 5, "velocity_bb": 6,
-        "open_neumann": 7}
+        "open_neumann": 7, "pressure_abc": 8, "velocity_eq": 9}
 
 
 def build(target: str = "all"):
@@ -107,9 +107,9 @@ def from_case(case, nthreads=0):
         if bc.kind == "fluid":
             continue
         d = {"kind": bc.kind, "nodes": bc.group.nodes, "direction": getattr(bc, "direction", 0)}
-        if bc.kind in ("velocity_neem", "velocity_bb"):
+        if bc.kind in ("velocity_neem", "velocity_bb", "velocity_eq"):
             d["u"] = bc.velocity_value
-        if bc.kind == "pressure_neem":
+        if bc.kind in ("pressure_neem", "pressure_abc"):
             d["rho"] = bc.rho_value
         bcs.append(d)
     kind = coll.KIND

@@ -512,7 +512,8 @@ def neem(vel1, rho1, vel2, rho2, fnb, dt):
 @dataclass
 class BC:
     """One entry of BoundaryEngine's ordered dict (BoundaryCondition.py:459-483)."""
-    kind: str            # fluid | periodic | inside | bounceback | velocity_neem | pressure_neem
+    kind: str            # fluid | periodic | inside | bounceback | velocity_neem | pressure_neem |
+                         # velocity_bb | open_neumann | velocity_eq | pressure_abc
     group: np.ndarray
     direction: Optional[int] = None
     value: object = None  # velocity (2,) or rho scalar
@@ -561,6 +562,28 @@ class BC:
                 full = neem(lat.vel[I, J], lat.rho[I, J, c], lat.vel[I2, J2], lat.rho[I2, J2, c],
                             lat.f[I2, J2, c, :], dt)
                 lat.f[I[:, None], J[:, None], c, unk[None, :]] = full[:, unk]
+        elif self.kind == "velocity_eq":     # VelocityEquilibrium :132-145: f = feq(u_w, rho_w), all 9
+            d = self.direction
+            uw = np.asarray(self.value, dt)
+            lat.vel[I, J] = uw
+            for c in range(lat.nc):
+                lat.rho[I, J, c] = rhow(d, uw, lat.f[I, J, c, :])
+            for c in range(lat.nc):
+                lat.f[I, J, c, :] = f_eq(lat.vel[I, J], lat.rho[I, J, c], dt)
+        elif self.kind == "pressure_abc":    # PressureABC :177-205 (anti-bounce-back on the unknowns)
+            d = self.direction
+            I2, J2 = I - C[d, 0], J - C[d, 1]
+            uw = 1.5 * lat.vel[I, J] - 0.5 * lat.vel[I2, J2]
+            lat.vel[I, J] = uw
+            lat.total_rho[I, J] = dt.type(self.value)
+            w = weights(dt)
+            cf = _cf(dt)
+            cu = uw @ cf.T                                    # (m, 9)
+            uw2 = (uw * uw).sum(axis=1)[:, None]
+            unk = np.array(UNKNOWN[d])
+            for c in range(lat.nc):
+                ftemp = -lat.f[I, J, c, :] + 2 * w[None, :] * dt.type(self.value) * (1.0 + 4.5 * cu * cu - 1.5 * uw2)
+                lat.f[I[:, None], J[:, None], c, unk[None, :]] = ftemp[:, unk]
         elif self.kind == "velocity_bb":     # :85-102 (additive on the unknowns)
             d = self.direction
             uw = np.asarray(self.value, dt)
@@ -614,22 +637,64 @@ def sc_psi(rho, rho0):
 
 
 @dataclass
+class EOS:
+    """Pseudopotential.get_psi with a cubic equation of state, EOS.py:5-95:
+    psi = sqrt(|6/G (p_EOS(rho, T) - rho/3)|).  kind: "vdw" (:50-58), "rk"
+    (:61-69), "pr" (:72-83, omega, T_cr), "cs" (:86-95)."""
+    kind: str
+    a: float
+    b: float
+    R: float = 1.0
+    G: float = 1.0
+    omega: float = 0.0
+    T_cr: float = 1.0
+
+    def pressure(self, rho, T):
+        dt = rho.dtype.type
+        a, b, R = dt(self.a), dt(self.b), dt(self.R)
+        if self.kind == "vdw":
+            return rho * R * T / (1.0 - b * rho) - a * rho ** 2.0
+        if self.kind == "rk":
+            return rho * R * T / (1.0 - b * rho) - a * rho ** 2.0 / np.sqrt(T) / (1.0 + b * rho)
+        if self.kind == "pr":
+            w = dt(self.omega)
+            alpha = (1 + (0.37464 + 1.54226 * w - 0.269922 * w ** 2) * (1 - np.sqrt(T / dt(self.T_cr)))) ** 2
+            return rho * R * T / (1.0 - b * rho) - a * alpha * rho ** 2 / (1.0 + 2.0 * b * rho - (b * rho) ** 2)
+        if self.kind == "cs":
+            x = b * rho / 4.0
+            return rho * R * T * (1.0 + x + x ** 2 - x ** 3) / (1 - x) ** 3 - a * rho ** 2
+        raise ValueError(self.kind)
+
+    def get_psi(self, rho, T):
+        dt = rho.dtype.type
+        return np.sqrt(np.abs(dt(6.0) / dt(self.G) * (self.pressure(rho, T) - rho / 3.0)))
+
+
+@dataclass
 class ShanChen:
     kind: str                        # "c1" | "c2"
     group: np.ndarray
     g_coh: object                    # scalar (c1) or (nc,nc) matrix (c2)
     rho0: Sequence[float] = (1.0,)   # SC_psi rho0 per psi object (c1: one; c2: one per component)
     g_adh: Optional[Sequence[float]] = None
+    fluid: Optional[np.ndarray] = None   # c2: fluid_strategy(x, y) over the halo grid (NX+2, NY+2),
+                                         # offset -1; None: fluid everywhere
+    eos: Optional[Sequence[Optional[EOS]]] = None   # per psi object: a cubic EOS in place of SC_psi
+
+    def _psi(self, q, rho, T):
+        e = self.eos[q] if self.eos is not None else None
+        return sc_psi(rho, self.rho0[q]) if e is None else e.get_psi(rho, T)
 
     def update_psi(self, lat: Lattice):
-        """C1 ExternalForce.py:70-75; C2 :114-120."""
+        """C1 ExternalForce.py:42-47; C2 :86-92 (psi objects: EOS.py)."""
         I, J = group_ij(self.group)
+        T = lat.T[I, J]
         if self.kind == "c1":
             for c in range(lat.nc):
-                lat.psi[I + 1, J + 1, c] = sc_psi(lat.rho[I, J, c], self.rho0[0])
+                lat.psi[I + 1, J + 1, c] = self._psi(0, lat.rho[I, J, c], T)
         else:
-            lat.psi[I + 1, J + 1, 0] = sc_psi(lat.rho[I, J, 0], self.rho0[0])
-            lat.psi[I + 1, J + 1, 1] = sc_psi(lat.rho[I, J, 1], self.rho0[1])
+            lat.psi[I + 1, J + 1, 0] = self._psi(0, lat.rho[I, J, 0], T)
+            lat.psi[I + 1, J + 1, 1] = self._psi(1, lat.rho[I, J, 1], T)
 
     def cau_scforce(self, lat: Lattice):
         """C1 ExternalForce.py:78-91; C2 :124-145 (fluid_strategy == everywhere fluid)."""
@@ -646,15 +711,64 @@ class ShanChen:
             F = F * (-dt.type(self.g_coh) * lat.psi[I + 1, J + 1, 0])[:, None]
             lat.SCforce[I, J, 0, :] = lat.SCforce[I, J, 0, :] + F
         else:
+            # :96-117: a neighbour for which fluid_strategy is false contributes
+            # the adhesion w_k c_k gadh[c1] (without psi(x + c_k): the factor is
+            # commented out at :114) in place of the cohesion terms
             G = np.asarray(self.g_coh, dt)
+            gadh = np.zeros(lat.nc, dt) if self.g_adh is None else np.asarray(self.g_adh, dt)
             for c1 in range(lat.nc):
                 Fc = np.zeros((len(I), 2), dt)
                 Fa = np.zeros((len(I), 2), dt)
                 for k in range(1, 9):
+                    I2, J2 = I + 1 + C[k, 0], J + 1 + C[k, 1]
+                    fl = np.ones(len(I), bool) if self.fluid is None else self.fluid[I2, J2]
                     for c2 in range(lat.nc):
                         if c2 != c1:
-                            Fc = Fc + wc[k][None, :] * lat.psi[I + 1 + C[k, 0], J + 1 + C[k, 1], c2][:, None] * G[c1, c2]
+                            add = wc[k][None, :] * lat.psi[I2, J2, c2][:, None] * G[c1, c2]
+                            Fc = np.where(fl[:, None], Fc + add, Fc)
+                    Fa = np.where(fl[:, None], Fa, Fa + wc[k][None, :] * gadh[c1])
                 lat.SCforce[I, J, c1, :] = (-lat.psi[I + 1, J + 1, c1])[:, None] * (Fc + Fa)
+
+
+@dataclass
+class PsiBC:
+    """One entry of BoundaryEngineSC (SCBoundaryCondition.py:149-160) other than
+    the periodic pair.  nodes: (m, 2) halo-grid coordinates (i, j) in
+    [-1, NX] x [-1, NY]; direction d: the interior neighbour is x - c_d.
+    kind: "neumann" (OpenNeumannPsi :19-33), "convective1" (:36-58),
+    "convective2" (:61-89), "extrapolation" (:92-107), "fixed" (FixedPsi
+    :110-123, component 0 only)."""
+    kind: str
+    nodes: np.ndarray
+    direction: Optional[int] = None
+    value: float = 0.0
+
+    def apply(self, lat: Lattice):
+        p = lat.psi
+        I = np.asarray(self.nodes, np.int64)[:, 0] + 1
+        J = np.asarray(self.nodes, np.int64)[:, 1] + 1
+        dt = p.dtype.type
+        if self.kind == "fixed":
+            p[I, J, 0] = dt(self.value)
+            return
+        d = self.direction
+        I2, J2 = I - C[d, 0], J - C[d, 1]
+        if self.kind == "neumann":
+            p[I, J, :] = p[I2, J2, :]
+        elif self.kind == "extrapolation":
+            I3, J3 = I2 - C[d, 0], J2 - C[d, 1]
+            p[I, J, :] = 2 * p[I2, J2, :] - p[I3, J3, :]
+        elif self.kind in ("convective1", "convective2"):
+            # U = mean velocity of the interior neighbours (lattice indices), lambda = |U|
+            U = lat.vel[I2 - 1, J2 - 1].sum(axis=0) / dt(len(I))
+            lam = np.sqrt(U @ U)
+            if self.kind == "convective1":
+                p[I, J, :] = (p[I, J, :] + lam * p[I2, J2, :]) / (1.0 + lam)
+            else:
+                I3, J3 = I - 2 * C[d, 0], J - 2 * C[d, 1]
+                p[I, J, :] = (p[I, J, :] + 2 * lam * p[I2, J2, :] - 0.5 * lam * p[I3, J3, :]) / (1.0 + 1.5 * lam)
+        else:
+            raise ValueError(self.kind)
 
 
 def periodic_psi(lat: Lattice):
@@ -679,6 +793,7 @@ class Solver:
     power_law: bool = False
     sc: Optional[ShanChen] = None
     advance_time: bool = False
+    psi_bcs: Optional[List] = None   # BoundaryEngineSC list: "periodic" (LR then BT) and PsiBC entries
 
     def step(self):
         lat = self.lat
@@ -688,7 +803,11 @@ class Solver:
         pressure(lat, self.group, self.sc)
         if self.sc is not None:
             self.sc.update_psi(lat)
-            periodic_psi(lat)
+            for bc in (self.psi_bcs if self.psi_bcs is not None else ["periodic"]):
+                if bc == "periodic":
+                    periodic_psi(lat)
+                else:
+                    bc.apply(lat)
             self.sc.cau_scforce(lat)
         force_density(lat, self.group)
         velocity(lat, self.group)

@@ -66,8 +66,21 @@ struct Lat {
   float visc_m, visc_nm1, tau_min, tau_max, gx, gy;
   float g_coh[LBM_MAX_NC * LBM_MAX_NC];
   float psi_rho0[LBM_MAX_NC];
+  int32_t psi_kind[LBM_MAX_NC];
+  float eos[LBM_MAX_NC][6];
+  float g_adh[LBM_MAX_NC];
+  int32_t sc_adhesion;         // some node carries LBM_CODE_SC_SOLID
+  const float* temperature;    // lb_field.T (cubic-EOS psi)
   lbm_neem_rule neem[LBM_MAX_NEEM];
 };
+
+// psi of component c from the node's density (ShanChenForceC1/C2.update_psi,
+// ExternalForce.py:42-47, 86-92, with the psi objects of EOS.py)
+template <typename R>
+__device__ __forceinline__ R node_psi(const Lat& a, int c, R rho, uint32_t n) {
+  if (a.psi_kind[c] == LBM_PSI_SC) return sc_psi<R>(rho, (R)a.psi_rho0[c]);
+  return psi_of<R>(a.psi_kind[c], rho, (R)a.temperature[n], (R)a.psi_rho0[c], a.eos[c]);
+}
 
 // Per-launch switches.
 struct Launch {
@@ -177,8 +190,50 @@ __device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, con
     case 3: di = -1; break;
     case 4: dj = -1; break;
   }
+  if (r.kind == LBM_RULE_VELOCITY_EQ) {
+    // BoundaryCondition.py:135-145: rho_w from the node's populations (Zou-He
+    // form), then all nine := feq(u_w, rho_w)
+    const R uxw = r.ux, uyw = r.uy;
+    const R uvw = uxw * uxw + uyw * uyw;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const R rw = neem_rhow<R>(d, uxw, uyw, f[c]);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) f[c][k] = feq_k<R>(k, uxw, uyw, rw, uvw);
+      if (mac_override) mac_override[(size_t)c * ps + n] = rw;
+    }
+    if (mac_override) {
+      mac_override[(size_t)(NC + 1) * ps + n] = uxw;
+      mac_override[(size_t)(NC + 2) * ps + n] = uyw;
+    }
+    return;
+  }
   const typename V::Node y = v.back(x, di, dj);
   const uint32_t bm_y = v.code_of(y) & LBM_CODE_BOUNCE_MASK;
+  if (r.kind == LBM_RULE_PRESSURE_ABC) {
+    // BoundaryCondition.py:190-205: u_w = 1.5 u(x) - 0.5 u(x - c_d) from the
+    // velocity pass of the step, then on the unknowns (written by an earlier
+    // bounce-back) f_k := -f_k + 2 w_k rho_value (1 + 4.5 (c_k . u_w)^2 - 1.5 u_w^2)
+    const R uxw = R(1.5) * v.macro(x, NC + 1) - R(0.5) * v.macro(y, NC + 1);
+    const R uyw = R(1.5) * v.macro(x, NC + 2) - R(0.5) * v.macro(y, NC + 2);
+    const R uw2 = uxw * uxw + uyw * uyw;
+    const R rw = r.rho;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const R cu = (R)cx(k) * uxw + (R)cy(k) * uyw;
+        const R upd = -f[c][k] + R(2) * wgt<R>(k) * rw * (R(1) + R(4.5) * cu * cu - R(1.5) * uw2);
+        f[c][k] = neem_unknown(d, k) ? upd : f[c][k];
+      }
+    }
+    if (mac_override) {
+      mac_override[(size_t)NC * ps + n] = rw;
+      mac_override[(size_t)(NC + 1) * ps + n] = uxw;
+      mac_override[(size_t)(NC + 2) * ps + n] = uyw;
+    }
+    return;
+  }
   if (r.kind == LBM_RULE_OPEN_NEUMANN) {
     // BoundaryCondition.py:327-334: unknowns := f(x - c_d), then density and
     // velocity of the node (OpenBoundary.density / vel, :274-298)
@@ -284,10 +339,16 @@ struct PsiPlanes {
     const Nbr nb(i, j, a.nx, a.ny);
     return ((const R*)a.psi[set])[(size_t)c * a.ps + (k == 0 ? nb.n : nb.src(opp(k)))];   // x - c_opp(k) = x + c_k
   }
+  // x + c_k is not fluid for the Shan-Chen force (fluid_strategy false)
+  __device__ __forceinline__ bool solid(int k) const {
+    const Nbr nb(i, j, a.nx, a.ny);
+    return (a.code[nb.src(opp(k))] & LBM_CODE_SC_SOLID) != 0;
+  }
 };
 template <typename R>
 struct PsiNone {
   __device__ __forceinline__ R at(int, int) const { return R(0); }
+  __device__ __forceinline__ bool solid(int) const { return false; }
 };
 
 // S1..S6 of one step at node n = (i, j).  state_in is the per-node relaxation
@@ -325,7 +386,7 @@ __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t 
       const R s = -(R)a.g_coh[0] * psi.at(0, 0);
       o.scx[0] = Fx * s;
       o.scy[0] = Fy * s;
-    } else {
+    } else if (!a.sc_adhesion) {
 #pragma unroll
       for (int c1 = 0; c1 < NC; ++c1) {
         R Fx = R(0), Fy = R(0);
@@ -343,6 +404,32 @@ __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t 
         const R s = -psi.at(c1, 0);
         o.scx[c1] = s * Fx;
         o.scy[c1] = s * Fy;
+      }
+    } else {
+      // ExternalForce.py:105-116: a neighbour where fluid_strategy is false
+      // adds w_k c_k gadh[c1] (without psi(x + c_k), commented out at :114)
+      // in place of its cohesion terms
+#pragma unroll
+      for (int c1 = 0; c1 < NC; ++c1) {
+        R Fx = R(0), Fy = R(0), Ax = R(0), Ay = R(0);
+#pragma unroll
+        for (int k = 1; k < 9; ++k) {
+          const bool sol = psi.solid(k);
+          const R ga = (R)a.g_adh[c1];
+          Ax = sol ? Ax + (wgt<R>(k) * (R)cx(k)) * ga : Ax;
+          Ay = sol ? Ay + (wgt<R>(k) * (R)cy(k)) * ga : Ay;
+#pragma unroll
+          for (int c2 = 0; c2 < NC; ++c2) {
+            if (c2 == c1) continue;
+            const R v = psi.at(c2, k);
+            const R G = a.g_coh[c1 * LBM_MAX_NC + c2];
+            Fx = sol ? Fx : Fx + (wgt<R>(k) * (R)cx(k)) * v * G;
+            Fy = sol ? Fy : Fy + (wgt<R>(k) * (R)cy(k)) * v * G;
+          }
+        }
+        const R s = -psi.at(c1, 0);
+        o.scx[c1] = s * (Fx + Ax);
+        o.scy[c1] = s * (Fy + Ay);
       }
     }
   }
@@ -504,7 +591,7 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
       R s = R(0);
 #pragma unroll
       for (int k = 0; k < 9; ++k) s += f[c][k];
-      psi_out[(size_t)c * ps + n] = sc_psi<R>(s, (R)a.psi_rho0[c]);
+      psi_out[(size_t)c * ps + n] = node_psi<R>(a, c, s, n);
     }
     return;
   }
@@ -929,9 +1016,12 @@ using ScShape = TileShape<1, kScCols, NT>;
 // psi of component c at x + c_k from the LDS tile (slot q of x)
 template <typename R, class Shape>
 struct PsiTile {
-  const R* L;   // psi of component c at L[c * Shape::PN + slot]
+  const R* L;         // psi of component c at L[c * Shape::PN + slot]
+  const uint8_t* S;   // 1 where the slot's node is not fluid for the force (LBM_CODE_SC_SOLID)
   uint32_t q;
   __device__ __forceinline__ R at(int c, int k) const { return L[c * Shape::PN + q + Shape::back(k)]; }
+  // x + c_k = slot q - back(opp(k)) = q + back(k)
+  __device__ __forceinline__ bool solid(int k) const { return S[q + Shape::back(k)] != 0; }
 };
 
 // Eight waves per SIMD where the registers allow it without spilling (one
@@ -945,6 +1035,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
   __shared__ R psiL[NC * Shape::PN];
+  __shared__ uint8_t solL[Shape::PN];   // adhesion flags of the ring-extended tile
   const uint32_t nt = q.tiles_i * q.tiles_j;
   uint32_t b = blockIdx.x;
   if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
@@ -965,6 +1056,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
     ring_coords<1>(rr, cc, i0, j0, nx, ny, gi, gj);
     const uint32_t n = gi * ny + gj;
     code = a.code[n];
+    if (a.sc_adhesion) solL[qs] = (uint8_t)(code >> 15);
     // the pulls issue without waiting for the code; the populations the code
     // bounces are loaded again from the node itself afterwards
     if (local) {
@@ -999,7 +1091,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
         R s = R(0);
 #pragma unroll
         for (int k = 0; k < 9; ++k) s += f[c][k];
-        psiL[c * Shape::PN + qs] = sc_psi<R>(s, (R)a.psi_rho0[c]);
+        psiL[c * Shape::PN + qs] = node_psi<R>(a, c, s, n);
       }
     } else {
 #pragma unroll
@@ -1024,7 +1116,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
 #pragma unroll
   for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode != LBM_TAU_CONST ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
   StepOut<NC, R> o;
-  step_math<COLL, NC, R>(a, PsiTile<R, Shape>{psiL, qs}, n, gi, gj, f, st_in, o);
+  step_math<COLL, NC, R>(a, PsiTile<R, Shape>{psiL, solL, qs}, n, gi, gj, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[pset];
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
 #pragma unroll
@@ -1093,7 +1185,7 @@ __global__ __launch_bounds__(block2_threads(ring_of(M), W)) __attribute__((amdgp
 
 // Tile-grid shifts that keep every rule node's interior neighbour in its own
 // tile (two-step mode 3).  For each active rule node that reads x - c_d
-// (VelocityNEEM, PressureNEEM, OpenNeumann), the one row shift (d = 1, 3;
+// (VelocityNEEM, PressureNEEM, OpenNeumann, PressureABC), the one row shift (d = 1, 3;
 // tiles of ti rows) or column shift (d = 2, 4; w columns) that would put x
 // on the tile edge facing its neighbour is marked forbidden in
 // masks[3l] (rows) and masks[3l + 1..2] (columns, 64 bits).
@@ -1106,7 +1198,7 @@ __global__ void k_rule_shifts(const Lat* __restrict__ items, uint32_t ti, uint32
     const uint32_t e = (code & LBM_CODE_NEEM_MASK) >> LBM_CODE_NEEM_SHIFT;
     if (!e || !(code & LBM_CODE_ACTIVE)) continue;
     const int kind = a.neem[e - 1].kind, d = a.neem[e - 1].direction;
-    if (kind == LBM_RULE_VELOCITY_BB) continue;   // reads its own node only
+    if (kind == LBM_RULE_VELOCITY_BB || kind == LBM_RULE_VELOCITY_EQ) continue;   // read their own node only
     const uint32_t i = n / a.ny, j = n - i * a.ny;
     if (d == 1 || d == 3) {
       const uint32_t o = d == 1 ? (ti - i % ti) % ti : (2 * ti - 1 - i % ti) % ti;   // x first / last row
@@ -1267,7 +1359,7 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
   if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
   for (int r = 0; r < lat->n_neem; ++r) {
     if (neem[r].direction < 1 || neem[r].direction > 4) return fail("NEEM rule %d: direction %d", r, neem[r].direction);
-    if (neem[r].kind < LBM_NEEM_VELOCITY || neem[r].kind > LBM_RULE_OPEN_NEUMANN)
+    if (neem[r].kind < LBM_NEEM_VELOCITY || neem[r].kind > LBM_RULE_VELOCITY_EQ)
       return fail("node rule %d: kind %d", r, neem[r].kind);
   }
   if (!fld->f[0] || !fld->f[1] || !fld->macro[0] || !fld->macro[1] || !fld->code)
@@ -1277,6 +1369,13 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
   if (lat->tau_mode != LBM_TAU_CONST && !fld->tau) return fail("tau_mode %d needs the tau buffer", lat->tau_mode);
   if (lat->shan_chen && (!fld->psi[0] || !fld->psi[1] || !fld->sc_force))
     return fail("Shan-Chen needs psi and sc_force buffers");
+  if (lat->shan_chen) {
+    for (int c = 0; c < lat->nc; ++c) {
+      if (lat->psi_kind[c] < LBM_PSI_SC || lat->psi_kind[c] > LBM_PSI_CS) return fail("bad psi kind %d", lat->psi_kind[c]);
+      if (lat->psi_kind[c] != LBM_PSI_SC && !fld->temperature) return fail("a cubic-EOS psi needs the temperature plane");
+    }
+    if (lat->sc_adhesion && lat->shan_chen != LBM_SC_C2) return fail("adhesion needs ShanChenForceC2");
+  }
   return LBM_OK;
 }
 
@@ -1313,7 +1412,12 @@ Lat make_lat(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
     a.s_shear[c] = lat->s_shear[c];
     a.s_bulk[c] = lat->s_bulk[c];
     a.psi_rho0[c] = lat->psi_rho0[c];
+    a.psi_kind[c] = lat->psi_kind[c];
+    a.g_adh[c] = lat->g_adh[c];
+    for (int q = 0; q < 6; ++q) a.eos[c][q] = lat->eos[c][q];
   }
+  a.sc_adhesion = lat->sc_adhesion;
+  a.temperature = fld->temperature;
   for (int q = 0; q < LBM_MAX_NC * LBM_MAX_NC; ++q) a.g_coh[q] = lat->g_coh[q];
   a.visc_m = lat->visc_m;
   a.visc_nm1 = lat->visc_nm1;

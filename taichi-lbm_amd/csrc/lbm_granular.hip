@@ -27,8 +27,14 @@ struct G {
   float s_bulk[LBM_MAX_NC];
   float g_coh[LBM_MAX_NC * LBM_MAX_NC];
   float psi_rho0[LBM_MAX_NC];
+  int32_t psi_kind[LBM_MAX_NC];
+  float eos[LBM_MAX_NC][6];
+  float g_adh[LBM_MAX_NC];
+  int32_t sc_adhesion;
   lbm_neem_rule rule;
   int32_t kind;
+  int32_t direction;   // psi boundaries
+  float value;         // FixedPsi
 };
 
 __device__ __forceinline__ bool entry(const G& a, uint32_t& i, uint32_t& j, uint32_t& n) {
@@ -292,6 +298,37 @@ __global__ void k_boundary(const G a) {
           *fp = *fp + 6.0f * wgt(k) * rhow * cv;
         }
     }
+  } else if (a.kind == LBM_BC_VELOCITY_EQ) {
+    // VelocityEquilibrium, BoundaryCondition.py:135-145
+    const int d = a.rule.direction;
+    const float uxw = a.rule.ux, uyw = a.rule.uy, uvw = uxw * uxw + uyw * uyw;
+    a.g.vel[n] = uxw;
+    a.g.vel[ps + n] = uyw;
+    for (uint32_t c = 0; c < a.nc; ++c) {
+      float f[9];
+      load9(a, a.g.f, c, n, f);
+      const float rw = neem_rhow(d, uxw, uyw, f);
+      a.g.rho[c * ps + n] = rw;
+      for (int k = 0; k < 9; ++k) P(a.g.f, c * 9 + k, a.ps)[n] = feq_k(k, uxw, uyw, rw, uvw);
+    }
+  } else if (a.kind == LBM_BC_PRESSURE_ABC) {
+    // PressureABC, BoundaryCondition.py:190-205 (the interior neighbour x - c_d
+    // is outside the group: its velocity is not written here)
+    const int d = a.rule.direction;
+    const uint32_t n2 = (uint32_t)((int)i - cx(d)) * ny + (uint32_t)((int)j - cy(d));
+    const float uxw = 1.5f * a.g.vel[n] - 0.5f * a.g.vel[n2];
+    const float uyw = 1.5f * a.g.vel[ps + n] - 0.5f * a.g.vel[ps + n2];
+    a.g.vel[n] = uxw;
+    a.g.vel[ps + n] = uyw;
+    a.g.total_rho[n] = a.rule.rho;
+    const float uw2 = uxw * uxw + uyw * uyw;
+    for (uint32_t c = 0; c < a.nc; ++c)
+      for (int k = 0; k < 9; ++k)
+        if (neem_unknown(d, k)) {
+          const float cu = (float)cx(k) * uxw + (float)cy(k) * uyw;
+          float* fp = P(a.g.f, c * 9 + k, a.ps) + n;
+          *fp = -*fp + 2.0f * wgt(k) * a.rule.rho * (1.0f + 4.5f * cu * cu - 1.5f * uw2);
+        }
   } else if (a.kind == LBM_BC_OPEN_NEUMANN) {
     // copy the unknowns from x - c_d, then OpenBoundary.density / vel at x (:274-298)
     const int d = a.rule.direction;
@@ -363,8 +400,63 @@ __global__ void k_boundary(const G a) {
 __global__ void k_update_psi(const G a) {
   uint32_t i, j, n;
   if (!entry(a, i, j, n)) return;
-  for (uint32_t c = 0; c < a.nc; ++c)
-    a.g.psi[(size_t)c * a.hs + hidx(a, (int)i, (int)j)] = sc_psi(a.g.rho[(size_t)c * a.ps + n], a.psi_rho0[c]);
+  for (uint32_t c = 0; c < a.nc; ++c) {
+    const float T = a.psi_kind[c] != LBM_PSI_SC ? a.g.temperature[n] : 0.0f;
+    a.g.psi[(size_t)c * a.hs + hidx(a, (int)i, (int)j)] =
+        psi_of<float>(a.psi_kind[c], a.g.rho[(size_t)c * a.ps + n], T, a.psi_rho0[c], a.eos[c]);
+  }
+}
+
+// psi boundaries, SCBoundaryCondition.py:19-123, on halo-grid nodes (i, j >=
+// -1).  One block: the convective kinds first average the velocity of the
+// group's interior neighbours (OpenConvective1orderPsi :41-49), then every
+// node is updated from psi at x - c_d (and x - 2 c_d).
+__global__ __launch_bounds__(256) void k_psi_boundary(const G a) {
+  __shared__ float red[2][256];
+  const int d = a.direction;
+  float lam = 0.0f;
+  if (a.kind == LBM_PSI_BC_CONVECTIVE1 || a.kind == LBM_PSI_BC_CONVECTIVE2) {
+    float ux = 0.0f, uy = 0.0f;
+    for (uint32_t m = threadIdx.x; m < a.count; m += 256) {
+      const int2 ij = a.grp[m];
+      const uint32_t n2 = (uint32_t)(ij.x - cx(d)) * a.ny + (uint32_t)(ij.y - cy(d));
+      ux += a.g.vel[n2];
+      uy += a.g.vel[(size_t)a.ps + n2];
+    }
+    red[0][threadIdx.x] = ux;
+    red[1][threadIdx.x] = uy;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + h];
+        red[1][threadIdx.x] += red[1][threadIdx.x + h];
+      }
+      __syncthreads();
+    }
+    const float Ux = red[0][0] / (float)a.count, Uy = red[1][0] / (float)a.count;
+    lam = 1.0f * sqrtf(Ux * Ux + Uy * Uy);
+  }
+  for (uint32_t m = threadIdx.x; m < a.count; m += 256) {
+    const int2 ij = a.grp[m];
+    const int i = ij.x, j = ij.y;
+    const uint32_t x = hidx(a, i, j), x2 = hidx(a, i - cx(d), j - cy(d));
+    if (a.kind == LBM_PSI_BC_FIXED) {
+      a.g.psi[x] = a.value;
+      continue;
+    }
+    for (uint32_t c = 0; c < a.nc; ++c) {
+      float* p = a.g.psi + (size_t)c * a.hs;
+      if (a.kind == LBM_PSI_BC_NEUMANN) {
+        p[x] = p[x2];
+      } else if (a.kind == LBM_PSI_BC_EXTRAPOLATION) {
+        p[x] = 2.0f * p[x2] - p[hidx(a, i - 2 * cx(d), j - 2 * cy(d))];
+      } else if (a.kind == LBM_PSI_BC_CONVECTIVE1) {
+        p[x] = (p[x] + lam * p[x2]) / (1.0f + lam);
+      } else {
+        p[x] = (p[x] + 2.0f * lam * p[x2] - 0.5f * lam * p[hidx(a, i - 2 * cx(d), j - 2 * cy(d))]) / (1.0f + 1.5f * lam);
+      }
+    }
+  }
 }
 
 // PeriodicPsi_LR (x == 0) then PeriodicPsi_BT (x == 1), SCBoundaryCondition.py:126-144
@@ -404,21 +496,29 @@ __global__ void k_sc_force(const G a) {
     a.g.sc_force[n] = a.g.sc_force[n] + Fx * s;
     a.g.sc_force[ps + n] = a.g.sc_force[ps + n] + Fy * s;
   } else {
+    // a neighbour where fluid_strategy is false adds w_k c_k gadh[c1] (without
+    // psi(x + c_k), commented out at :114) in place of the cohesion terms
     for (uint32_t c1 = 0; c1 < a.nc; ++c1) {
-      float Fx = 0.0f, Fy = 0.0f;
+      float Fx = 0.0f, Fy = 0.0f, Ax = 0.0f, Ay = 0.0f;
 #pragma unroll
       for (int k = 1; k < 9; ++k) {
+        const uint32_t h = hidx(a, (int)i + cx(k), (int)j + cy(k));
+        if (a.sc_adhesion && !a.g.sc_fluid[h]) {
+          Ax += (wgt(k) * (float)cx(k)) * a.g_adh[c1];
+          Ay += (wgt(k) * (float)cy(k)) * a.g_adh[c1];
+          continue;
+        }
         for (uint32_t c2 = 0; c2 < a.nc; ++c2) {
           if (c2 == c1) continue;
-          const float p = a.g.psi[(size_t)c2 * a.hs + hidx(a, (int)i + cx(k), (int)j + cy(k))];
+          const float p = a.g.psi[(size_t)c2 * a.hs + h];
           const float Gc = a.g_coh[c1 * LBM_MAX_NC + c2];
           Fx += (wgt(k) * (float)cx(k)) * p * Gc;
           Fy += (wgt(k) * (float)cy(k)) * p * Gc;
         }
       }
       const float s = -a.g.psi[(size_t)c1 * a.hs + hidx(a, (int)i, (int)j)];
-      a.g.sc_force[(2 * c1) * ps + n] = s * Fx;
-      a.g.sc_force[(2 * c1 + 1) * ps + n] = s * Fy;
+      a.g.sc_force[(2 * c1) * ps + n] = s * (Fx + Ax);
+      a.g.sc_force[(2 * c1 + 1) * ps + n] = s * (Fy + Ay);
     }
   }
 }
@@ -454,7 +554,11 @@ G make(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32
     a.magic[c] = lat->magic[c];
     a.s_bulk[c] = lat->s_bulk[c];
     a.psi_rho0[c] = lat->psi_rho0[c];
+    a.psi_kind[c] = lat->psi_kind[c];
+    a.g_adh[c] = lat->g_adh[c];
+    for (int q = 0; q < 6; ++q) a.eos[c][q] = lat->eos[c][q];
   }
+  a.sc_adhesion = lat->sc_adhesion;
   for (int q = 0; q < LBM_MAX_NC * LBM_MAX_NC; ++q) a.g_coh[q] = lat->g_coh[q];
   return a;
 }
@@ -543,7 +647,7 @@ extern "C" int lbm_g_collide(const lbm_lattice* lat, const lbm_gfields* g, const
 extern "C" int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                               const int32_t* group, int32_t count, void* stream) {
   GRANULAR_PROLOGUE
-  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_OPEN_NEUMANN) return fail("bad boundary kind %d", kind);
+  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_VELOCITY_EQ) return fail("bad boundary kind %d", kind);
   if ((kind == LBM_BC_INSIDE || kind == LBM_BC_BOUNCEBACK) && !g->neighbor)
     return fail("boundary kind %d needs the neighbour table", kind);
   if (kind >= LBM_BC_VELOCITY_NEEM) {
@@ -555,10 +659,30 @@ extern "C" int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int3
   return run_group(k_boundary, a, s);
 }
 
+extern "C" int lbm_g_sc_psi_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, int32_t direction,
+                                     float value, const int32_t* group, int32_t count, void* stream) {
+  GRANULAR_PROLOGUE
+  if (!g->psi) return fail("psi boundary needs psi");
+  if (kind < LBM_PSI_BC_NEUMANN || kind > LBM_PSI_BC_FIXED) return fail("bad psi boundary kind %d", kind);
+  if (kind != LBM_PSI_BC_FIXED && (direction < 1 || direction > 4)) return fail("psi boundary direction %d", direction);
+  if ((kind == LBM_PSI_BC_CONVECTIVE1 || kind == LBM_PSI_BC_CONVECTIVE2) && !g->vel)
+    return fail("convective psi boundary needs the velocity field");
+  a.kind = kind;
+  a.direction = direction;
+  a.value = value;
+  if (a.count == 0) return LBM_OK;
+  hipLaunchKernelGGL(k_psi_boundary, dim3(1), dim3(256), 0, s, a);
+  return hip_check(hipGetLastError(), "psi boundary launch");
+}
+
 extern "C" int lbm_g_sc_update_psi(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group,
                                    int32_t count, void* stream) {
   GRANULAR_PROLOGUE
   if (!g->psi) return fail("update_psi needs psi");
+  for (int c = 0; c < lat->nc; ++c) {
+    if (lat->psi_kind[c] < LBM_PSI_SC || lat->psi_kind[c] > LBM_PSI_CS) return fail("bad psi kind %d", lat->psi_kind[c]);
+    if (lat->psi_kind[c] != LBM_PSI_SC && !g->temperature) return fail("a cubic-EOS psi needs the temperature field");
+  }
   return run_group(k_update_psi, a, s);
 }
 
@@ -583,6 +707,7 @@ extern "C" int lbm_g_sc_force(const lbm_lattice* lat, const lbm_gfields* g, cons
   GRANULAR_PROLOGUE
   if (!g->psi || !g->sc_force) return fail("cau_SCforce needs psi and SCforce");
   if (a.sc == LBM_SC_NONE) return fail("cau_SCforce without a Shan-Chen variant");
+  if (a.sc_adhesion && (a.sc != LBM_SC_C2 || !g->sc_fluid)) return fail("adhesion needs ShanChenForceC2 and sc_fluid");
   if ((rc = fill(g->sc_force, 2 * a.nc, a.nx * a.ny, a.ps, 0.0f, s))) return rc;
   return run_group(k_sc_force, a, s);
 }

@@ -211,6 +211,31 @@ __device__ __forceinline__ void collide(const R (&f)[9], R (&out)[9], R rho, R u
 template <typename R>
 __device__ __forceinline__ R sc_psi(R rho, R rho0) { return rho0 * (R(1) - exp(-rho / rho0)); }
 
+// Pseudopotential.get_psi with a cubic equation of state, EOS.py:5-95:
+// psi = sqrt(|6/G (p(rho, T) - rho/3)|); e = (a, b, R, G, omega, T_cr).
+template <typename R>
+__device__ __forceinline__ R eos_pressure(int kind, R rho, R T, const float (&e)[6]) {
+  const R a = e[0], b = e[1], Rg = e[2];
+  const R rep = rho * Rg * T / (R(1) - b * rho);
+  if (kind == LBM_PSI_VDW) return rep - a * (rho * rho);
+  if (kind == LBM_PSI_RK) return rep - a * (rho * rho) / sqrt(T) / (R(1) + b * rho);
+  if (kind == LBM_PSI_PR) {
+    const R w = e[4];
+    const R k = R(1) + (R(0.37464) + R(1.54226) * w - R(0.269922) * (w * w)) * (R(1) - sqrt(T / (R)e[5]));
+    const R br = b * rho;
+    return rep - a * (k * k) * (rho * rho) / (R(1) + R(2) * br - br * br);
+  }
+  // Carnahan-Starling
+  const R x = b * rho / R(4);
+  const R om = R(1) - x;
+  return rho * Rg * T * (R(1) + x + x * x - x * x * x) / (om * om * om) - a * (rho * rho);
+}
+template <typename R>
+__device__ __forceinline__ R psi_of(int kind, R rho, R T, R rho0, const float (&e)[6]) {
+  if (kind == LBM_PSI_SC) return sc_psi<R>(rho, rho0);
+  return sqrt(fabs(R(6) / (R)e[3] * (eos_pressure<R>(kind, rho, T, e) - rho / R(3))));
+}
+
 // error reporting shared by every translation unit of the library
 extern thread_local std::string g_err;
 int fail(const char* fmt, ...) __attribute__((format(printf, 1, 2)));

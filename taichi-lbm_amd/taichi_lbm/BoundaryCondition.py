@@ -91,16 +91,25 @@ def evaluate_predicate(fn, NX: int, NY: int) -> np.ndarray:
 
 class BoundaryClassifier:
     """BoundaryCondition.py:22-39.  ``shape`` is any object with ``.shape`` (the
-    reference passes a Taichi field) or an (NX, NY) tuple."""
+    reference passes a Taichi field) or an (NX, NY) tuple; ``offset`` (or the
+    object's own ``.offset``) shifts the index range, as the psi halo field of
+    shape (NX+2, NY+2) with offset (-1, -1) does (the classifier of the psi
+    boundaries, cases/Bubble or two-phase separation.py:149-150): the group
+    then holds (i, j) from the offset on."""
 
-    def __init__(self, shape):
+    def __init__(self, shape, offset=None):
         self.shape = shape
         sh = shape.shape if hasattr(shape, "shape") else tuple(shape)
         self.NX, self.NY = int(sh[0]), int(sh[1])
+        off = offset if offset is not None else getattr(shape, "offset", (0, 0))
+        self.offset = (int(off[0]), int(off[1]))
 
     def get_group(self, spec: BoundarySpec) -> MaskAndGroup:
-        hit = evaluate_predicate(spec.geometry_fn, self.NX, self.NY)
-        return MaskAndGroup(self.NX, self.NY, np.argwhere(hit).astype(np.int32))
+        ox, oy = self.offset
+        fn = spec.geometry_fn if (ox, oy) == (0, 0) else (lambda I, J: spec.geometry_fn(I + ox, J + oy))
+        hit = evaluate_predicate(fn, self.NX, self.NY)
+        nodes = np.argwhere(hit).astype(np.int32) + np.array([ox, oy], np.int32)
+        return MaskAndGroup(self.NX, self.NY, nodes)
 
 
 _C = np.array([[0, 0], [1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [-1, 1], [-1, -1], [1, -1]])
@@ -111,8 +120,8 @@ def check_granular_bounds(kind, nodes, NX, NY, direction=None, table=None, name=
     from this group lies inside the NX x NY lattice: the group's nodes; for
     the neighbour-table kinds (BounceBackWall, InsideBoundary) every table
     entry of those nodes is (-1, -1) or a node of the lattice; for the node
-    rules that read x - c_d (VelocityNEEM, PressureNEEM, OpenNeumann) that
-    interior neighbour.  The reference reads out of range silently
+    rules that read x - c_d (VelocityNEEM, PressureNEEM, OpenNeumann,
+    PressureABC) that interior neighbour.  The reference reads out of range silently
     (BoundaryCondition.py:154-166, :215-226, :321-334, :446-449); the kernels
     trust these addresses, so they are checked here before any launch."""
     nodes = np.asarray(nodes, np.int64).reshape(-1, 2)
@@ -130,9 +139,11 @@ def check_granular_bounds(kind, nodes, NX, NY, direction=None, table=None, name=
             m, k = np.argwhere(bad)[0]
             raise H.LBMError(f"{name}: neighbour table entry {tuple(nb[m, k])} of node {tuple(nodes[m])}, "
                              f"population {k}, is neither -1 nor a node of the lattice")
-    elif kind in (H.BC_VELOCITY_NEEM, H.BC_PRESSURE_NEEM, H.BC_OPEN_NEUMANN):
+    elif kind in (H.BC_VELOCITY_NEEM, H.BC_PRESSURE_NEEM, H.BC_OPEN_NEUMANN, H.BC_PRESSURE_ABC, H.BC_VELOCITY_EQ):
         if direction not in (1, 2, 3, 4):
             raise H.LBMError(f"{name}: direction {direction} (1..4)")
+        if kind == H.BC_VELOCITY_EQ:     # reads its own node only
+            return
         I2, J2 = I - _C[direction, 0], J - _C[direction, 1]
         out = (I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)
         if out.any():
@@ -235,6 +246,49 @@ class PressureNEEM(PressureBoundary):
         self._granular(lb_field, H.BC_PRESSURE_NEEM, self.rule())
 
 
+class VelocityEquilibrium(VelocityBoundary):
+    """BoundaryCondition.py:132-145: vel = u_w, rho_w from the node's populations
+    (Zou-He form), then all nine populations := feq(u_w, rho_w)."""
+
+    kind = "velocity_eq"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = H.RULE_VELOCITY_EQ, int(self.direction)
+        r.ux, r.uy = (float(np.float32(v)) for v in self.velocity_value)
+        return r
+
+    def fused_spec(self, name=""):
+        ux, uy = (float(np.float32(v)) for v in self.velocity_value)
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), ux, uy, 0.0, name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_VELOCITY_EQ, self.rule())
+
+
+class PressureABC(PressureBoundary):
+    """BoundaryCondition.py:177-205: anti-bounce-back pressure outlet.  The
+    velocity is extrapolated, u_w = 1.5 u(x) - 0.5 u(x - c_d), total_rho :=
+    rho_value, and each unknown population (written by an earlier
+    bounce-back) becomes -f_k + 2 w_k rho_value (1 + 4.5 (c_k . u_w)^2 -
+    1.5 u_w^2), which imposes rho_value."""
+
+    kind = "pressure_abc"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = H.RULE_PRESSURE_ABC, int(self.direction)
+        r.rho = float(np.float32(self.rho_value))
+        return r
+
+    def fused_spec(self, name=""):
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), 0.0, 0.0,
+                      float(np.float32(self.rho_value)), name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_PRESSURE_ABC, self.rule())
+
+
 class VelocityBB(VelocityBoundary):
     """BoundaryCondition.py:80-102: adds 6 w_d rho c_d . u_w to the unknown
     populations (after an earlier bounce-back has written them)."""
@@ -318,8 +372,6 @@ def _out_of_scope(name, where):
     return type(name, (BoundaryCondition,), {"__init__": init})
 
 
-VelocityEquilibrium = _out_of_scope("VelocityEquilibrium", "BoundaryCondition.py:132-147")
-PressureABC = _out_of_scope("PressureABC", "BoundaryCondition.py:177-205")
 OpenConvective1order = _out_of_scope("OpenConvective1order", "BoundaryCondition.py:337-359")
 OpenConvective2order = _out_of_scope("OpenConvective2order", "BoundaryCondition.py:362-390")
 OpenExtrapolation = _out_of_scope("OpenExtrapolation", "BoundaryCondition.py:394-415")

@@ -146,11 +146,16 @@ class ShanChenForceC1(ShanChenModel):
         L.shan_chen = H.SC_C1
         L.g_coh[0] = float(f32(self.g_coh))
         for c in range(self.nc):
-            L.psi_rho0[c] = float(f32(self.psi.rho0))
+            self.psi.fill(L, c)
 
 
 class ShanChenForceC2(ShanChenModel):
-    """ExternalForce.py:94-145: two-component cohesion (every neighbour fluid)."""
+    """ExternalForce.py:66-116: two-component cohesion; a neighbour x + c_k for
+    which fluid_strategy(x2, y2) is false contributes the adhesion w_k c_k
+    gadh[c1] instead (without psi(x + c_k), commented out at :114).
+    fluid_strategy is evaluated once on the host over the halo grid
+    (x2, y2 in [-1, NX] x [-1, NY]), vectorised (integer index arrays) or
+    node by node."""
 
     SC_KIND = H.SC_C2
 
@@ -165,17 +170,47 @@ class ShanChenForceC2(ShanChenModel):
         self.g_adh = params['gadh']
         self.psi1, self.psi2 = params['psi_list'][0], params['psi_list'][1]
         self.fluid_strategy = params['fluid_strategy']
+        self.fluid_map = None          # (NX+2, NY+2) bool over the halo grid, offset -1
+        self._fluid_dev = None
         if self.fluid_strategy is not None:
-            I, J = np.meshgrid(np.arange(-1, self.NX + 1), np.arange(-1, self.NY + 1), indexing="ij")
-            ok = np.asarray(self.fluid_strategy(I, J))
-            if not np.all(ok):
-                raise NotImplementedError("ShanChenForceC2 adhesion (fluid_strategy false somewhere, "
-                                          "ExternalForce.py:141-142) is not supported")
+            from .BoundaryCondition import evaluate_predicate
+            fm = evaluate_predicate(lambda I, J: self.fluid_strategy(I - 1, J - 1), self.NX + 2, self.NY + 2)
+            if not fm.all():
+                self.fluid_map = fm
+
+    @property
+    def adhesion(self):
+        return self.fluid_map is not None
+
+    def sc_fluid_device(self):
+        """The fluid map as uint8 on the device (lbm_gfields.sc_fluid)."""
+        if self.fluid_map is None:
+            return None
+        if self._fluid_dev is None:
+            self._fluid_dev = torch().as_tensor(np.ascontiguousarray(self.fluid_map.astype(np.uint8)).reshape(-1),
+                                                device=device())
+        return self._fluid_dev
+
+    def solid_codes(self):
+        """Node-code bits (LBM_CODE_SC_SOLID) of the lattice's nodes for the
+        fused kernel, which wraps the halo periodically: refused unless the
+        halo's flags are the periodic wrap of the lattice's."""
+        if self.fluid_map is None:
+            return None
+        inner = self.fluid_map[1:-1, 1:-1]
+        if not np.array_equal(np.pad(inner, 1, mode="wrap"), self.fluid_map):
+            raise NotImplementedError("the fused Shan-Chen path reads the psi halo as a periodic wrap: fluid_strategy "
+                                      "on the halo must equal its value at the wrapped node (use the granular "
+                                      "cau_SCforce otherwise)")
+        return np.where(inner.reshape(-1), 0, H.CODE_SC_SOLID).astype(np.uint16)
 
     def fill_lattice(self, L):
         L.shan_chen = H.SC_C2
         for a in range(2):
             for b in range(2):
                 L.g_coh[a * H.MAX_NC + b] = float(self.g_coh[a, b])
-        L.psi_rho0[0] = float(f32(self.psi1.rho0))
-        L.psi_rho0[1] = float(f32(self.psi2.rho0))
+        self.psi1.fill(L, 0)
+        self.psi2.fill(L, 1)
+        gadh = [0.0, 0.0] if self.g_adh is None else list(self.g_adh)
+        L.g_adh[0], L.g_adh[1] = (float(f32(v)) for v in gadh[:2])
+        L.sc_adhesion = 1 if self.fluid_map is not None else 0

@@ -50,7 +50,7 @@ class UnsupportedBoundaryList(ValueError):
 class BCSpec:
     """Host description of one boundary-list entry."""
     kind: str                 # fluid | periodic | inside | bounceback | velocity_neem | pressure_neem
-                              # | velocity_bb | open_neumann
+                              # | velocity_bb | open_neumann | pressure_abc | velocity_eq
     nodes: np.ndarray         # (m, 2) int (i, j)
     direction: int = 0
     ux: float = 0.0
@@ -108,7 +108,8 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     neem_rules: List[Tuple] = []
     neem_nodes: List[Tuple[np.ndarray, np.ndarray]] = []   # (nodes, interior neighbours or None)
     rule_kinds = {"velocity_neem": H.NEEM_VELOCITY, "pressure_neem": H.NEEM_PRESSURE,
-                  "velocity_bb": H.RULE_VELOCITY_BB, "open_neumann": H.RULE_OPEN_NEUMANN}
+                  "velocity_bb": H.RULE_VELOCITY_BB, "open_neumann": H.RULE_OPEN_NEUMANN,
+                  "pressure_abc": H.RULE_PRESSURE_ABC, "velocity_eq": H.RULE_VELOCITY_EQ}
     nb_ok = (neighbor.reshape(N, 9) if neighbor.ndim == 3 else neighbor.reshape(N, 9, 2)[:, :, 0] != -1)
 
     for bc in bcs:
@@ -139,7 +140,7 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
             e = len(neem_rules)
             pre_rule[e] = rule[flat].copy()
             nbf = None
-            if bc.kind != "velocity_bb":
+            if bc.kind not in ("velocity_bb", "velocity_eq"):     # rules that read x - c_d
                 I2, J2 = nodes[:, 0] - C[d, 0], nodes[:, 1] - C[d, 1]
                 if np.any((I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)):
                     raise UnsupportedBoundaryList(
@@ -198,8 +199,8 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     for e, (flat, nbf) in enumerate(neem_nodes):
         kind = neem_rules[e][0]
         code[flat] |= np.uint16((e + 1) << H.CODE_NEEM_SHIFT)
-        if kind in (H.NEEM_VELOCITY, H.NEEM_PRESSURE):
+        if kind in (H.NEEM_VELOCITY, H.NEEM_PRESSURE, H.RULE_PRESSURE_ABC):
             code[nbf] |= np.uint16(H.CODE_KEEP_MACRO)        # neighbour's rho/u of the previous step
-        if kind in (H.NEEM_PRESSURE, H.RULE_VELOCITY_BB):
-            code[flat] |= np.uint16(H.CODE_KEEP_MACRO)       # own rho of the previous step
+        if kind in (H.NEEM_PRESSURE, H.RULE_VELOCITY_BB, H.RULE_PRESSURE_ABC):
+            code[flat] |= np.uint16(H.CODE_KEEP_MACRO)       # own rho (ABC: u) of the previous step
     return code, neem_rules

@@ -14,9 +14,11 @@ LBM_BGK, LBM_TRT, LBM_MRT = 0, 1, 2
 STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT, STORE_F64 = 0, 1, 2, 3
 SC_NONE, SC_C1, SC_C2 = 0, 1, 2
 TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
-NEEM_VELOCITY, NEEM_PRESSURE, RULE_VELOCITY_BB, RULE_OPEN_NEUMANN = 1, 2, 3, 4
+NEEM_VELOCITY, NEEM_PRESSURE, RULE_VELOCITY_BB, RULE_OPEN_NEUMANN, RULE_PRESSURE_ABC, RULE_VELOCITY_EQ = \
+    1, 2, 3, 4, 5, 6
 BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM, BC_VELOCITY_BB, BC_OPEN_NEUMANN = \
     1, 2, 3, 4, 5, 6, 7
+BC_PRESSURE_ABC, BC_VELOCITY_EQ = 8, 9
 
 CODE_BOUNCE_MASK = 0x00FF
 CODE_ACTIVE = 0x0100
@@ -24,9 +26,12 @@ CODE_KEEP_MACRO = 0x0200
 CODE_NEEM_SHIFT = 10
 CODE_NEEM_MASK = 0x3C00
 CODE_PULLED = 0x4000      # inactive node pulled by an active one (plain pull)
+CODE_SC_SOLID = 0x8000    # fluid_strategy false: a Shan-Chen neighbour giving adhesion
+PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
+PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 
 class NeemRule(C.Structure):
@@ -47,7 +52,10 @@ class Lattice(C.Structure):
         ("gx", C.c_float), ("gy", C.c_float),
         ("g_coh", C.c_float * (MAX_NC * MAX_NC)),
         ("psi_rho0", C.c_float * MAX_NC),
-        ("pad0", C.c_int32), ("pad1", C.c_int32),
+        ("psi_kind", C.c_int32 * MAX_NC),
+        ("eos", (C.c_float * 6) * MAX_NC),
+        ("g_adh", C.c_float * MAX_NC),
+        ("sc_adhesion", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -55,14 +63,14 @@ class Fields(C.Structure):
     _fields_ = [
         ("f", C.c_void_p * 2), ("macro", C.c_void_p * 2), ("tau", C.c_void_p),
         ("psi", C.c_void_p * 2), ("body_force", C.c_void_p), ("sc_force", C.c_void_p),
-        ("shear_rate", C.c_void_p), ("code", C.c_void_p), ("tau2", C.c_void_p),
+        ("shear_rate", C.c_void_p), ("code", C.c_void_p), ("tau2", C.c_void_p), ("temperature", C.c_void_p),
     ]
 
 
 class GFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "f", "f2", "rho", "total_rho", "vel", "body_force", "sc_force", "shear_rate", "pressure",
-        "total_pressure", "temperature", "psi", "neighbor", "tau")]
+        "total_pressure", "temperature", "psi", "neighbor", "tau", "sc_fluid")]
 
 
 class LBMError(RuntimeError):
@@ -103,6 +111,7 @@ def lib():
         "lbm_copy_f32": ([P, P, I64, P], C.c_int),
         "lbm_g_boundary": ([LP, GP, I32, NP, P, I32, P], C.c_int),
         "lbm_g_sc_periodic_psi": ([LP, GP, I32, P], C.c_int),
+        "lbm_g_sc_psi_boundary": ([LP, GP, I32, I32, C.c_float, P, I32, P], C.c_int),
     }
     for name in ("lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density", "lbm_g_velocity",
                  "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_sc_update_psi",
@@ -149,7 +158,8 @@ def exported_symbols():
             "lbm_finish", "lbm_reduce_macro",
             "lbm_residual", "lbm_copy_f32", "lbm_g_init_lbm", "lbm_g_density", "lbm_g_pressure", "lbm_g_force_density",
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
-            "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_batch_create",
+            "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_g_sc_psi_boundary",
+            "lbm_batch_create",
             "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
             "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2", "lbm_batch_release",

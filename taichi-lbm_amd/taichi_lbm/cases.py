@@ -56,7 +56,7 @@ def _finish(lbf, bce, fluid_bc, coll, rho0=1.0, u0=(0.0, 0.0)):
 
 
 def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0, collision="bgk", magic=0.25,
-             tau_bulk=None, name="cylinder", inlet="neem"):
+             tau_bulk=None, name="cylinder", inlet="neem", outlet="neem"):
     """Cylinder in a channel: bounce-back top/bottom rows and cylinder,
     PressureNEEM outlet (d=1) on 0 < j < NY-1, f = w at start.  The inlet
     (i = 0, 0 < j < NY-1, d = 3) is
@@ -68,9 +68,13 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
               few hundred steps (in FP64 as in FP32);
       "bb"    VelocityBB over a bounce-back inlet column, the inlet every
               reference case uses (cases/Circular cylinder.py:91-118,
-              cases/contraction flow.py:110-138); stable."""
-    if inlet not in ("neem", "bb"):
-        raise ValueError("inlet must be 'neem' or 'bb'")
+              cases/contraction flow.py:110-138); stable;
+      "eq"    VelocityEquilibrium (BoundaryCondition.py:132-145): the node's
+              populations replaced by feq(u_in, rho_w).
+    outlet="abc" replaces the PressureNEEM outlet with PressureABC over a
+    bounced outlet column (BoundaryCondition.py:177-205)."""
+    if inlet not in ("neem", "bb", "eq"):
+        raise ValueError("inlet must be 'neem', 'bb' or 'eq'")
     lbf = LBField(name, NX, NY, 1)
     bce = BoundaryEngine()
     clf = BoundaryClassifier((NX, NY))
@@ -85,6 +89,8 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
     wmap = solid_neighbour_map(m, y_walls=True)
     if inlet == "bb":
         wmap[0, :] |= m[0, :] == 1
+    if outlet == "abc":
+        wmap[NX - 1, :] |= m[NX - 1, :] == 1
     wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
     wall.precompute(clf)
     bce.add_boundary_condition("wall", wall)
@@ -92,12 +98,20 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
     if inlet == "bb":
         from .BoundaryCondition import VelocityBB
         inlet_bc = VelocityBB(inlet_spec, (u_in, 0.0), 3)
+    elif inlet == "eq":
+        from .BoundaryCondition import VelocityEquilibrium
+        inlet_bc = VelocityEquilibrium(inlet_spec, (u_in, 0.0), 3)
     else:
         inlet_bc = VelocityNEEM(inlet_spec, (u_in, 0.0), 3)
     inlet_bc.precompute(clf)
     bce.add_boundary_condition("inlet", inlet_bc)
-    outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
-                          rho_out, 1)
+    outlet_spec = BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1))
+    if outlet == "abc":
+        from .BoundaryCondition import PressureABC
+        outlet_bc = PressureABC(outlet_spec, rho_out, 1)
+    else:
+        outlet_bc = PressureNEEM(outlet_spec, rho_out, 1)
+    outlet = outlet_bc
     outlet.precompute(clf)
     bce.add_boundary_condition("outlet", outlet)
     lbf.neighbor_classify()
@@ -107,7 +121,7 @@ def cylinder(NX=400, NY=100, cx=80, cy=50, r=10, tau=0.6, u_in=0.05, rho_out=1.0
     macro, rho, vel = _finish(lbf, bce, fluid, coll)
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
                 spec=dict(kind="cylinder", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision, magic=magic,
-                          inlet=inlet))
+                          inlet=inlet, outlet=outlet_bc.kind))
 
 
 def place_disks(NX, NY, seed, rmin=8.0, rmax=32.0, solid_fraction=0.3, margin=2.0, max_tries=10_000_000):
@@ -220,17 +234,29 @@ def stepper(case):
 
 
 def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt", nu=1.0 / 6.0, rho_mean=None,
-              amp=None, name="shan_chen", disk=None):
+              amp=None, name="shan_chen", disk=None, gadh=None, psi_wall=None, eos=None):
     """Periodic Shan-Chen lattice.  nc=1: ShanChenForceC1 with cohesion G, rho =
     rho_mean + amp*U(0,1) (cases/Bubble or two-phase separation.py:174);
     nc=2: ShanChenForceC2 with G = [[0, gcross], [gcross, 0]] and each rho_c =
-    1 + amp*U(-1, 1), seeded per component.  psi = SC_psi(rho0=1).
+    rho_mean (default 1) + amp*U(-1, 1), seeded per component.  psi = SC_psi(rho0=1).
     disk=(cx, cy, r): a solid disk (Mask_cricle_identify) with a
     BounceBackWall group around it; psi stays 0 on its nodes (update_psi
-    writes the group only)."""
-    from .EOS import SC_psi
+    writes the group only) unless psi_wall gives a FixedPsi over them
+    (SCBoundaryCondition.py:110-123, after the periodic pair).  gadh=(a0, a1)
+    (nc=2): ShanChenForceC2 adhesion at the disk, fluid_strategy = mask == 1
+    (periodic on the halo; ExternalForce.py:105-116).  eos=(kind, params)
+    with kind "vdw" | "rk" | "pr" | "cs": the cubic-EOS pseudopotential of
+    EOS.py in place of SC_psi, at T = 1."""
+    from .EOS import CS_psi, PR_psi, RK_psi, SC_psi, vdW_psi
     from .ExternalForce import ShanChenForceC1, ShanChenForceC2
-    from .SCBoundaryCondition import BoundaryEngineSC, PeriodicPsi_BT, PeriodicPsi_LR
+    from .SCBoundaryCondition import BoundaryEngineSC, FixedPsi, PeriodicPsi_BT, PeriodicPsi_LR
+    import contextlib
+    import io
+    if eos is not None:
+        with contextlib.redirect_stdout(io.StringIO()):
+            psi_obj = {"vdw": vdW_psi, "rk": RK_psi, "pr": PR_psi, "cs": CS_psi}[eos[0]](dict(eos[1]))
+    else:
+        psi_obj = SC_psi(1.0)
     lbf = LBField(name, NX, NY, nc)
     bce = BoundaryEngine()
     clf = BoundaryClassifier((NX, NY))
@@ -256,22 +282,30 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
         rho_mean = math.log(2.0) if rho_mean is None else rho_mean
         amp = 0.1 if amp is None else amp
         rho[..., 0] = rho_mean + amp * rng.random((NX, NY))
-        sc = ShanChenForceC1({'lb_field': lbf, 'g_coh': G, 'group': fluid.group, 'psi': SC_psi(1.0)})
+        sc = ShanChenForceC1({'lb_field': lbf, 'g_coh': G, 'group': fluid.group, 'psi': psi_obj})
         g_spec = G
     else:
         amp = 0.01 if amp is None else amp
+        rho_mean = 1.0 if rho_mean is None else rho_mean
         for c in range(nc):
-            rho[..., c] = 1.0 + amp * rng.uniform(-1.0, 1.0, (NX, NY))
+            rho[..., c] = rho_mean + amp * rng.uniform(-1.0, 1.0, (NX, NY))
         Gm = np.array([[0.0, gcross], [gcross, 0.0]])
-        sc = ShanChenForceC2({'lb_field': lbf, 'g_coh': Gm, 'gadh': [0.0, 0.0], 'group': fluid.group,
-                              'psi_list': [SC_psi(1.0), SC_psi(1.0)], 'fluid_strategy': None})
+        strategy = None if gadh is None else (lambda I, J: m[I % NX, J % NY] == 1)
+        sc = ShanChenForceC2({'lb_field': lbf, 'g_coh': Gm, 'gadh': list(gadh) if gadh else [0.0, 0.0],
+                              'group': fluid.group, 'psi_list': [psi_obj, psi_obj], 'fluid_strategy': strategy})
         g_spec = Gm
     lbf.sc_field = sc
     bsc = BoundaryEngineSC()
     bsc.add_boundary_condition("periodicLR", PeriodicPsi_LR())
     bsc.add_boundary_condition("periodicBT", PeriodicPsi_BT())
+    if psi_wall is not None:
+        clf_psi = BoundaryClassifier((NX + 2, NY + 2), offset=(-1, -1))
+        wall_psi = FixedPsi(BoundarySpec(lambda I, J: (I >= 0) & (I < NX) & (J >= 0) & (J < NY) &
+                                         (m[np.clip(I, 0, NX - 1), np.clip(J, 0, NY - 1)] == -1)), psi_wall)
+        wall_psi.precompute(clf_psi)
+        bsc.add_boundary_condition("wall_psi", wall_psi)
     psi = np.zeros((NX + 2, NY + 2, nc), np.float32)
-    psi[1:-1, 1:-1, :] = SC_psi(1.0).get_psi(rho)
+    psi[1:-1, 1:-1, :] = psi_obj.get_psi(rho, np.float32(1.0))
     psi[1:-1, 1:-1, :][m == -1] = 0.0
     psi = np.pad(psi[1:-1, 1:-1, :], ((1, 1), (1, 1), (0, 0)), mode="wrap")
     macro = MacroscopicEngine(fluid.group)
@@ -279,7 +313,7 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
                 psi=psi, bsc=bsc, sc_spec=dict(kind="c1" if nc == 1 else "c2", g_coh=g_spec, rho0=(1.0,) * nc),
                 spec=dict(kind="shan_chen", NX=NX, NY=NY, nc=nc, G=G, gcross=gcross, seed=seed,
-                          collision=collision, nu=nu, disk=disk))
+                          collision=collision, nu=nu, disk=disk, gadh=gadh, psi_wall=psi_wall, eos=eos))
 
 
 
@@ -312,7 +346,7 @@ def edge_or_solid_map(mask: np.ndarray) -> np.ndarray:
 
 
 def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt", magic=0.25, tau_min=0.51,
-                tau_max=2.0, name="contraction", inlet="bb", stream="periodic"):
+                tau_max=2.0, name="contraction", inlet="bb", stream="periodic", outlet="neem"):
     """4:1 planar contraction scaled from cases/contraction flow.py:16-33,80-81
     (a 40 x 8 channel with solid blocks over x in (25, 35) leaving the gap
     y in (3, 5)): VelocityBB inlet over a bounce-back inlet column (d=3,
@@ -326,7 +360,11 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     fluid nodes next to a solid or beyond a lattice edge, VelocityBB,
     PressureNEEM; every -1 table entry the table stream would read is then
     bounced.  stream="periodic" streams with PeriodicAllBoundary instead and
-    bounces the inlet column: the same fields."""
+    bounces the inlet column: the same fields.  outlet="abc" replaces the
+    PressureNEEM outlet with PressureABC (BoundaryCondition.py:177-205,
+    anti-bounce-back over a bounced outlet column), which imposes rho = 1
+    there; PressureNEEM extrapolates the outlet density from the node itself,
+    so the channel's mass drifts and the flow never becomes steady."""
     NX, NY = int(W), int(H)
     lbf = LBField(name, NX, NY, 1)
     bce = BoundaryEngine()
@@ -351,6 +389,8 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
         wmap = solid_neighbour_map(m, y_walls=True)
         if inlet == "bb":
             wmap[0, :] |= m[0, :] == 1
+        if outlet == "abc":
+            wmap[NX - 1, :] |= m[NX - 1, :] == 1
     wall = BounceBackWall(BoundarySpec(lambda I, J: wmap[I, J]))
     wall.precompute(clf)
     bce.add_boundary_condition("wall", wall)
@@ -362,8 +402,12 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
         inlet_bc = VelocityNEEM(inlet_spec, (u_in, 0.0), 3)
     inlet_bc.precompute(clf)
     bce.add_boundary_condition("inlet", inlet_bc)
-    outlet = PressureNEEM(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1)),
-                          1.0, 1)
+    outlet_spec = BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1) & (J > 0) & (J < NY - 1))
+    if outlet == "abc":
+        from .BoundaryCondition import PressureABC
+        outlet = PressureABC(outlet_spec, 1.0, 1)
+    else:
+        outlet = PressureNEEM(outlet_spec, 1.0, 1)
     outlet.precompute(clf)
     bce.add_boundary_condition("outlet", outlet)
     lbf.neighbor_classify()
@@ -376,7 +420,7 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=True,
                 spec=dict(kind="contraction", NX=NX, NY=NY, power_n=power_n, power_m=power_m, u_in=u_in,
                           nu_ref=nu_ref, collision=collision, tau_min=tau_min, tau_max=tau_max, inlet=inlet,
-                          stream=stream))
+                          stream=stream, outlet=outlet if isinstance(outlet, str) else outlet.kind))
 
 
 def cavity(N=256, power_n=1.0, Re=10.0, u_lid=5e-3, collision="mrt", magic=0.25, name="cavity"):

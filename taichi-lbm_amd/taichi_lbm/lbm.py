@@ -261,6 +261,8 @@ def gfields(lbf: LBField, collision=None, sc=None, neighbor=False) -> H.GFields:
         g.tau = H.ptr(collision.state_tensor())
     if sc is not None:
         g.psi = H.ptr(sc.psi_halo())
+        fm = sc.sc_fluid_device() if hasattr(sc, "sc_fluid_device") else None
+        g.sc_fluid = H.ptr(fm) if fm is not None else None
     if neighbor:
         g.neighbor = H.ptr(lbf.neighbor_device())
     return g

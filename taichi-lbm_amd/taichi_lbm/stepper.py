@@ -68,10 +68,13 @@ class Stepper:
             raise NotImplementedError("shifted population storage is selected through BatchRunner")
         self.storage = _STORAGE[storage]
         self.sc = lb_field.sc_field
+        self._psi_fixed = []
         if self.sc is not None:
-            if boundary_engine_sc is None or not boundary_engine_sc.is_periodic():
+            if boundary_engine_sc is None:
                 raise NotImplementedError("the fused Shan-Chen path needs the psi boundaries "
                                           "PeriodicPsi_LR then PeriodicPsi_BT")
+            # FixedPsi over walls: constant psi on non-fluid nodes (component 0)
+            self._psi_fixed = boundary_engine_sc.fused_plan(lb_field, macroscopic_engine.group.nodes)
         if not _same_nodes(macroscopic_engine.group, collision_engine.group, lb_field.NY):
             raise NotImplementedError("the macroscopic and collision groups differ")
         self._compiled_version = None
@@ -81,6 +84,8 @@ class Stepper:
         lbf = self.lbf
         code, rules = compile_codes(lbf.NX, lbf.NY, self.macro.group.nodes, self.bce.fused_list(),
                                     lbf.neighbor_valid())
+        if self.sc is not None and getattr(self.sc, "adhesion", False):
+            code = code | self.sc.solid_codes()      # LBM_CODE_SC_SOLID where fluid_strategy is false
         self.code_host = code
         self.neem_host = rules
         t = torch()
@@ -111,8 +116,14 @@ class Stepper:
         F.shear_rate = H.ptr(lbf.aux("shear_rate")) if tau_mode == H.TAU_POWER_LAW else None
         if self.sc is not None:
             planes = self.sc.planes_for_run()
+            for nodes, value in self._psi_fixed:
+                if len(nodes):
+                    flat = torch().as_tensor(nodes[:, 0] * lbf.NY + nodes[:, 1], device=device())
+                    planes[:, 0, flat] = value
             F.psi[0], F.psi[1] = H.ptr(planes[0]), H.ptr(planes[1])
             F.sc_force = H.ptr(lbf.aux("SCforce"))
+            if any(L.psi_kind[c] != H.PSI_SC for c in range(lbf.nc)):
+                F.temperature = H.ptr(lbf.aux("T"))
         F.code = H.ptr(self.code)
         return L, F
 

@@ -61,8 +61,25 @@ CASES = {
     # (n = 1 through the power-law path, as the reference runs it: with n != 1 a cavity started at rest
     # takes tau from the FP32 rounding noise of its near-zero shear rates, see tests/test_inside_cases.py)
     "cavity_mrt_40x40": ("cavity", dict(N=40, power_n=1.0, u_lid=0.02), 30),
+    # the reference's other plane rules: a VelocityEquilibrium inlet and PressureABC outlets
+    # (BoundaryCondition.py:132-145, 177-205)
+    "cylinder_eq_abc_mrt_48x24": ("cylinder", dict(NX=48, NY=24, cx=12, cy=11, r=4, tau=0.6, u_in=0.05,
+                                                   collision="mrt", tau_bulk=0.8, inlet="eq", outlet="abc"), 30),
+    "contraction_abc_trt_80x24": ("contraction", dict(W=80, H=24, power_n=0.7, outlet="abc"), 30),
     "shan_chen_c1_32x32": ("shan_chen", dict(NX=32, NY=32, nc=1, G=-5.0, seed=7), 20),
     "shan_chen_c2_32x24": ("shan_chen", dict(NX=32, NY=24, nc=2, gcross=0.9, seed=9), 20),
+    # multiphase breadth: C2 adhesion at a solid disk (ExternalForce.py:105-116), a FixedPsi wall
+    # (SCBoundaryCondition.py:110-123), the Carnahan-Starling pseudopotential (EOS.py:86-95; T = 1,
+    # a = 1.17, b = 4, R = 0.1: T_c = 0.3773 a / (b R) = 1.10)
+    "shan_chen_c2_adhesion_40x32": ("shan_chen", dict(NX=40, NY=32, nc=2, gcross=0.9, seed=11, disk=(20, 16, 6),
+                                                      gadh=(0.3, -0.3)), 20),
+    "shan_chen_c1_wall_psi_40x32": ("shan_chen", dict(NX=40, NY=32, nc=1, G=-5.0, seed=12, disk=(20, 16, 6),
+                                                      psi_wall=0.6), 20),
+    "shan_chen_c1_cs_eos_32x32": ("shan_chen", dict(NX=32, NY=32, nc=1, G=-1.0, seed=13, rho_mean=0.0326 * 0.9,
+                                                    amp=0.0326 * 0.2,
+                                                    eos=("cs", dict(a=1.17, b=4.0, R=0.1, G=-1.0, rho_cr=0.0326,
+                                                                    pressure_cr=0.00517, temperature_cr=1.104))),
+                                  20),
     # the reference's own cylinder channel at full size (cases/Circular cylinder.py: 400 x 100, BGK
     # tau = 0.6, r = 10 at (80, 50), u = 0.05): with the VelocityBB inlet list, and with the
     # VelocityNEEM inlet, which diverges after about 150 steps, so its vectors stop at 60

@@ -51,9 +51,9 @@ def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None, 
     for name, bc in case["bce"].boundary_conditions.items():
         k = bc.kind
         val = None
-        if k in ("velocity_neem", "velocity_bb"):
+        if k in ("velocity_neem", "velocity_bb", "velocity_eq"):
             val = tuple(_r32(v) for v in bc.velocity_value) if params32 else bc.velocity_value
-        elif k == "pressure_neem":
+        elif k in ("pressure_neem", "pressure_abc"):
             val = bc.rho_value
         bcs.append(O.BC(k, bc.group.nodes.astype(np.int64), getattr(bc, "direction", None), val))
     dt = np.dtype(dtype)
@@ -67,11 +67,30 @@ def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None, 
         if psi is not None:
             lat.psi[...] = psi.astype(dt)
         lat.T[...] = 1.0
-        sc = O.ShanChen(s["kind"], group, s["g_coh"], s["rho0"])
+        obj = lbf.sc_field
+        psis = [obj.psi] * lbf.nc if s["kind"] == "c1" else [obj.psi1, obj.psi2]
+        eos = [None if type(q).__name__ == "SC_psi" else
+               O.EOS({"vdW_psi": "vdw", "RK_psi": "rk", "PR_psi": "pr", "CS_psi": "cs"}[type(q).__name__],
+                     q.a, q.b, q.R, q.G, q.omega, q.temperature_cr or 1.0) for q in psis]
+        sc = O.ShanChen(s["kind"], group, s["g_coh"], s["rho0"], g_adh=getattr(obj, "g_adh", None),
+                        fluid=getattr(obj, "fluid_map", None), eos=eos if any(e is not None for e in eos) else None)
+    psi_bcs = None
+    if case.get("bsc") is not None:
+        psi_bcs = []
+        kinds = {"OpenNeumannPsi": "neumann", "OpenConvective1orderPsi": "convective1",
+                 "OpenConvective2orderPsi": "convective2", "OpenExtrapolationPsi": "extrapolation", "FixedPsi": "fixed"}
+        for b in case["bsc"].boundary_conditions.values():
+            name = type(b).__name__
+            if name in ("PeriodicPsi_LR", "PeriodicPsi_BT"):
+                if name == "PeriodicPsi_LR":
+                    psi_bcs.append("periodic")
+                continue
+            psi_bcs.append(O.PsiBC(kinds[name], np.asarray(b.group.nodes, np.int64), getattr(b, "direction", None),
+                                   getattr(b, "psi_value", 0.0)))
     if f is None:
         O.init_lbm(lat, group)
     else:
         lat.f[...] = f.astype(dt)
         lat.f2[...] = f.astype(dt)
-    solver = O.Solver(lat, group, oc, bcs, power_law=case.get("power_law", False), sc=sc)
+    solver = O.Solver(lat, group, oc, bcs, power_law=case.get("power_law", False), sc=sc, psi_bcs=psi_bcs)
     return solver

@@ -69,7 +69,8 @@ def test_fused_against_fixture(name):
 
 
 @pytest.mark.parametrize("name", ["cylinder_bgk_48x24", "cylinder_refbcs_bgk_60x24", "porous_powerlaw_mrt_48x40",
-                                  "contraction_inside_trt_80x24", "cavity_mrt_40x40"])
+                                  "contraction_inside_trt_80x24", "cavity_mrt_40x40", "cylinder_eq_abc_mrt_48x24",
+                                  "contraction_abc_trt_80x24"])
 def test_granular_sequence_against_fixture(name):
     """lbm_solve's order (cases/Circular cylinder.py:182-193, cases/Power Law
     Fluid.py:160-173) through the one-kernel-per-method entry points; the

@@ -744,3 +744,193 @@ def test_inside_boundary_table_stream():
     for k in range(9):
         if lat.neighbor[w0[0], w0[1], k, 0] == -1:
             assert np.array_equal(lat.f[w0[0], w0[1], :, k], lat.f2[w0[0], w0[1], :, O.OPP[k]])
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4])
+def test_pressure_abc_returns_the_equilibrium_after_bounce_back(d):
+    # PressureABC (BoundaryCondition.py:177-205) on an equilibrium state after a
+    # halfway bounce-back of the unknowns: f_opp(k) = feq_opp(k)(rho, u), and
+    # -feq_opp + 2 w rho (1 + 4.5 (c.u)^2 - 1.5 u^2) = feq_k(rho, u) exactly, so
+    # with rho_value = rho and a uniform u (extrapolation returns u) the node
+    # is back at equilibrium; a different rho_value moves it to rho_value's
+    # equilibrium (the imposed pressure) on the unknowns only
+    NX, NY = 5, 4
+    lat, grp = _lat_all_fluid(NX, NY)
+    u0, rho0 = np.array([0.03, -0.02]), 1.04
+    fe = O.f_eq(np.tile(u0, (NX * NY, 1)), np.full(NX * NY, rho0), f64).reshape(NX, NY, 9)
+    lat.f[:, :, 0, :] = fe
+    lat.vel[...] = u0
+    line = {1: [[NX - 1, j] for j in range(NY)], 3: [[0, j] for j in range(NY)],
+            2: [[i, NY - 1] for i in range(NX)], 4: [[i, 0] for i in range(NX)]}[d]
+    I, J = np.array(line).T
+    unk = O.UNKNOWN[d]
+    for k in unk:       # what the bounce-back leaves there
+        lat.f[I, J, 0, k] = fe[I, J, O.OPP[k]]
+    O.BC("pressure_abc", np.array(line), d, rho0).apply(lat)
+    assert np.allclose(lat.f[I, J, 0], fe[I, J], atol=1e-16)
+    assert np.allclose(lat.vel[I, J], u0) and np.allclose(lat.total_rho[I, J], rho0)
+    for k in unk:
+        lat.f[I, J, 0, k] = fe[I, J, O.OPP[k]]
+    O.BC("pressure_abc", np.array(line), d, 0.9).apply(lat)
+    known = [k for k in range(9) if k not in unk]
+    assert np.array_equal(lat.f[I, J, 0][:, known], fe[I, J][:, known])
+    # the unknowns moved by exactly 2 w_k (rho_value - rho) (1 + 4.5 (c.u)^2 - 1.5 u^2)
+    cu = O.C.astype(f64) @ u0
+    shift = 2 * O.weights(f64) * (0.9 - rho0) * (1 + 4.5 * cu ** 2 - 1.5 * u0 @ u0)
+    assert np.allclose(lat.f[I, J, 0][:, unk] - fe[I, J][:, unk], shift[unk], atol=1e-16)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4])
+def test_velocity_equilibrium_sets_the_zou_he_equilibrium(d):
+    # VelocityEquilibrium (BoundaryCondition.py:132-145): vel = u_w, rho = the
+    # Zou-He wall density of the node's populations, all nine = feq(u_w, rho);
+    # on an equilibrium at u_w it is a fixed point
+    lat, grp = _lat_all_fluid(4, 4)
+    rng = np.random.default_rng(d)
+    uw = np.array([0.04, 0.01])
+    fe = O.f_eq(uw[None], np.array([1.02]), f64)[0]
+    lat.f[...] = fe
+    line = np.array([[1, 1], [2, 2]])
+    O.BC("velocity_eq", line, d, tuple(uw)).apply(lat)
+    assert np.allclose(lat.f[1, 1, 0], fe, atol=1e-15) and np.allclose(lat.rho[1, 1, 0], 1.02)
+    lat.f[2, 2, 0] = fe + 1e-3 * rng.standard_normal(9)
+    pre = lat.f[2, 2, 0][None].copy()
+    O.BC("velocity_eq", line, d, tuple(uw)).apply(lat)
+    rw = O.rhow(d, uw, pre)[0]
+    assert np.allclose(lat.rho[2, 2, 0], rw)
+    assert np.allclose(lat.f[2, 2, 0], O.f_eq(uw[None], np.array([rw]), f64)[0], atol=1e-16)
+    assert np.allclose(lat.vel[2, 2], uw)
+
+
+# ---------------------------------------------------------------------------
+# Multiphase breadth: C2 adhesion (ExternalForce.py:105-116), the psi
+# boundaries (SCBoundaryCondition.py:19-123), the cubic-EOS pseudopotentials
+# (EOS.py:5-95)
+# ---------------------------------------------------------------------------
+def test_sc_c2_adhesion_flat_wall_closed_form():
+    # a solid row below j = 0 (fluid_strategy false there), uniform psi: at
+    # j = 0 the three populations facing the wall (k = 4, 7, 8) give
+    # sum w_k c_k = (0, -1/6), the five fluid ones (0, +1/6), so
+    # F_c1 = -psi_c1 (G[c1, c2] psi_c2 (0, 1/6) + gadh[c1] (0, -1/6)); nodes
+    # further in see uniform psi and feel nothing
+    NX, NY = 6, 5
+    lat, grp = _lat_all_fluid(NX, NY, nc=2)
+    psi = (0.37, 0.81)
+    lat.psi = np.zeros((NX + 2, NY + 2, 2))
+    lat.psi[..., 0], lat.psi[..., 1] = psi
+    fluid = np.ones((NX + 2, NY + 2), bool)
+    fluid[:, 0] = False                      # halo row j = -1: the wall
+    gadh = (0.25, -0.4)
+    O.ShanChen("c2", grp, G2, (1.0, 1.0), g_adh=gadh, fluid=fluid).cau_scforce(lat)
+    for c1 in range(2):
+        c2 = 1 - c1
+        fy = -psi[c1] * (G2[c1, c2] * psi[c2] / 6 - gadh[c1] / 6)
+        assert np.allclose(lat.SCforce[:, 0, c1, 1], fy, rtol=1e-13)
+        assert np.allclose(lat.SCforce[:, 0, c1, 0], 0, atol=1e-17)
+        assert np.allclose(lat.SCforce[:, 1:, c1, :], 0, atol=1e-16)
+    # the adhesion term does not carry psi(x + c_k) (the factor is commented
+    # out at :114): the wall's own psi does not enter
+    lat.psi[:, 0, :] = 5.0
+    O.ShanChen("c2", grp, G2, (1.0, 1.0), g_adh=gadh, fluid=fluid).cau_scforce(lat)
+    assert np.allclose(lat.SCforce[:, 0, 0, 1], -psi[0] * (G2[0, 1] * psi[1] / 6 - gadh[0] / 6), rtol=1e-13)
+
+
+def test_psi_boundaries():
+    NX, NY = 5, 4
+    lat, grp = _lat_all_fluid(NX, NY, nc=2)
+    I = np.arange(-1, NX + 1)[:, None, None]
+    lat.psi = (0.2 + 0.03 * I + 0.0 * np.arange(NY + 2)[None, :, None] + np.array([0.0, 0.5])).astype(f64)
+    lat.psi = np.broadcast_to(lat.psi, (NX + 2, NY + 2, 2)).copy()
+    right = np.array([[NX, j] for j in range(-1, NY + 1)])           # halo column x = NX, d = 1
+    # extrapolation reproduces a linear profile exactly
+    p0 = lat.psi.copy()
+    lat.psi[NX + 1] = 0.0
+    O.PsiBC("extrapolation", right, 1).apply(lat)
+    assert np.allclose(lat.psi, p0, atol=1e-15)
+    # Neumann copies x - c_d
+    O.PsiBC("neumann", right, 1).apply(lat)
+    assert np.array_equal(lat.psi[NX + 1], lat.psi[NX])
+    # fixed: component 0 only
+    O.PsiBC("fixed", right, None, 0.9).apply(lat)
+    assert np.all(lat.psi[NX + 1, :, 0] == 0.9) and np.array_equal(lat.psi[NX + 1, :, 1], lat.psi[NX, :, 1])
+    # convective: lambda = |mean u(x - c_d)| over the group; a resting fluid
+    # (lambda = 0) leaves psi as it is
+    lat.psi[...] = p0
+    lat.vel[...] = 0.0
+    O.PsiBC("convective1", np.array([[NX - 1, j] for j in range(NY)]), 1).apply(lat)
+    assert np.array_equal(lat.psi, p0)
+    lat.vel[...] = (3.0, 4.0)                                       # lambda = 5
+    O.PsiBC("convective1", np.array([[NX - 1, j] for j in range(NY)]), 1).apply(lat)
+    x, x2 = p0[NX, 1:-1], p0[NX - 1, 1:-1]
+    assert np.allclose(lat.psi[NX, 1:-1], (x + 5 * x2) / 6)
+    lat.psi[...] = p0
+    O.PsiBC("convective2", np.array([[NX - 1, j] for j in range(NY)]), 1).apply(lat)
+    x3 = p0[NX - 2, 1:-1]
+    assert np.allclose(lat.psi[NX, 1:-1], (x + 10 * x2 - 2.5 * x3) / 8.5)
+
+
+# published critical constants of the cubic equations of state (Yuan & Schaefer,
+# Phys. Fluids 18, 042101 (2006)): a = alpha R^2 T_c^m / p_c, b = beta R T_c / p_c
+EOS_CRITICAL = {
+    "vdw": (27 / 64, 2, 1 / 8),
+    "rk": (0.42748, 2.5, 0.08664),
+    "pr": (0.45724, 2, 0.0778),
+    "cs": (0.4963, 2, 0.18727),
+}
+
+
+@pytest.mark.parametrize("kind", sorted(EOS_CRITICAL))
+def test_cubic_eos_critical_point(kind):
+    # at T = T_c the isotherm has an inflection with zero slope at rho_c and
+    # p(rho_c) = p_c (Z_c = p_c / (rho_c R T_c) is the EOS's critical
+    # compressibility): a wrong coefficient or sign in any term breaks one of them
+    alpha, m, beta = EOS_CRITICAL[kind]
+    R, Tc, pc = 1.0, 0.1, 0.02
+    a, b = alpha * R ** 2 * Tc ** m / pc, beta * R * Tc / pc
+    e = O.EOS(kind, a, b, R, -1.0, omega=0.344, T_cr=Tc)
+    rho = np.linspace(0.01, 0.99 / b * (4 if kind == "cs" else 1) * 0.9, 20001)
+    p = e.pressure(rho, np.full_like(rho, Tc))
+    dp = np.gradient(p, rho)
+    d2 = np.gradient(dp, rho)
+    k = int(np.argmin(np.abs(dp)))             # the isotherm's flattest point
+    assert abs(dp[k]) < 1e-3 * np.abs(dp).max()
+    assert abs(d2[k]) < 2e-2 * np.abs(d2).max()
+    assert abs(p[k] / pc - 1) < 2e-2, (kind, p[k], pc)
+    # and psi encodes p: rho/3 + G psi^2 / 6 = p wherever p < rho/3 (G < 0)
+    T = np.full_like(rho, 0.8 * Tc)
+    pr = e.pressure(rho, T)
+    psi = e.get_psi(rho, T)
+    below = pr < rho / 3
+    assert below.any()
+    assert np.allclose((rho / 3 + e.G * psi ** 2 / 6)[below], pr[below], rtol=1e-12, atol=1e-15)
+
+
+def test_peng_robinson_acentric_factor():
+    # omega enters PR only below T_c, through alpha(T); by definition
+    # omega = -1 - log10(p_sat(0.7 T_c) / p_c), and PR's alpha was fitted to
+    # reproduce it: the Maxwell construction on the oracle's PR isotherm at
+    # 0.7 T_c gives back the input omega to a few percent
+    from scipy.optimize import brentq
+    R, Tc, pc, omega = 1.0, 0.1, 0.02, 0.344
+    a, b = 0.45724 * R ** 2 * Tc ** 2 / pc, 0.0778 * R * Tc / pc
+    e = O.EOS("pr", a, b, R, -1.0, omega=omega, T_cr=Tc)
+    T = 0.7 * Tc
+    p = lambda r: float(e.pressure(np.array([r]), np.array([T]))[0])
+    rr = np.linspace(1e-4, 0.98 / b, 40001)
+    pp = e.pressure(rr, np.full_like(rr, T))
+    dp = np.diff(pp)
+    r_max = rr[np.argmax(dp < 0)]               # spinodal bounds of the loop
+    r_min = rr[len(dp) - np.argmax(dp[::-1] < 0)]
+
+    def area(ps):                               # equal areas in the (v, p) plane, v = 1/rho
+        rg = brentq(lambda r: p(r) - ps, 1e-6, r_max)
+        rl = brentq(lambda r: p(r) - ps, r_min, 0.98 / b)
+        v = np.linspace(1 / rl, 1 / rg, 20001)
+        pv = e.pressure(1 / v, np.full_like(v, T))
+        return np.trapezoid(pv - ps, v)
+    lo, hi = max(p(r_min), p(1e-6)) * 1.01, p(r_max) * 0.9999
+    ps = brentq(area, lo, hi)
+    assert abs((-1 - np.log10(ps / pc)) / omega - 1) < 0.05
+    # a dropped omega term moves it far off
+    e0 = O.EOS("pr", a, b, R, -1.0, omega=0.0, T_cr=Tc)
+    assert abs(float(e0.pressure(np.array([0.5 / b]), np.array([T]))[0]) - p(0.5 / b)) > 1e-3 * pc
