@@ -1134,6 +1134,194 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
   store_macro<NC, R>(a, (R*)a.mac[pset], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
+// ---------------------------------------------------------------------------
+// Shan-Chen, two steps per launch.
+//
+// Step n+1 on a tile needs psi of step n+1 on the tile and a one-node ring
+// (R1), i.e. the post-stream populations of step n+1 there, i.e. step n
+// computed on a two-node ring (R2), whose force needs psi of step n on a
+// three-node ring (R3), i.e. the post-stream populations of step n gathered
+// on R3.  A block owns one thread per node of R3 and runs:
+//   1. (R3) gather f_n from HBM (pull, bounce-back), psi_n -> LDS;
+//   2. (R2) step n (force from LDS psi_n), its post-collision populations
+//      into the LDS slots their pulls read, then the reflections of bounced
+//      populations (as the two-step kernel above);
+//   3. (R1) pull f_{n+1} from LDS, psi_{n+1} -> LDS (over psi_n, now dead);
+//   4. (tile) step n+1 (force from LDS psi_{n+1}) -> HBM.
+// Per pair of steps HBM sees the gathers of R3 (the rings are mostly L2
+// hits) and one write, instead of two reads and two writes.  The per-node
+// arithmetic is step_math's, with psi summed in the same order as k_sc: the
+// fields are bit-identical to two k_sc launches.  No node rules, no power
+// law (their lattices keep one step per launch).
+// ---------------------------------------------------------------------------
+template <int NC, int NT>
+struct ScTwoLds {
+  using Shape = TileShape<3, kScCols, NT>;
+  float f[NC * 9][Shape::PN];   // post-collision populations of step n, at the slots their pulls read
+  float psi[NC][Shape::PN];     // psi of step n (phases 1-2), then of step n+1 (phases 3-4)
+  uint8_t sol[Shape::PN];       // adhesion flags (LBM_CODE_SC_SOLID)
+};
+
+template <int COLL, int NC, bool BUF, int NT>
+__global__ __launch_bounds__(NT) void k_sc2(const Lat a, const Launch2 q) {
+  using Shape = TileShape<3, kScCols, NT>;
+  using R = float;
+  using T = typename Pop<LBM_STORE_F32>::T;
+  constexpr int S = LBM_STORE_F32;
+  __shared__ ScTwoLds<NC, NT> L;
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  const uint32_t i0 = ti * Shape::TI, j0 = tj * kScCols;
+  const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
+  const uint32_t r1 = threadIdx.x;
+  const uint32_t rr = r1 / (uint32_t)Shape::RJ, cc = r1 - rr * (uint32_t)Shape::RJ;
+  const uint32_t qs = Shape::slot(rr, cc);
+  // distance of this node from the tile (0: a tile node; 1..3: ring)
+  const int dr = rr < 3u ? 3 - (int)rr : ((int)rr >= 3 + Shape::TI ? (int)rr - (2 + Shape::TI) : 0);
+  const int dc = cc < 3u ? 3 - (int)cc : ((int)cc >= 3 + kScCols ? (int)cc - (2 + kScCols) : 0);
+  const int ring = dr > dc ? dr : dc;
+  const bool has = r1 < (uint32_t)Shape::RN;
+  // psi plane sets: step n's psi goes to set `parity`, step n+1's to the set
+  // of the launch's result (parity ^ 1), so that the newest psi and the
+  // newest populations share an index and the psi of the step before is in
+  // the other set, as after one-step launches; inactive nodes hold equal sets
+  const uint32_t pa = q.parity, pb = q.parity ^ 1u;
+  const bool local = q.local_read != 0;
+  const T* __restrict__ src = (const T*)a.pop[q.parity];
+  const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+  R f[NC][9];
+  uint32_t code = 0, gi = 0, gj = 0, n = 0;
+  // 1. gather f_n on R3, psi_n
+  __builtin_amdgcn_s_setprio(3);
+  if (has) {
+    ring_coords<3>(rr, cc, i0, j0, nx, ny, gi, gj);
+    n = gi * ny + gj;
+    code = a.code[n];
+    if (a.sc_adhesion) L.sol[qs] = (uint8_t)(code >> 15);
+    if (code & LBM_CODE_ACTIVE) {
+      if (local) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) f[c][k] = Pop<S>::ld(src + (size_t)(c * 9 + k) * ps, n, k);
+      } else if constexpr (BUF) {
+        gather_buf<NC, S>(rsrc, ps, nx, ny, gi, gj, code, f);
+      } else {
+        gather<NC, S>(src, ps, Nbr(gi, gj, nx, ny), code & LBM_CODE_BOUNCE_MASK, f);
+      }
+      __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s += f[c][k];
+        L.psi[c][qs] = node_psi<R>(a, c, s, n);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) L.psi[c][qs] = ((const R*)a.psi[pa])[(size_t)c * ps + n];
+    }
+  }
+  __builtin_amdgcn_s_setprio(0);
+  __syncthreads();
+  // 2. step n on R2
+  R refl[NC][9];
+  const bool act = (code & LBM_CODE_ACTIVE) != 0;
+  if (has && ring <= 2) {
+    if (act) {
+      R st_in[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        st_in[c] = a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
+      StepOut<NC, R> o;
+      step_math<COLL, NC, R>(a, PsiTile<R, Shape>{&L.psi[0][0], L.sol, qs}, n, gi, gj, f, st_in, o);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          L.f[c * 9 + k][qs] = o.f[c][k];
+          refl[c][k] = o.f[c][opp(k)];
+        }
+      }
+      // psi of step n of the tile's nodes (the run's second-last step); tile
+      // slots past the lattice's last row / column wrap onto other tiles' nodes
+      if (ring == 0 && q.write_psi && i0 + rr - 3 < nx && j0 + cc - 3 < ny) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ((R*)a.psi[pa])[(size_t)c * ps + n] = L.psi[c][qs];
+      }
+    } else if (code & LBM_CODE_PULLED) {
+      // an inactive node that an active node pulls: its unchanged f2
+      const T* stale = (const T*)a.pop[q.parity ^ 1u];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) L.f[c * 9 + k][qs] = Pop<S>::ld(stale + (size_t)(c * 9 + k) * ps, n, k);
+    }
+  }
+  __syncthreads();
+  if (has && ring <= 2 && act && (code & LBM_CODE_BOUNCE_MASK)) {
+#pragma unroll
+    for (int k = 1; k < 9; ++k) {
+      if (code & (1u << (k - 1))) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) L.f[c * 9 + k][qs - Shape::back(k)] = refl[c][k];
+      }
+    }
+  }
+  __syncthreads();
+  // 3. pull f_{n+1} on R1, psi_{n+1}
+  if (has && ring <= 1) {
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R s = R(0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          f[c][k] = L.f[c * 9 + k][qs - Shape::back(k)];
+          s += f[c][k];
+        }
+        L.psi[c][qs] = node_psi<R>(a, c, s, n);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) L.psi[c][qs] = ((const R*)a.psi[pb])[(size_t)c * ps + n];
+    }
+  }
+  __syncthreads();
+  // 4. step n+1 on the tile
+  if (!has || ring != 0) return;
+  if (i0 + rr - 3 >= nx || j0 + cc - 3 >= ny) return;   // past the lattice's last tile row / column
+  if (!act) {
+    if (q.write_macro) {
+      R* mo = (R*)a.mac[pb];
+#pragma unroll
+      for (int c = 0; c < NC + 3; ++c) mo[(size_t)c * ps + n] = R(0);
+    }
+    return;
+  }
+  R st_in[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) st_in[c] = a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0);
+  StepOut<NC, R> o;
+  step_math<COLL, NC, R>(a, PsiTile<R, Shape>{&L.psi[0][0], L.sol, qs}, n, gi, gj, f, st_in, o);
+  T* __restrict__ dst = (T*)a.pop[pb];
+  const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if constexpr (BUF)
+        Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
+      else
+        Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+    }
+    if (q.write_psi) ((R*)a.psi[pb])[(size_t)c * ps + n] = L.psi[c][qs];
+  }
+  store_macro<NC, R>(a, (R*)a.mac[pb], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
+}
+
 // One tile of a batched two-step launch (32 bytes): what the tile's first
 // loads need, so that the node codes are fetched together with the
 // lattice's descriptor instead of after it.
@@ -1661,6 +1849,68 @@ ScFn pick_sc(int nc, int store, int coll, bool buf) {
   return nc == 1 ? pick_sc_coll<1, LBM_STORE_F32, false>(coll) : pick_sc_coll<2, LBM_STORE_F32, false>(coll);
 }
 
+// Two Shan-Chen steps per launch (k_sc2): FP32 storage, no node rules, no
+// power law.  Threads per block: one per node of the three-node-ring tile.
+#ifndef LBM_SC2_THREADS
+#define LBM_SC2_THREADS 1024
+#endif
+constexpr int kSc2Threads = LBM_SC2_THREADS;
+template <int NC, bool BUF>
+ScFn pick_sc2_coll(int coll) {
+  switch (coll) {
+    case LBM_BGK: return k_sc2<LBM_BGK, NC, BUF, kSc2Threads>;
+    case LBM_TRT: return k_sc2<LBM_TRT, NC, BUF, kSc2Threads>;
+    default: return k_sc2<LBM_MRT, NC, BUF, kSc2Threads>;
+  }
+}
+ScFn pick_sc2(const lbm_lattice* lat) {
+  if (lat->storage != LBM_STORE_F32 || lat->n_neem != 0 || lat->tau_mode == LBM_TAU_POWER_LAW) return nullptr;
+  const bool buf = fits_buffer(lat);
+  if (buf) return lat->nc == 1 ? pick_sc2_coll<1, true>(lat->collision) : pick_sc2_coll<2, true>(lat->collision);
+  return lat->nc == 1 ? pick_sc2_coll<1, false>(lat->collision) : pick_sc2_coll<2, false>(lat->collision);
+}
+
+// Shan-Chen steps two per launch where k_sc2 takes the lattice (a last odd
+// step, and lattices it does not take, through run_sc); false where neither
+// kernel is built.  *out_parity: the buffer holding the newest f2.
+bool run_sc2(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
+             hipStream_t s, int* rc, int32_t* out_parity) {
+  ScFn fn2 = pick_sc2(lat);
+  const int pairs = fn2 ? nsteps / 2 : 0;
+  uint32_t p = parity;
+  *rc = LBM_OK;
+  if (pairs > 0) {
+    using Shape = TileShape<3, kScCols, kSc2Threads>;
+    Launch2 q;
+    memset(&q, 0, sizeof q);
+    q.tiles_i = (a.nx + Shape::TI - 1) / Shape::TI;
+    q.tiles_j = (a.ny + kScCols - 1) / kScCols;
+    const uint64_t nt = (uint64_t)q.tiles_i * q.tiles_j;
+    if (nt >= (1ull << 31)) {
+      *rc = fail("lattice too large for the Shan-Chen tiling");
+      return true;
+    }
+    q.xcd_remap = (nt % 8 == 0 && nt >= 64) ? 1 : 0;
+    for (int t = 0; t < pairs; ++t) {
+      q.parity = p;
+      q.local_read = (t == 0 && first_local) ? 1 : 0;
+      q.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
+      q.write_psi = (t == pairs - 1) ? 1 : 0;
+      hipLaunchKernelGGL(fn2, dim3((unsigned)nt), dim3(kSc2Threads), 0, s, a, q);
+      if ((*rc = hip_check(hipGetLastError(), "Shan-Chen two-step launch"))) return true;
+      p ^= 1u;
+    }
+  }
+  const int rest = nsteps - 2 * pairs;
+  if (rest > 0) {
+    if (!run_sc(lat, a, rest, p, (pairs == 0 && first_local) ? 1 : 0, s, rc)) return false;
+    if (*rc) return true;
+    p ^= (uint32_t)(rest & 1);
+  }
+  *out_parity = (int32_t)p;
+  return true;
+}
+
 bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
             hipStream_t s, int* rc) {
   ScFn fn = pick_sc(lat->nc, lat->storage, lat->collision, fits_buffer(lat));
@@ -1710,6 +1960,10 @@ int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_ru
   const bool buf = fits_buffer(lat);
   const int w = buf ? tile_cols_of(tile_cols) : 64;
   Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, w, buf);
+  if (lat->shan_chen != LBM_SC_NONE) {
+    const Lat a = make_lat(lat, fld, neem);
+    if (run_sc2(lat, a, nsteps, (uint32_t)parity, first_local, s, &rc, out_parity)) return rc;
+  }
   const bool ok2 = two && lat->shan_chen == LBM_SC_NONE && (!pl || tau_next);
   if (!ok2) {
     rc = lbm_run(lat, fld, neem, nsteps, parity, first_local, stream);

@@ -423,6 +423,23 @@ def contraction(W=512, H=128, power_n=1.0, u_in=0.02, nu_ref=0.1, collision="trt
                           stream=stream, outlet=outlet if isinstance(outlet, str) else outlet.kind))
 
 
+# The serving workload's request: a 4:1 power-law contraction whose free
+# parameters are chosen so that small requests reach a steady state within
+# the 50k-step budget: an anti-bounce-back pressure outlet (PressureABC) holds
+# the outlet density (with PressureNEEM the channel's mass drifts for good),
+# u_in = 0.02 keeps the FP32 rounding jitter of the velocity (about 2e-8)
+# below the 1e-6 relative residual, and nu_ref = 0.4 with tau >= 0.55 keeps
+# the shear-thinning corners of the largest requests stable (with nu_ref =
+# 0.3 the 1840 x 455 and 2048 x 512 requests at n = 0.5 diverge).
+SERVING_CONTRACTION = dict(u_in=0.02, nu_ref=0.4, collision="trt", magic=0.25, tau_min=0.55, tau_max=2.0,
+                           outlet="abc")
+
+
+def serving_contraction(W, H, power_n, name="request"):
+    """One request of the serving workload (SERVING_CONTRACTION)."""
+    return contraction(W=W, H=H, power_n=power_n, name=name, **SERVING_CONTRACTION)
+
+
 def cavity(N=256, power_n=1.0, Re=10.0, u_lid=5e-3, collision="mrt", magic=0.25, name="cavity"):
     """Power-law lid-driven cavity of cases/Power Law Fluid.py:14-141: an
     N x N lattice without solids, InsideBoundary over the interior-neighbour

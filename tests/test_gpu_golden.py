@@ -140,11 +140,12 @@ def test_fp32_batch_of_all_fixtures():
             check("vel", out["vel"][_s(fx, "out32")], fx["out32_vel"])
 
 
-@pytest.mark.parametrize("storage,bound", [("f16", 2e-4), ("bf16", 2e-3)])
+# measured at most 4.7e-5 (FP16) and 3.3e-4 (BF16, above the 1e-4 bar) over the fixtures
+@pytest.mark.parametrize("storage,bound", [("f16", 7e-5), ("bf16", 5e-4)])
 def test_shifted_storage_against_fixture(storage, bound):
     """FP16 / BF16 shifted population storage (f - w stored, arithmetic in
-    FP32): velocity within the same bound against the FP64 vectors as the
-    batch tests allow after 200 steps."""
+    FP32): velocity against the FP64 vectors, within about 1.5 times the
+    largest error measured over the fixtures."""
     from golden.make_golden import build
     from taichi_lbm.batch import BatchRunner, spec_from_case
     for name in [n for n in NAMES if not n.startswith("shan_chen")]:

@@ -64,7 +64,9 @@ def test_shifted_storage_runs_in_the_server():
             u64, _, _ = solve_alone(specs[r.rid], 600, 0.0, 100, "f64")
             err = np.abs(r.result - u64).max()
             print(f"  {storage} request {r.rid}: max |u - u_fp64| = {err:.2e}")
-            assert err < (3e-4 if storage == "f16" else 3e-3)
+            # measured 7.7e-5 / 3.9e-5 (FP16) and 1.6e-3 / 9.6e-4 (BF16): BF16's 8-bit
+            # mantissa misses the 1e-4 bar on these flows, FP16 meets it
+            assert err < (1.2e-4 if storage == "f16" else 2.4e-3)
 
 
 def test_diverged_solve_is_retired_without_disturbing_the_mix():
@@ -90,3 +92,27 @@ def test_diverged_solve_is_retired_without_disturbing_the_mix():
     assert steps == 2000 and np.array_equal(u, done[1].result)
     u, steps, res = solve_alone(bad, 2000, 0.0, 40, "f32")
     assert steps == done[0].steps and np.isnan(res)
+
+
+def test_serving_request_converges_and_retires_early():
+    """A small request of the serving workload (cases.serving_contraction:
+    PressureABC outlet, u_in = 0.02) reaches the 1e-6 residual well within
+    the 50k-step budget and is retired at once; its velocity equals the same
+    solve alone bit for bit and the FP64 path's at that step count within
+    1e-5 lattice units."""
+    with contextlib.redirect_stdout(io.StringIO()):
+        spec = spec_from_case(cases.serving_contraction(256, 64, 0.5, name="small"))
+        other = spec_from_case(cases.serving_contraction(320, 80, 1.0, name="other"))
+    reqs = [Request(0, 0.0, spec, max_steps=50_000, tol=1e-6), Request(1, 0.0, other, max_steps=4000, tol=0.0)]
+    srv = SolveServer(storage="f32", hbm_budget_bytes=1 << 28, max_slots=4, tick_steps=200)
+    stats = srv.serve(reqs)
+    srv.close()
+    r = {q.rid: q for q in stats.completed}[0]
+    print(f"  converged {r.converged} after {r.steps} steps, residual {r.residual:.2e}", flush=True)
+    assert r.converged and r.steps < 50_000
+    u, steps, res = solve_alone(spec, 50_000, 1e-6, 200, "f32")
+    assert steps == r.steps and np.array_equal(u, r.result)
+    u64, _, _ = solve_alone(spec, r.steps, 0.0, 200, "f64")
+    err = np.abs(u - u64).max()
+    print(f"  max |u - u_fp64| at convergence = {err:.2e} (max |u| {np.abs(u64).max():.3f})", flush=True)
+    assert err < 1e-5
