@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 11
+#define LBM_ABI_VERSION 12
 
 /* status codes */
 #define LBM_OK 0
@@ -144,7 +144,13 @@ typedef struct lbm_lattice {
   float g_adh[LBM_MAX_NC];     /* ShanChenForceC2 gadh (ExternalForce.py:105-116) */
   int32_t sc_adhesion;         /* 1: some node is not fluid for the Shan-Chen   *
                                 * force (LBM_CODE_SC_SOLID / lbm_gfields.sc_fluid) */
-  int32_t pad0;
+  int32_t huang;               /* 1: HuangMRTCollision (CollisionEngine.py:453-515): *
+                                * MRT with the source S Q, Q from the component's   *
+                                * force and psi; needs LBM_MRT and Shan-Chen         */
+  float huang_k1[LBM_MAX_NC];  /* k1                                               */
+  float huang_k2[LBM_MAX_NC];  /* k2 = epsilon / (-8) - k1 (:462)                  */
+  float huang_g[LBM_MAX_NC];   /* G of the component with itself (:463)            */
+  int32_t pad1;
 } lbm_lattice;
 
 /* Device buffers of one lattice (see the plane layout above). */

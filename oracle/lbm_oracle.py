@@ -236,6 +236,9 @@ class Collision:
     nu_bulk: Optional[Sequence[float]] = None
     tau_min: float = TAU_MIN
     tau_max: float = TAU_MAX
+    # HuangMRTCollision (CollisionEngine.py:453-515): per component k1,
+    # epslion and G (the coupling of the component with itself)
+    huang: Optional[dict] = None
     # state
     tau: Optional[np.ndarray] = None          # BGK tau / TRT tau_sym
     tau_antisym: Optional[np.ndarray] = None  # TRT
@@ -347,6 +350,12 @@ class Collision:
                 a = S * (m - meq)
                 b = (1.0 - S / 2.0) * mF
                 m2 = m - a + b
+                if self.huang is not None:   # HuangMRTCollision.apply, :503-507
+                    h = self.huang
+                    k1 = dt.type(h["k1"][c])
+                    k2 = dt.type(h["epslion"][c]) / dt.type(-8.0) - k1
+                    Q = m_q(F, lat.psi[I + 1, J + 1, c], k1, k2, dt.type(h["g"][c]), dt)
+                    m2 = m2 + S * Q
                 lat.f2[I, J, c, :] = minv_apply(m2, dt)
 
 
@@ -390,6 +399,19 @@ def m_eq(vel, rho, dt):
     meq[:, 7] = jx ** 2 - jy ** 2
     meq[:, 8] = jx * jy
     return meq * rho[:, None]
+
+
+def m_q(F, psi, k1, k2, g, dt):
+    """HuangMRTCollision.m_Q, CollisionEngine.py:466-478."""
+    Q = np.zeros((F.shape[0], 9), dt)
+    F2 = F[:, 0] * F[:, 0] + F[:, 1] * F[:, 1]
+    a = 3 * (k1 + 2 * k2) * F2
+    b = g * psi ** 2
+    Q[:, 1] = a / b
+    Q[:, 2] = -Q[:, 1]
+    Q[:, 7] = k1 * (F[:, 0] ** 2 - F[:, 1] ** 2) / b
+    Q[:, 8] = k1 * F[:, 0] * F[:, 1] / b
+    return Q
 
 
 def m_force(F, vel, dt):

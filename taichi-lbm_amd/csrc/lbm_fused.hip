@@ -35,6 +35,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,8 @@ struct Lat {
   float g_adh[LBM_MAX_NC];
   int32_t sc_adhesion;         // some node carries LBM_CODE_SC_SOLID
   const float* temperature;    // lb_field.T (cubic-EOS psi)
+  int32_t huang;               // HuangMRTCollision: MRT with the source S Q (huang_q)
+  float huang_k1[LBM_MAX_NC], huang_k2[LBM_MAX_NC], huang_g[LBM_MAX_NC];
   lbm_neem_rule neem[LBM_MAX_NEEM];
 };
 
@@ -497,7 +500,16 @@ __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t 
       }
     }
     if (is_mrt) {
-      mrt_collide_m(mom[c], nq, o.f[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, s_shear, s_bulk);
+      // HuangMRTCollision only with a Shan-Chen force (its psi): kernels
+      // without one carry no trace of it
+      constexpr bool may_huang = !std::is_same<PS, PsiNone<R>>::value;
+      if (may_huang && a.huang) {
+        const HuangQ<R> hq =
+            huang_q<R>(o.Fx[c], o.Fy[c], psi.at(c, 0), (R)a.huang_k1[c], (R)a.huang_k2[c], (R)a.huang_g[c]);
+        mrt_collide_m(mom[c], nq, o.f[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, s_shear, s_bulk, &hq);
+      } else {
+        mrt_collide_m(mom[c], nq, o.f[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, s_shear, s_bulk);
+      }
     } else if constexpr (COLL >= 0) {
       collide<COLL, R>(f[c], o.f[c], o.rho[c], o.ux, o.uy, o.Fx[c], o.Fy[c], force, tau, tau_anti, s_shear, s_bulk);
     } else if (is_trt) {
@@ -1564,6 +1576,10 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
     }
     if (lat->sc_adhesion && lat->shan_chen != LBM_SC_C2) return fail("adhesion needs ShanChenForceC2");
   }
+  if (lat->huang) {
+    if (lat->collision != LBM_MRT) return fail("HuangMRTCollision needs the MRT collision");
+    if (lat->shan_chen == LBM_SC_NONE) return fail("HuangMRTCollision needs a Shan-Chen force (its psi)");
+  }
   return LBM_OK;
 }
 
@@ -1606,6 +1622,12 @@ Lat make_lat(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
   }
   a.sc_adhesion = lat->sc_adhesion;
   a.temperature = fld->temperature;
+  a.huang = lat->huang;
+  for (int c = 0; c < LBM_MAX_NC; ++c) {
+    a.huang_k1[c] = lat->huang_k1[c];
+    a.huang_k2[c] = lat->huang_k2[c];
+    a.huang_g[c] = lat->huang_g[c];
+  }
   for (int q = 0; q < LBM_MAX_NC * LBM_MAX_NC; ++q) a.g_coh[q] = lat->g_coh[q];
   a.visc_m = lat->visc_m;
   a.visc_nm1 = lat->visc_nm1;

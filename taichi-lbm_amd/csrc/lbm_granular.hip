@@ -31,6 +31,8 @@ struct G {
   float eos[LBM_MAX_NC][6];
   float g_adh[LBM_MAX_NC];
   int32_t sc_adhesion;
+  int32_t huang;       // HuangMRTCollision
+  float huang_k1[LBM_MAX_NC], huang_k2[LBM_MAX_NC], huang_g[LBM_MAX_NC];
   lbm_neem_rule rule;
   int32_t kind;
   int32_t direction;   // psi boundaries
@@ -253,6 +255,16 @@ __global__ void k_collide(const G a) {
       mF[8] = Fy * ux + Fx * uy;
 #pragma unroll
       for (int r = 0; r < 9; ++r) m2[r] = m[r] - S[r] * (m[r] - meq[r]) + (1.0f - S[r] / 2.0f) * mF[r];
+      if (a.huang) {
+        // HuangMRTCollision.apply (CollisionEngine.py:480-515): m2 + S Q with
+        // the component's psi of this step (psi_field[i, j, c])
+        const float psi = a.g.psi[(size_t)c * a.hs + hidx(a, (int)i, (int)j)];
+        const HuangQ<float> q = huang_q<float>(Fx, Fy, psi, a.huang_k1[c], a.huang_k2[c], a.huang_g[c]);
+        m2[1] = m2[1] + S[1] * q.q1;
+        m2[2] = m2[2] - S[2] * q.q1;
+        m2[7] = m2[7] + S[7] * q.q7;
+        m2[8] = m2[8] + S[8] * q.q8;
+      }
       mrt_minv(m2, out);
     }
 #pragma unroll
@@ -559,6 +571,12 @@ G make(const lbm_lattice* lat, const lbm_gfields* g, const int32_t* group, int32
     for (int q = 0; q < 6; ++q) a.eos[c][q] = lat->eos[c][q];
   }
   a.sc_adhesion = lat->sc_adhesion;
+  a.huang = lat->huang;
+  for (int c = 0; c < LBM_MAX_NC; ++c) {
+    a.huang_k1[c] = lat->huang_k1[c];
+    a.huang_k2[c] = lat->huang_k2[c];
+    a.huang_g[c] = lat->huang_g[c];
+  }
   for (int q = 0; q < LBM_MAX_NC * LBM_MAX_NC; ++q) a.g_coh[q] = lat->g_coh[q];
   return a;
 }
@@ -641,6 +659,10 @@ extern "C" int lbm_g_collide(const lbm_lattice* lat, const lbm_gfields* g, const
                              void* stream) {
   GRANULAR_PROLOGUE
   if (!g->tau) return fail("collide needs the relaxation-time field");
+  if (lat->huang) {
+    if (lat->collision != LBM_MRT) return fail("HuangMRTCollision needs the MRT collision");
+    if (!g->psi) return fail("HuangMRTCollision.apply needs the Shan-Chen psi field");
+  }
   return run_group(k_collide, a, s);
 }
 

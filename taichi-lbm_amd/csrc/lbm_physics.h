@@ -100,9 +100,26 @@ __device__ __forceinline__ MrtNeq<R> mrt_neq(const R* m, R rho, R ux, R uy) {
 
 // MRT collision from the moments: S = diag(0, s_bulk, 1, 0, 1, 0, 1, s, s)
 // (CollisionEngine.py:301-318), Guo forcing (1 - S/2) M F in moment space.
+// HuangMRTCollision.m_Q (CollisionEngine.py:466-478): the source moments
+// Q1 = 3 (k1 + 2 k2) |F|^2 / (G psi^2), Q2 = -Q1, Q7 = k1 (Fx^2 - Fy^2) /
+// (G psi^2), Q8 = k1 Fx Fy / (G psi^2), F the component's body force, psi
+// its own psi and G its coupling with itself; the collision adds S Q
+// (:507), S the node's MRT rates (the reference indexes the rates per
+// component only, :507, which the per-node MRT state does not have).
+template <typename R>
+struct HuangQ {
+  R q1, q7, q8;
+};
+template <typename R>
+__device__ __forceinline__ HuangQ<R> huang_q(R Fx, R Fy, R psi, R k1, R k2, R g) {
+  const R F2 = Fx * Fx + Fy * Fy;
+  const R b = g * (psi * psi);
+  return HuangQ<R>{R(3) * (k1 + R(2) * k2) * F2 / b, k1 * (Fx * Fx - Fy * Fy) / b, k1 * Fx * Fy / b};
+}
+
 template <typename R>
 __device__ __forceinline__ void mrt_collide_m(const R* m, const MrtNeq<R>& q, R (&out)[9], R ux, R uy, R Fx, R Fy,
-                                              bool force, R s_shear, R s_bulk) {
+                                              bool force, R s_shear, R s_bulk, const HuangQ<R>* hq = nullptr) {
   R m2[9];
   m2[0] = m[0];
   m2[1] = m[1] - s_bulk * q.n1;
@@ -123,6 +140,13 @@ __device__ __forceinline__ void mrt_collide_m(const R* m, const MrtNeq<R>& q, R 
     m2[6] = m2[6] + R(0.5) * (-Fy);
     m2[7] = m2[7] + (R(1.0) - s_shear / R(2.0)) * (R(2.0) * (Fx * ux - Fy * uy));
     m2[8] = m2[8] + (R(1.0) - s_shear / R(2.0)) * (Fy * ux + Fx * uy);
+  }
+  if (hq) {
+    // HuangMRTCollision: + S Q (S_2 = 1)
+    m2[1] = m2[1] + s_bulk * hq->q1;
+    m2[2] = m2[2] - hq->q1;
+    m2[7] = m2[7] + s_shear * hq->q7;
+    m2[8] = m2[8] + s_shear * hq->q8;
   }
   mrt_minv(m2, out);
 }

@@ -251,9 +251,55 @@ class MRTCollision(Collision):
             L.s_bulk[c] = float(sb)
 
 
-class HuangMRTCollision:
-    """CollisionEngine.py:453-515 -- not part of the hot path (its constructor is
-    unusable in the reference: :455-456, :461)."""
+class HuangMRTCollision(MRTCollision):
+    """CollisionEngine.py:453-515: MRT with the source moments of m_Q
+    (:466-478) added to the collision as S Q (:503-507):
+    Q1 = 3 (k1 + 2 k2) |F|^2 / (G psi^2), Q2 = -Q1,
+    Q7 = k1 (Fx^2 - Fy^2) / (G psi^2), Q8 = k1 Fx Fy / (G psi^2),
+    with k2 = epslion / (-8) - k1 (:462), G = g_coh[c][c] (:463), F the
+    component's body force and psi its Shan-Chen psi of the step.
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("HuangMRTCollision is not supported")
+    The reference class cannot run: its constructor passes
+    (num_components, group) to MRTCollision's params-dict constructor
+    (:455) and apply indexes the per-node rates diag[i, j, c] as
+    diag[c, ii] (:507).  Here the constructor takes MRTCollision's params
+    plus 'k1', 'epslion' (one value per component) and 'g_coh' (the
+    coupling matrix), and S is the node's MRT rates.  apply(lb_field,
+    sc_field) needs the Shan-Chen field's psi; the fused path takes it
+    from the Shan-Chen force of the run."""
+
+    def __init__(self, params: dict):
+        for key in ("k1", "epslion", "g_coh"):
+            if params.get(key) is None:
+                raise ValueError(f"HuangMRTCollision needs '{key}'")
+        super().__init__(params)
+        self.k1 = np.array([f32(v) for v in params["k1"]], f32)
+        eps = np.array([f32(v) for v in params["epslion"]], f32)
+        self.epslion = [float(v) for v in eps]
+        self.k2 = (eps / f32(-8.0) - self.k1).astype(f32)
+        g = np.asarray(params["g_coh"], f32).reshape(self.nc, self.nc) if np.ndim(params["g_coh"]) \
+            else np.full((self.nc, self.nc), f32(params["g_coh"]), f32)
+        self.g_coh = np.array([g[c, c] for c in range(self.nc)], f32)
+        if np.any(self.g_coh == 0):
+            raise ValueError("HuangMRTCollision divides by G psi^2: g_coh[c][c] must be non-zero")
+        if len(self.k1) != self.nc or len(eps) != self.nc:
+            raise ValueError("k1 and epslion need one value per component")
+
+    def fill_lattice(self, L):
+        super().fill_lattice(L)
+        L.huang = 1
+        for c in range(self.nc):
+            L.huang_k1[c] = float(self.k1[c])
+            L.huang_k2[c] = float(self.k2[c])
+            L.huang_g[c] = float(self.g_coh[c])
+
+    def apply(self, lb_field, sc_field=None):
+        """Collide f -> f2 with the Huang source (CollisionEngine.py:480-515)."""
+        sc = sc_field if sc_field is not None else lb_field.sc_field
+        if sc is None:
+            raise ValueError("HuangMRTCollision.apply needs the Shan-Chen field (its psi)")
+        lb_field.materialize()
+        g = group_device(self.group)
+        H.check(H.lib().lbm_g_collide(ctypes.byref(lattice_struct(lb_field, collision=self, sc=sc)),
+                                      ctypes.byref(gfields(lb_field, collision=self, sc=sc)), g[0], g[1],
+                                      stream()), "HuangMRTCollision.apply")

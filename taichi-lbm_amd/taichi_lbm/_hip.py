@@ -31,7 +31,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 
 class NeemRule(C.Structure):
@@ -55,7 +55,9 @@ class Lattice(C.Structure):
         ("psi_kind", C.c_int32 * MAX_NC),
         ("eos", (C.c_float * 6) * MAX_NC),
         ("g_adh", C.c_float * MAX_NC),
-        ("sc_adhesion", C.c_int32), ("pad0", C.c_int32),
+        ("sc_adhesion", C.c_int32), ("huang", C.c_int32),
+        ("huang_k1", C.c_float * MAX_NC), ("huang_k2", C.c_float * MAX_NC), ("huang_g", C.c_float * MAX_NC),
+        ("pad1", C.c_int32),
     ]
 
 

@@ -234,7 +234,7 @@ def stepper(case):
 
 
 def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt", nu=1.0 / 6.0, rho_mean=None,
-              amp=None, name="shan_chen", disk=None, gadh=None, psi_wall=None, eos=None):
+              amp=None, name="shan_chen", disk=None, gadh=None, psi_wall=None, eos=None, huang=None):
     """Periodic Shan-Chen lattice.  nc=1: ShanChenForceC1 with cohesion G, rho =
     rho_mean + amp*U(0,1) (cases/Bubble or two-phase separation.py:174);
     nc=2: ShanChenForceC2 with G = [[0, gcross], [gcross, 0]] and each rho_c =
@@ -246,7 +246,9 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
     (nc=2): ShanChenForceC2 adhesion at the disk, fluid_strategy = mask == 1
     (periodic on the halo; ExternalForce.py:105-116).  eos=(kind, params)
     with kind "vdw" | "rk" | "pr" | "cs": the cubic-EOS pseudopotential of
-    EOS.py in place of SC_psi, at T = 1."""
+    EOS.py in place of SC_psi, at T = 1.  huang=(k1, epslion): the MRT
+    collision with Huang's source (HuangMRTCollision, the same k1 and
+    epslion for every component, G = the component's own coupling)."""
     from .EOS import CS_psi, PR_psi, RK_psi, SC_psi, vdW_psi
     from .ExternalForce import ShanChenForceC1, ShanChenForceC2
     from .SCBoundaryCondition import BoundaryEngineSC, FixedPsi, PeriodicPsi_BT, PeriodicPsi_LR
@@ -276,6 +278,14 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
         bce.add_boundary_condition("wall", wall)
     lbf.neighbor_classify()
     coll = make_collision(collision, fluid.group, NX, NY, nc, [nu] * nc, nu_bulk=[nu] * nc)
+    if huang is not None:
+        from .CollisionEngine import HuangMRTCollision
+        if collision != "mrt":
+            raise ValueError("HuangMRTCollision is an MRT collision")
+        Gself = np.diag([G] * nc) if nc == 1 else np.array([[G, gcross], [gcross, G]])
+        coll = HuangMRTCollision({'group': fluid.group, 'fluid_model': coll.fluid_model, 'NX': NX, 'NY': NY,
+                                  'num_components': nc, 'shearviscosity': [nu] * nc, 'bulkviscosity': [nu] * nc,
+                                  'k1': [huang[0]] * nc, 'epslion': [huang[1]] * nc, 'g_coh': Gself})
     rng = np.random.default_rng(seed)
     rho = np.zeros((NX, NY, nc), np.float32)
     if nc == 1:
@@ -313,7 +323,8 @@ def shan_chen(NX=128, NY=128, nc=1, G=-5.0, gcross=0.9, seed=0, collision="mrt",
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
                 psi=psi, bsc=bsc, sc_spec=dict(kind="c1" if nc == 1 else "c2", g_coh=g_spec, rho0=(1.0,) * nc),
                 spec=dict(kind="shan_chen", NX=NX, NY=NY, nc=nc, G=G, gcross=gcross, seed=seed,
-                          collision=collision, nu=nu, disk=disk, gadh=gadh, psi_wall=psi_wall, eos=eos))
+                          collision=collision, nu=nu, disk=disk, gadh=gadh, psi_wall=psi_wall, eos=eos,
+                          huang=huang))
 
 
 

@@ -38,6 +38,9 @@ def oracle_solver(case, dtype=np.float32, rho=None, vel=None, f=None, psi=None, 
         magic = [_r32(v) for v in magic] if magic else None
         tmin, tmax = _r32(tmin), _r32(tmax)
     args = dict(magic=magic, nu_bulk=list(getattr(coll, "nu_bulk", [])) or None, tau_min=tmin, tau_max=tmax)
+    if type(coll).__name__ == "HuangMRTCollision":
+        args["huang"] = dict(k1=[float(v) for v in coll.k1], epslion=list(coll.epslion),
+                             g=[float(v) for v in coll.g_coh])
     oc = O.Collision(kind, group, model, list(coll.nu), **args)
     oc.init_tau(lat)
     if params32 and np.dtype(dtype) != np.float32:

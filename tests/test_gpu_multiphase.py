@@ -1,7 +1,8 @@
 """Multiphase breadth on the GPU against the oracle: ShanChenForceC2
 adhesion at solids (ExternalForce.py:105-116), FixedPsi walls and the open
 psi boundaries (SCBoundaryCondition.py:9-123), the cubic-EOS
-pseudopotentials (EOS.py:5-95), fused (LDS-tiled Shan-Chen step) and through
+pseudopotentials (EOS.py:5-95), HuangMRTCollision's source
+(CollisionEngine.py:453-515), fused (LDS-tiled Shan-Chen steps) and through
 the reference's granular call sequence."""
 import contextlib
 import io
@@ -41,6 +42,8 @@ for kind in EOS_PARAMS:
                                    amp=0.2 * RHO_MEAN[kind], eos=(kind, EOS_PARAMS[kind]))
 CASES["eos_cs_c2"] = dict(NX=40, NY=36, nc=2, gcross=0.9, seed=25, eos=("cs", EOS_PARAMS["cs"]), rho_mean=0.0326,
                          amp=0.003)
+CASES["huang_c1"] = dict(NX=48, NY=40, nc=1, G=-5.0, seed=27, huang=(0.1, 0.5))
+CASES["huang_c2"] = dict(NX=40, NY=36, nc=2, gcross=0.9, seed=28, huang=(0.05, 0.3))
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -57,7 +60,7 @@ def test_multiphase_fused_matches_oracle(name):
     close("psi", case["lbf"].sc_field.psi_field.to_numpy(), sol.lat.psi)
 
 
-@pytest.mark.parametrize("name", ["adhesion_c2", "wall_psi_c1", "eos_cs_c1", "eos_pr_c1"])
+@pytest.mark.parametrize("name", ["adhesion_c2", "wall_psi_c1", "eos_cs_c1", "eos_pr_c1", "huang_c1", "huang_c2"])
 def test_multiphase_granular_matches_oracle(name):
     print(f"case multiphase granular {name}", flush=True)
     case = _quiet(cases.shan_chen, **CASES[name])

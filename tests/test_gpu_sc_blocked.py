@@ -4,7 +4,8 @@ tile, all through LDS) against one step per launch (k_sc): bit-identical
 populations, macroscopic fields, forces, psi and pressure observables, over
 odd and even step counts, runs split into pieces, tiles cut by the lattice
 edges, lattices smaller than one tile, solid disks with bounce-back,
-adhesion, fixed wall psi and cubic-EOS pseudopotentials."""
+adhesion, fixed wall psi, cubic-EOS pseudopotentials and the Huang MRT
+source."""
 import contextlib
 import io
 
@@ -30,6 +31,7 @@ CASES = {
                                       temperature_cr=1.104))),
     "c1_bgk": dict(NX=64, NY=40, nc=1, G=-5.0, seed=38, collision="bgk", nu=0.1),
     "c2_trt": dict(NX=64, NY=40, nc=2, gcross=0.9, seed=39, collision="trt", nu=0.1),
+    "c1_huang": dict(NX=45, NY=38, nc=1, G=-5.0, seed=40, huang=(0.1, 0.5)),
 }
 
 

@@ -70,3 +70,34 @@ def test_psi_objects_fill_the_descriptor():
     L = H.Lattice()
     case["lbf"].sc_field.fill_lattice(L)
     assert L.sc_adhesion == 1 and list(L.g_adh) == pytest.approx([0.2, -0.1]) and L.psi_kind[1] == H.PSI_SC
+
+
+def test_huang_collision_descriptor_and_effect():
+    """HuangMRTCollision fills k1, k2 = epslion / -8 - k1 and G = g_coh[c][c]
+    into the descriptor, refuses a zero self-coupling, and on the GPU test
+    lattice (48 x 40, G = -5, k1 = 0.1, epslion = 0.5) changes the oracle's
+    populations after 25 steps by about 40 times the parity bound (so the GPU
+    parity test of that lattice would see a kernel that dropped the source)."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent))
+    from oracle_bridge import oracle_solver
+    from taichi_lbm.CollisionEngine import HuangMRTCollision
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.shan_chen(NX=48, NY=40, nc=1, G=-5.0, seed=27, huang=(0.1, 0.5))
+        plain = cases.shan_chen(NX=48, NY=40, nc=1, G=-5.0, seed=27)
+    col = case["coll"]
+    L = H.Lattice()
+    col.fill_lattice(L)
+    assert L.huang == 1 and L.collision == H.LBM_MRT
+    assert L.huang_k1[0] == pytest.approx(0.1) and L.huang_k2[0] == pytest.approx(0.5 / -8 - 0.1)
+    assert L.huang_g[0] == pytest.approx(-5.0)
+    with pytest.raises(ValueError):
+        HuangMRTCollision({'group': col.group, 'fluid_model': None, 'NX': 48, 'NY': 40, 'num_components': 1,
+                           'shearviscosity': [0.1], 'bulkviscosity': [0.1], 'k1': [0.1], 'epslion': [0.5],
+                           'g_coh': [[0.0]]})
+    a, b = oracle_solver(case, np.float32), oracle_solver(plain, np.float32)
+    a.run(25)
+    b.run(25)
+    d = np.abs(a.lat.f - b.lat.f).max()
+    assert d > 10 * 1e-5 * np.abs(b.lat.f).max(), d   # about 40x the bound

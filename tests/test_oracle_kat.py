@@ -934,3 +934,64 @@ def test_peng_robinson_acentric_factor():
     # a dropped omega term moves it far off
     e0 = O.EOS("pr", a, b, R, -1.0, omega=0.0, T_cr=Tc)
     assert abs(float(e0.pressure(np.array([0.5 / b]), np.array([T]))[0]) - p(0.5 / b)) > 1e-3 * pc
+
+
+# ---------------------------------------------------------------------------
+# HuangMRTCollision (CollisionEngine.py:453-515)
+# ---------------------------------------------------------------------------
+def _huang_pair(k1, eps, G, F=(0.02, -0.01), psi=0.8, nu=0.1, nu_bulk=0.2):
+    """Post-collision populations of the plain MRT collision and of the Huang
+    collision on one 3 x 2 state (FP64), with their relaxation rates."""
+    rng = np.random.default_rng(11)
+    out = []
+    for huang in (None, dict(k1=[k1], epslion=[eps], g=[G])):
+        lat = O.Lattice(3, 2, 1, dtype=np.float64)
+        lat.f[...] = 1.0 / 9.0 + 0.01 * rng.random(lat.f.shape) if not out else out[0][2]
+        lat.rho[..., 0] = lat.f[..., 0, :].sum(-1)
+        lat.vel[...] = 0.01
+        lat.body_force[..., 0, :] = F
+        lat.psi = np.full((5, 4, 1), psi)
+        group = np.array([[i, j] for i in range(3) for j in range(2)], np.int64)
+        col = O.Collision("mrt", group, O.FluidModel("newtonian", nu=nu), [nu], nu_bulk=[nu_bulk], huang=huang)
+        col.init_tau(lat)
+        col.apply(lat)
+        out.append((lat.f2[..., 0, :].copy(), col.diag[0, 0, 0].copy(), lat.f.copy()))
+    return out
+
+
+def test_huang_source_closed_form():
+    """F = (0.02, -0.01), psi = 0.8, G = -5, k1 = 0.1, epslion = 0.4, so
+    k2 = 0.4 / -8 - 0.1 = -0.15 and G psi^2 = -3.2:
+    Q1 = 3 (0.1 - 0.3) (4e-4 + 1e-4) / -3.2 = 9.375e-5, Q2 = -Q1,
+    Q7 = 0.1 (4e-4 - 1e-4) / -3.2 = -9.375e-6, Q8 = 0.1 (0.02)(-0.01) / -3.2
+    = 6.25e-6.  The Huang collision's moments exceed plain MRT's by S Q
+    (S_1 the bulk rate, S_2 = 1, S_7 = S_8 the shear rate) and by nothing
+    else: mass and momentum are plain MRT's."""
+    (f_mrt, S, _), (f_h, S2, _) = _huang_pair(0.1, 0.4, -5.0)
+    assert np.array_equal(S, S2)
+    dm = O.mf(f_h.reshape(-1, 9)) - O.mf(f_mrt.reshape(-1, 9))
+    want = np.zeros(9)
+    want[1] = S[1] * 9.375e-5
+    want[2] = -9.375e-5
+    want[7] = S[7] * -9.375e-6
+    want[8] = S[8] * 6.25e-6
+    assert S[1] != 1.0 and S[7] != 1.0 and S[2] == 1.0
+    assert np.abs(dm - want[None, :]).max() < 1e-15, dm[0]
+
+
+def test_huang_with_zero_coefficients_is_plain_mrt():
+    """k1 = 0 and epslion = 0 make Q vanish: the Huang collision is the MRT
+    collision bit for bit."""
+    (f_mrt, _, _), (f_h, _, _) = _huang_pair(0.0, 0.0, -5.0)
+    assert np.array_equal(f_mrt, f_h)
+
+
+def test_huang_source_scales_with_inverse_g_psi_squared():
+    """Q is k-linear over G psi^2: doubling psi divides the moment change by 4,
+    flipping G flips it."""
+    base = _huang_pair(0.1, 0.4, -5.0)
+    d0 = base[1][0] - base[0][0]
+    d_psi = _huang_pair(0.1, 0.4, -5.0, psi=1.6)
+    d_g = _huang_pair(0.1, 0.4, 5.0)
+    assert np.allclose(d_psi[1][0] - d_psi[0][0], d0 / 4, rtol=1e-9, atol=1e-18)
+    assert np.allclose(d_g[1][0] - d_g[0][0], -d0, rtol=1e-9, atol=1e-18)

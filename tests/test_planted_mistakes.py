@@ -195,3 +195,48 @@ def test_planted_multicomponent_mistake_is_caught(name, monkeypatch):
     failed = [p.__name__ for p in pins if _fails(p)]
     print(f"  {name}: pins that fail: {failed}")
     assert failed == [p.__name__ for p in pins], f"{name}: a pin does not catch it"
+
+
+# HuangMRTCollision's source moments (oracle m_q)
+def _mq_q2_sign(F, psi, k1, k2, g, dt):
+    Q = _ORIG_M_Q(F, psi, k1, k2, g, dt)
+    Q[:, 2] = Q[:, 1]
+    return Q
+
+
+def _mq_psi_not_squared(F, psi, k1, k2, g, dt):
+    # G psi in place of G psi^2 (psi > 0 here: sqrt(psi)^2 = psi)
+    return _ORIG_M_Q(F, np.sqrt(psi), k1, k2, g, dt)
+
+
+def _mq_k2_sign(F, psi, k1, k2, g, dt):
+    # k2 = epslion / 8 - k1 instead of epslion / -8 - k1
+    eps = (k2 + k1) * dt.type(-8.0)
+    return _ORIG_M_Q(F, psi, k1, eps / dt.type(8.0) - k1, g, dt)
+
+
+def _mq_q8_factor_two(F, psi, k1, k2, g, dt):
+    Q = _ORIG_M_Q(F, psi, k1, k2, g, dt)
+    Q[:, 8] = 2 * Q[:, 8]
+    return Q
+
+
+_ORIG_M_Q = O.m_q
+MUTATIONS_HUANG = {
+    "q2_sign": (O, "m_q", _mq_q2_sign, [kat.test_huang_source_closed_form]),
+    "psi_not_squared": (O, "m_q", _mq_psi_not_squared, [kat.test_huang_source_closed_form,
+                                                        kat.test_huang_source_scales_with_inverse_g_psi_squared]),
+    "k2_sign": (O, "m_q", _mq_k2_sign, [kat.test_huang_source_closed_form]),
+    "q8_factor_two": (O, "m_q", _mq_q8_factor_two, [kat.test_huang_source_closed_form]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS_HUANG))
+def test_planted_huang_mistake_is_caught(name, monkeypatch):
+    obj, attr, value, pins = MUTATIONS_HUANG[name]
+    for p in pins:
+        p()
+    monkeypatch.setattr(obj, attr, value)
+    failed = [p.__name__ for p in pins if _fails(p)]
+    print(f"  {name}: pins that fail: {failed}")
+    assert failed == [p.__name__ for p in pins], f"{name}: a pin does not catch it"
