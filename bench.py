@@ -504,6 +504,77 @@ def configs_leg(cyl_steps=10_000, sc_N=2048, sc_steps=20_000, sc_every=5000):
         "mass_drift_rel": drift, "nonfinite": red["nonfinite"]}
     del st, case
     torch.cuda.empty_cache()
+    out.update(ib_leg())
+    return out
+
+
+def ib_leg(NX=2048, NY=1024, n_particles=256, steps=200, mdf=3):
+    """Immersed-boundary particles: an all-fluid periodic NX x NY channel
+    (BGK tau = 0.7, g = 1e-6) with n_particles circular particles (radius
+    U[6, 10], speed 0.01, turning 0.002 rad/step, one crossing the edge),
+    coupled by multi-direct forcing with `mdf` iterations through the
+    granular sequence density -> force_density -> velocity ->
+    SphereInfoUpdata -> MultiDirectForcingLoop -> ComputeParticleForce ->
+    collide -> boundaries, `steps` steps timed with HIP events on the launch
+    stream; the IB calls alone are timed on the same state."""
+    import numpy as np
+    import torch
+    from taichi_lbm import cases
+    from taichi_lbm.ibm import IBDEMCouplerEngine, IBEngine
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.ib_particles(NX=NX, NY=NY, n_particles=n_particles, radius=(6.0, 10.0), u_particle=0.01,
+                                  seed=2)
+        cases.initialise(case)
+        lbf, macro, coll, bce, sp = case["lbf"], case["macro"], case["coll"], case["bce"], case["sp"]
+        sph = case["spheres"]
+        n = n_particles
+        sp.init_Balls(n, np.arange(n), sph["pos"], sph["vel"], np.zeros((n, 2)), np.zeros(n), sph["radius"],
+                      sph["angle"])
+        sp.init_IB_nodes()
+    coupler, ibe = IBDEMCouplerEngine(lbf), IBEngine(mdf)
+
+    def ib_part(t):
+        pos, vel, ang = cases.ib_particle_state(case, t)
+        coupler.SphereInfoUpdata(sp, pos, vel, ang)
+        ibe.MultiDirectForcingLoop(sp, lbf)
+        ibe.ComputeParticleForce(sp)
+
+    def step(t):
+        macro.density(lbf)
+        macro.force_density(lbf)
+        macro.velocity(lbf)
+        ib_part(t)
+        coll.apply(lbf)
+        bce.apply_boundary_conditions(lbf)
+
+    for t in range(10):
+        step(t)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for t in range(10, 10 + steps):
+        step(t)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    e0.record(s)
+    for t in range(steps):
+        ib_part(10 + steps + t)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ib_us = e0.elapsed_time(e1) / steps * 1e3
+    markers = int(sp.IBInsiderCount[None])
+    u = lbf.vel.to_numpy()
+    force = coupler.GetSphereForce(sp)
+    out = {f"ib_particles_{NX}x{NY}_{n_particles}": {
+        "steps": steps + 10, "ms_per_step": round(ms, 4), "mlups": round(NX * NY / (ms * 1e-3) / 1e6, 1),
+        "ib_us_per_step": round(ib_us, 1), "markers": markers, "mdf_iterations": mdf,
+        "path": "granular sequence (one HIP kernel per reference kernel) with the IB kernels of csrc/lbm_ibm.hip",
+        "finite": bool(np.isfinite(u).all() and np.isfinite(force).all()),
+        "max_abs_particle_force": float(np.abs(force).max())}}
+    del case, sp
+    torch.cuda.empty_cache()
     return out
 
 
@@ -528,6 +599,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--only-configs", action="store_true")
+    ap.add_argument("--only-ib", action="store_true")
     ap.add_argument("--configs-cyl-steps", type=int, default=10_000)
     ap.add_argument("--configs-sc-steps", type=int, default=20_000)
     ap.add_argument("--headline-only", action="store_true",
@@ -547,6 +619,9 @@ def main():
 
     if args.only_serving:
         print(json.dumps(serving_leg(args.serve_requests, 4.0, 5, 50_000, 1e-6, 200)), flush=True)
+        return
+    if args.only_ib:
+        print(json.dumps(ib_leg()), flush=True)
         return
     if args.only_configs:
         print(json.dumps(configs_leg(args.configs_cyl_steps, sc_steps=args.configs_sc_steps)), flush=True)

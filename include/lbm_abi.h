@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 12
+#define LBM_ABI_VERSION 13
 
 /* status codes */
 #define LBM_OK 0
@@ -408,6 +408,66 @@ enum {
  * neighbours (one block reduces the group, so groups are boundary lines). */
 int lbm_g_sc_psi_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, int32_t direction,
                           float value, const int32_t* group, int32_t count, void* stream);
+
+/* ---- immersed-boundary particles (ibm.py) --------------------------------
+ * Circular particles (Sphere, ibm.py:17-26) carrying a ring of Lagrangian
+ * markers (IBMNodes, ibm.py:8-14), coupled to the lattice by multi-direct
+ * forcing (IBEngine, ibm.py:166-253).  Sphere fields are arrays over the
+ * max_number capacity; marker fields are arrays over max_number *
+ * max_num_ib with marker (m, n) at m * max_num_ib + n; vectors are two
+ * consecutive planes (x, then y) of such an array.                         */
+typedef struct lbm_ib_particles {
+  int32_t max_number;      /* SphereIB Max_Number                              */
+  int32_t max_num_ib;      /* SphereIB Max_NumIB                               */
+  int32_t num;             /* SphereIB.num[None]: spheres in use, <= max_number */
+  float numib_cof;         /* SphereIB.NUMIB_cof (marker spacing)              */
+  float* pos;              /* Sphere.pos, 2 planes of max_number               */
+  float* vel;              /* Sphere.vel                                       */
+  float* force;            /* Sphere.force                                     */
+  float* torque;           /* Sphere.torque, 1 plane                           */
+  float* radius;           /* Sphere.radius                                    */
+  float* angle;            /* Sphere.angle                                     */
+  int32_t* num_ib;         /* Sphere.NumIB                                     */
+  float* node_pos;         /* IBMNodes.pos, 2 planes of max_number*max_num_ib  */
+  float* node_vel;         /* IBMNodes.vel                                     */
+  float* node_vel_desired; /* IBMNodes.vel_desired                             */
+  float* node_force;       /* IBMNodes.force                                   */
+  float* node_force_temp;  /* IBMNodes.force_temp                              */
+  int32_t* group;          /* IBNodeInsiderGroup: (max_number*max_num_ib, 2)   *
+                            * int32 (m, n), the first *count entries in use     */
+  int32_t* count;          /* IBInsiderCount[None], one int32                  */
+  int32_t* work;           /* lbm_ib_work_ints(max_number, max_num_ib) int32 of *
+                            * scratch, zero-filled before first use: marker      *
+                            * ranks, per-sphere counts and offsets of the last   *
+                            * lbm_ib_update (read by lbm_ib_particle_force)      */
+} lbm_ib_particles;
+
+/* Size of lbm_ib_particles.work in int32 (0 for a bad capacity). */
+int64_t lbm_ib_work_ints(int32_t max_number, int32_t max_num_ib);
+/* SphereIB.init_IB_nodes, ibm.py:57-67: NumIB = round(2 pi r / numib_cof)
+ * markers on each sphere's circle (markers beyond max_num_ib are not set). */
+int lbm_ib_init_nodes(const lbm_ib_particles* p, void* stream);
+/* IBDEMCouplerEngine.SphereIBUpdate, ibm.py:120-141 (reads lat->nx, ny):
+ * moves the markers of spheres inside or crossing the lattice
+ * (computer_IB_nodes_pos, :68-77), sets their desired velocity, and lists
+ * the markers in use in (m, n) order.                                      */
+int lbm_ib_update(const lbm_lattice* lat, const lbm_ib_particles* p, void* stream);
+/* SphereIB.reset_force, ibm.py:80-83 */
+int lbm_ib_reset_force(const lbm_ib_particles* p, void* stream);
+/* IBEngine.InterpolateBodyVelocities, ibm.py:201-223 (reads g->vel) */
+int lbm_ib_interpolate(const lbm_lattice* lat, const lbm_gfields* g, const lbm_ib_particles* p, void* stream);
+/* IBEngine.SpreadBodyForces, ibm.py:225-244 (reads g->rho, g->total_rho;
+ * adds to g->vel and g->body_force with float atomics)                    */
+int lbm_ib_spread(const lbm_lattice* lat, const lbm_gfields* g, const lbm_ib_particles* p, void* stream);
+/* IBEngine.MultiDirectForcingLoop, ibm.py:248-253: reset_force, then
+ * `iterations` x (interpolate, spread).                                    */
+int lbm_ib_mdf(const lbm_lattice* lat, const lbm_gfields* g, const lbm_ib_particles* p, int32_t iterations,
+               void* stream);
+/* IBEngine.ComputeParticleForce, ibm.py:171-185: each sphere with markers
+ * in the list of the last lbm_ib_update gets force = sum of their forces and
+ * torque = sum (F_x r_y - F_y r_x), r = marker - centre; other spheres keep
+ * theirs.  (The reference resets the sums inside the marker loop, a race.) */
+int lbm_ib_particle_force(const lbm_ib_particles* p, void* stream);
 
 /* ---- memory-bandwidth probe -----------------------------------------------*/
 /* Plain fp32 copy of n elements (the achievable-bandwidth reference used to

@@ -12,6 +12,7 @@ from .BoundaryCondition import (BoundarySpec, BoundaryClassifier, MaskAndGroup, 
                                 BounceBackWall, FluidBoundary, PeriodicAllBoundary, OpenNeumann,
                                 OpenExtrapolation, OpenConvective1order, OpenConvective2order)
 from .CollisionEngine import BGKCollision, TRTCollision, MRTCollision, HuangMRTCollision
+from .ibm import IBMNodes, Sphere, SphereIB, IBDEMCouplerEngine, IBEngine
 from .ExternalForce import ShanChenForceC1, ShanChenForceC2
 from .SCBoundaryCondition import (BoundaryEngineSC, OpenNeumannPsi, OpenConvective1orderPsi,
                                   OpenConvective2orderPsi, OpenExtrapolationPsi, PeriodicPsi_LR, PeriodicPsi_BT,

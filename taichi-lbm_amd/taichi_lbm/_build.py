@@ -20,7 +20,7 @@ INCLUDE = ROOT.parent / "include"
 LIB_DIR = PKG_DIR / "_lib"
 LIB_PATH = LIB_DIR / "liblbm_hip.so"
 STAMP_PATH = LIB_DIR / "liblbm_hip.stamp"
-SOURCES = ["lbm_fused.hip", "lbm_granular.hip", "lbm_reduce.hip"]
+SOURCES = ["lbm_fused.hip", "lbm_granular.hip", "lbm_reduce.hip", "lbm_ibm.hip"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -31,7 +31,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 
 class NeemRule(C.Structure):
@@ -73,6 +73,13 @@ class GFields(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in (
         "f", "f2", "rho", "total_rho", "vel", "body_force", "sc_force", "shear_rate", "pressure",
         "total_pressure", "temperature", "psi", "neighbor", "tau", "sc_fluid")]
+
+
+class IBParticles(C.Structure):
+    _fields_ = [("max_number", C.c_int32), ("max_num_ib", C.c_int32), ("num", C.c_int32),
+                ("numib_cof", C.c_float)] + [(n, C.c_void_p) for n in (
+                    "pos", "vel", "force", "torque", "radius", "angle", "num_ib", "node_pos", "node_vel",
+                    "node_vel_desired", "node_force", "node_force_temp", "group", "count", "work")]
 
 
 class LBMError(RuntimeError):
@@ -144,6 +151,17 @@ def lib():
         "lbm_batch_residual": ([BP, I32, D, P], C.c_int),
         "lbm_batch_destroy": ([BP], C.c_int),
     })
+    IP = C.POINTER(IBParticles)
+    sig.update({
+        "lbm_ib_work_ints": ([I32, I32], C.c_int64),
+        "lbm_ib_init_nodes": ([IP, P], C.c_int),
+        "lbm_ib_update": ([LP, IP, P], C.c_int),
+        "lbm_ib_reset_force": ([IP, P], C.c_int),
+        "lbm_ib_interpolate": ([LP, GP, IP, P], C.c_int),
+        "lbm_ib_spread": ([LP, GP, IP, P], C.c_int),
+        "lbm_ib_mdf": ([LP, GP, IP, I32, P], C.c_int),
+        "lbm_ib_particle_force": ([IP, P], C.c_int),
+    })
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
@@ -166,7 +184,9 @@ def exported_symbols():
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
             "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2", "lbm_batch_release",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
-            "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy"]
+            "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy",
+            "lbm_ib_work_ints", "lbm_ib_init_nodes", "lbm_ib_update", "lbm_ib_reset_force", "lbm_ib_interpolate",
+            "lbm_ib_spread", "lbm_ib_mdf", "lbm_ib_particle_force"]
 
 
 def check(rc: int, what: str = ""):

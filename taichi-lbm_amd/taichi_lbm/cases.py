@@ -529,3 +529,52 @@ def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt
     macro, rho, vel = _finish(lbf, bce, fluid, coll, u0=(u_in, 0.0))
     return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
                 spec=dict(kind="cylinder_ref", NX=NX, NY=NY, tau=tau, u_in=u_in, collision=collision))
+
+
+def ib_particles(NX=256, NY=128, n_particles=6, radius=(6.0, 10.0), u_particle=0.01, tau=0.7, collision="bgk",
+                 g=(1e-6, 0.0), seed=0, max_num_ib=None, name="ib_particles"):
+    """Immersed-boundary particles in a periodic channel: an all-fluid lattice
+    with periodic streaming and body force g, and ``n_particles`` circular
+    particles (ibm.py Sphere) of seeded radii in ``radius`` and velocities of
+    size ``u_particle``, the first one crossing the left edge of the lattice.
+    The particles couple to the fluid by multi-direct forcing (IBEngine),
+    called between MacroscopicEngine.velocity and the collision."""
+    from .ibm import SphereIB
+    lbf = LBField(name, NX, NY, 1)
+    bce = BoundaryEngine()
+    clf = BoundaryClassifier((NX, NY))
+    fluid = FluidBoundary(BoundarySpec(lambda I, J: np.ones(I.shape, bool)))
+    fluid.precompute(clf)
+    bce.add_boundary_condition("fluid", fluid)
+    periodic = PeriodicAllBoundary(BoundarySpec(lambda I, J: np.ones(I.shape, bool)))
+    periodic.precompute(clf)
+    bce.add_boundary_condition("periodic", periodic)
+    lbf.neighbor_classify()
+    lbf.set_gravity(g)
+    nu = (tau - 0.5) / 3.0
+    coll = make_collision(collision, fluid.group, NX, NY, 1, [nu])
+    macro, rho, vel = _finish(lbf, bce, fluid, coll)
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(radius[0], radius[1], n_particles).astype(np.float32)
+    pos = np.stack([rng.uniform(2.5 * r, NX - 1 - 2.5 * r), rng.uniform(2.5 * r, NY - 1 - 2.5 * r)], 1)
+    pos[0, 0] = 0.5 * r[0]                       # crosses the edge i = 0
+    ang = rng.uniform(0, 2 * math.pi, n_particles)
+    vdir = rng.uniform(0, 2 * math.pi, n_particles)
+    pvel = u_particle * np.stack([np.cos(vdir), np.sin(vdir)], 1)
+    K = max_num_ib or int(math.ceil(2 * math.pi * radius[1])) + 1
+    sp = SphereIB(n_particles, K)
+    spheres = dict(pos=pos.astype(np.float32), vel=pvel.astype(np.float32), radius=r,
+                   angle=ang.astype(np.float32), omega=np.float32(0.002))
+    return dict(lbf=lbf, bce=bce, macro=macro, coll=coll, fluid=fluid, rho=rho, vel=vel, power_law=False,
+                sp=sp, spheres=spheres,
+                spec=dict(kind="ib_particles", NX=NX, NY=NY, n_particles=n_particles, radius=radius,
+                          u_particle=u_particle, tau=tau, collision=collision, g=g, seed=seed, max_num_ib=K))
+
+
+def ib_particle_state(case, step):
+    """Prescribed particle motion of an ib_particles case at ``step``: each
+    particle translates with its velocity and turns at ``omega`` rad/step."""
+    s = case["spheres"]
+    pos = (s["pos"] + np.float32(step) * s["vel"]).astype(np.float32)
+    ang = (s["angle"] + np.float32(step) * s["omega"]).astype(np.float32)
+    return pos, s["vel"], ang

@@ -15,7 +15,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "lbm_abi.h"
 
 def declared():
     txt = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(lbm_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(lbm_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_lists_entry_points():
@@ -40,7 +40,7 @@ def test_abi_version_and_error_string():
 def test_struct_layout_matches_header(tmp_path):
     # every ctypes field sits at the offset the C compiler gives it
     structs = {"lbm_neem_rule": H.NeemRule, "lbm_lattice": H.Lattice, "lbm_fields": H.Fields,
-               "lbm_gfields": H.GFields}
+               "lbm_gfields": H.GFields, "lbm_ib_particles": H.IBParticles}
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

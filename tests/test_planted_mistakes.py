@@ -240,3 +240,41 @@ def test_planted_huang_mistake_is_caught(name, monkeypatch):
     failed = [p.__name__ for p in pins if _fails(p)]
     print(f"  {name}: pins that fail: {failed}")
     assert failed == [p.__name__ for p in pins], f"{name}: a pin does not catch it"
+
+
+# Immersed-boundary coupling (oracle/ibm_oracle.py, the reference's ibm.py)
+import test_ibm_oracle as ibk  # noqa: E402
+from oracle import ibm_oracle as IBO  # noqa: E402
+
+
+def _weight_transposed(pos, ix, iy):
+    dt = type(pos[0])
+    return (dt(1.0) - abs(pos[1] - dt(ix))) * (dt(1.0) - abs(pos[0] - dt(iy)))
+
+
+def _torque_sign_flipped(fx, fy, rx, ry):
+    return rx * fy - ry * fx
+
+
+MUTATIONS_IBM = {
+    "stencil_weight_transposed": ("weight", _weight_transposed,
+                                  [ibk.test_interpolation_reproduces_linear_fields]),
+    "force_gain_dropped": ("FORCE_GAIN", 1.0, [ibk.test_interpolation_reproduces_linear_fields]),
+    "velocity_share_not_halved": ("VEL_SHARE", 1.0,
+                                  [ibk.test_spread_conserves_force_and_is_the_transpose_of_interpolation]),
+    "torque_sign_flipped": ("torque_arm", _torque_sign_flipped, [ibk.test_particle_force_and_torque]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS_IBM))
+def test_planted_ibm_mistake_is_caught(name, monkeypatch):
+    attr, value, pins = MUTATIONS_IBM[name]
+    for p in pins:
+        p()
+    monkeypatch.setattr(IBO, attr, value)
+    failed = [p.__name__ for p in pins if _fails(p)]
+    print(f"  {name}: pins that fail: {failed}")
+    assert failed == [p.__name__ for p in pins], f"{name}: a pin does not catch it"
+    # the multi-direct-forcing closed form catches every slip of the coupling itself
+    if attr != "torque_arm":
+        assert _fails(lambda: ibk.test_multi_direct_forcing_closed_form(7.25, 9.6)), name
