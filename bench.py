@@ -30,6 +30,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "MLUPS & % HBM roofline; batched solves/s + p50/p99 latency vs plain path"
 HBM_PEAK_GBS = 8000.0
 BYTES_POWER_LAW = 80   # 9 reads + 9 writes x 4 B, + tau read + write (4 + 4 B)
+BYTES_POWER_LAW_SHIFTED = 44   # 9 reads + 9 writes x 2 B (FP16 / BF16 shifted), + tau read + write
 
 
 def log(*a):
@@ -43,7 +44,7 @@ def build_case(N, collision, seed):
     return case
 
 
-def time_gpu(case, steps, warmup):
+def time_gpu(case, steps, warmup, storage="f32"):
     import torch
     from taichi_lbm import cases
     from taichi_lbm.stepper import Stepper
@@ -52,7 +53,8 @@ def time_gpu(case, steps, warmup):
     # the tile width is tuned in a pass of its own before the warm-up (its
     # steps are ordinary steps of the run; the fields do not depend on the
     # width), so the timed kernel is the tuned one whatever --warmup is
-    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True, tile_cols="auto")
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True, tile_cols="auto",
+                 storage=storage)
     st.run(Stepper.tune_steps())
     st.run(warmup)
     torch.cuda.synchronize()
@@ -662,6 +664,28 @@ def main():
         extra["trt_hbm_frac"] = round(n_fluid * BYTES_POWER_LAW / (ms_t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         tcase["coll"]._state = None
         lbf._st = None
+        torch.cuda.empty_cache()
+
+    if not args.no_trt:
+        # the same MRT lattice with shifted 16-bit populations (FP32 arithmetic):
+        # 36 B of populations + 8 B of power-law state per fluid node-step
+        lbf = case["lbf"]
+        for storage in ("f16", "bf16"):
+            lbf._st = None
+            lbf._finisher = None
+            lbf._pop_owner = None
+            case["coll"]._state = None
+            torch.cuda.empty_cache()
+            ms_s, red_s = time_gpu(case, args.steps, args.warmup, storage=storage)
+            log(f"MRT {storage}: {ms_s:.4f} ms/step")
+            extra[f"{storage}_ms_per_step"] = round(ms_s, 5)
+            extra[f"{storage}_tile_cols"] = red_s["tile_cols"]
+            extra[f"{storage}_mlups"] = round(n_fluid / (ms_s * 1e-3) / 1e6, 1)
+            extra[f"{storage}_hbm_frac"] = round(n_fluid * BYTES_POWER_LAW_SHIFTED / (ms_s * 1e-3) / 1e9 /
+                                                 HBM_PEAK_GBS, 4)
+            extra[f"{storage}_nonfinite"] = red_s["nonfinite"]
+        lbf._st = None
+        case["coll"]._state = None
         torch.cuda.empty_cache()
 
     batched = None

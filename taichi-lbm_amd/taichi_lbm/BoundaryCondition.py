@@ -185,6 +185,7 @@ class BoundaryCondition:
                                        ctypes.byref(gfields(lb_field, neighbor=need_nb)),
                                        kind, ctypes.byref(rule) if rule is not None else None, g[0], g[1],
                                        stream()), type(self).__name__)
+        lb_field._pops_written()
 
 
 class PlaneBoundary(BoundaryCondition):

@@ -120,6 +120,7 @@ class Collision:
     def apply(self, lb_field):
         """Collide f -> f2 (BGK :96-110, TRT :223-245, MRT :424-451)."""
         self._granular(H.lib().lbm_g_collide, lb_field, f"{type(self).__name__}.apply")
+        lb_field._pops_written()
 
     def shear_rate(self, lb_field):
         """BGK :80-92, TRT :166-178, MRT :332-377."""
@@ -303,3 +304,4 @@ class HuangMRTCollision(MRTCollision):
         H.check(H.lib().lbm_g_collide(ctypes.byref(lattice_struct(lb_field, collision=self, sc=sc)),
                                       ctypes.byref(gfields(lb_field, collision=self, sc=sc)), g[0], g[1],
                                       stream()), "HuangMRTCollision.apply")
+        lb_field._pops_written()

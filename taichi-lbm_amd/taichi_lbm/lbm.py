@@ -54,6 +54,7 @@ class LBField:
         self._neighbor_dev = None
         self._st = None          # device state, allocated on first device use
         self._finisher = None    # pending boundary pass of the last fused run
+        self._pop_owner = None   # a Stepper whose 16-bit populations are the newest state (None: FP32)
         self._pressure_refresh = None
         self.geometry_version = 0
 
@@ -61,7 +62,8 @@ class LBField:
         pv = planes_view
         self.f = DeviceField(lambda: pv(self._pop(f=True), (nc, 9), NX_, NY_), self.materialize,
                              self._after_f_write, name="f")
-        self.f2 = DeviceField(lambda: pv(self._pop(f=False), (nc, 9), NX_, NY_), self.materialize, name="f2")
+        self.f2 = DeviceField(lambda: pv(self._pop(f=False), (nc, 9), NX_, NY_), self.materialize,
+                              self._pops_written, name="f2")
         self.rho = DeviceField(lambda: pv(self._mac()[:nc], (nc,), NX_, NY_), self.materialize, name="rho")
         self.total_rho = DeviceField(lambda: pv(self._mac()[nc:nc + 1], (), NX_, NY_), self.materialize,
                                      name="total_rho")
@@ -121,6 +123,7 @@ class LBField:
         g = group_device(group)
         H.check(H.lib().lbm_g_init_lbm(ctypes.byref(lattice_struct(self)), ctypes.byref(gfields(self)),
                                        g[0], g[1], stream()), "init_LBM")
+        self._pops_written()
         print("init LBM")
 
     def neighbor_classify(self):
@@ -200,7 +203,13 @@ class LBField:
             st["f_valid"] = True
 
     def _after_f_write(self):
-        pass
+        self._pops_written()
+
+    def _pops_written(self):
+        """The FP32 populations were written directly (field writes, granular
+        collide / boundary kernels, init_LBM): they are the newest state, and a
+        Stepper with 16-bit storage encodes them again before its next run."""
+        self._pop_owner = None
 
     def _refresh_pressure(self):
         if self._pressure_refresh is not None:

@@ -51,7 +51,15 @@ class Stepper:
         of the run).  no_alloc: two-step runs go through
         lbm_run_blocked_noalloc, which keeps the power-law state's second
         buffer in the library's fixed device workspace (nothing is allocated
-        per run; lattices up to 2^22 state values)."""
+        per run; lattices up to 2^22 state values).  storage: "f32", or "f16" /
+        "bf16" for shifted 16-bit populations g = f - w_k (FP32 arithmetic,
+        half the population bytes): the FP32 populations are encoded at the
+        first run (and again after anything else writes them), the kernels
+        run on the 16-bit buffers, and the boundary pass of a readback
+        decodes them into the LBField's FP32 fields (active nodes).  Runs
+        without a readback in between equal one run; a readback rounds the
+        post-boundary populations, which the next run starts from, to the
+        storage format."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
         self.no_alloc = bool(no_alloc)
@@ -64,9 +72,11 @@ class Stepper:
         self.bce = boundary_engine
         self.power_law = bool(power_law)
         self.advance_time = bool(advance_time)
-        if storage != "f32":
-            raise NotImplementedError("shifted population storage is selected through BatchRunner")
+        if storage not in _STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
+        self.storage_name = storage
         self.storage = _STORAGE[storage]
+        self._pop16 = None       # (2, nc*9, ps) shifted 16-bit populations (storage f16 / bf16)
         self.sc = lb_field.sc_field
         self._psi_fixed = []
         if self.sc is not None:
@@ -103,13 +113,46 @@ class Stepper:
             return H.TAU_POWER_LAW
         return H.TAU_CONST if self.coll._uniform else H.TAU_FIELD
 
+    # ---- shifted 16-bit storage ---------------------------------------------
+    def _shift_weights(self):
+        t = torch()
+        w = np.array([4, 1, 1, 1, 1, 1 / 4, 1 / 4, 1 / 4, 1 / 4], np.float32) / np.float32(9.0)
+        return t.as_tensor(np.tile(w, self.lbf.nc)[:, None], device=device())
+
+    def _encode16(self):
+        """FP32 populations (both buffers) -> g = f - w_k in FP16 / BF16, rounded
+        to nearest even as the kernels store them."""
+        t = torch()
+        lbf = self.lbf
+        lbf.materialize()
+        pop = lbf.state()["pop"]
+        dt = t.float16 if self.storage_name == "f16" else t.bfloat16
+        if self._pop16 is None or self._pop16.shape != pop.shape:
+            self._pop16 = t.empty(pop.shape, dtype=dt, device=pop.device)
+        w = self._shift_weights()
+        for b in range(2):
+            self._pop16[b].copy_((pop[b] - w).to(dt))
+        lbf._pop_owner = self
+
+    def _decode16(self):
+        """16-bit populations -> FP32 fields of the LBField (active nodes; the
+        kernels leave the others' populations untouched)."""
+        t = torch()
+        lbf = self.lbf
+        pop = lbf.state()["pop"]
+        w = self._shift_weights()
+        active = ((self.code.to(t.int32) & H.CODE_ACTIVE) != 0)[None, :]
+        for b in range(2):
+            pop[b].copy_(t.where(active, self._pop16[b].float() + w, pop[b]))
+
     def _structs(self, tau_mode):
         lbf = self.lbf
         st = lbf.state()
         L = lattice_struct(lbf, collision=self.coll, sc=self.sc, storage=self.storage, tau_mode=tau_mode,
                            neem=self.neem_host)
         F = H.Fields()
-        F.f[0], F.f[1] = H.ptr(st["pop"][0]), H.ptr(st["pop"][1])
+        pops = st["pop"] if self.storage == H.STORE_F32 else self._pop16
+        F.f[0], F.f[1] = H.ptr(pops[0]), H.ptr(pops[1])
         F.macro[0], F.macro[1] = H.ptr(st["mac"][0]), H.ptr(st["mac"][1])
         F.tau = H.ptr(self.coll.state_tensor())
         F.body_force = H.ptr(lbf.aux("body_force"))
@@ -175,6 +218,9 @@ class Stepper:
         lbf = self.lbf
         if self._compiled_version != lbf.geometry_version:
             self._compile()
+        shifted = self.storage != H.STORE_F32
+        if shifted and lbf._pop_owner is not self:
+            self._encode16()
         st = lbf.state()
         tau_mode = self._tau_mode()
         L, F = self._structs(tau_mode)
@@ -209,6 +255,7 @@ class Stepper:
                                     stream()), "lbm_run")
             st["f2_idx"] = (parity + nsteps) % 2
         st["f_valid"] = False
+        lbf._pop_owner = self if shifted else None
         if tau_mode == H.TAU_POWER_LAW:
             self.coll._uniform = False
         if self.sc is not None:
@@ -221,6 +268,8 @@ class Stepper:
         def finish():
             H.check(H.lib().lbm_finish(ctypes.byref(keep[0]), ctypes.byref(keep[1]), keep[2], last, stream()),
                     "lbm_finish")
+            if shifted:
+                self._decode16()
         lbf._finisher = finish
 
         def refresh_pressure():
