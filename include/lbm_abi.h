@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 13
+#define LBM_ABI_VERSION 14
 
 /* status codes */
 #define LBM_OK 0
@@ -408,6 +408,30 @@ enum {
  * neighbours (one block reduces the group, so groups are boundary lines). */
 int lbm_g_sc_psi_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, int32_t direction,
                           float value, const int32_t* group, int32_t count, void* stream);
+
+/* ---- batched Shan-Chen lattices -----------------------------------------
+ * Many independent Shan-Chen lattices (ShanChenForceC1/C2, ExternalForce.py:
+ * 55-145, each with its own lbm_solve() of cases/Bubble or two-phase
+ * separation.py:197-212) advanced together: one launch per pair of steps
+ * and collision model runs the psi-in-LDS two-step tile of every lattice
+ * (one launch per step where a lattice has node rules or a power law, and
+ * for a run's odd last step).  Each lattice keeps its own
+ * buffers; lattice l's rules are neem[l * LBM_MAX_NEEM ...].  FP32 storage,
+ * the same num_components for all.  parity_offset[l] (0 / 1, NULL: all 0)
+ * is lattice l's ping-pong phase against the run's `parity`: a launch reads
+ * pop[launch parity ^ parity_offset[l]].  Every lattice's fields equal its
+ * lbm_run_blocked alone bit for bit.                                           */
+typedef struct lbm_sc_batch lbm_sc_batch;
+int lbm_sc_batch_create(const lbm_lattice* lats, const lbm_fields* flds, const lbm_neem_rule* neem,
+                        const int32_t* parity_offset, int32_t count, lbm_sc_batch** out);
+/* nsteps steps of every lattice (first_local: the first step reads each
+ * lattice's post-stream f in place, as lbm_run's); *out_parity: the run's
+ * parity after the last launch (a launch of two steps flips it once), so
+ * lattice l's newest f2 is pop[*out_parity ^ parity_offset[l]]            */
+int lbm_sc_batch_run(lbm_sc_batch* b, int32_t nsteps, int32_t parity, int32_t first_local, int32_t* out_parity,
+                     void* stream);
+int lbm_sc_batch_tiles(const lbm_sc_batch* b);
+int lbm_sc_batch_destroy(lbm_sc_batch* b);
 
 /* ---- immersed-boundary particles (ibm.py) --------------------------------
  * Circular particles (Sphere, ibm.py:17-26) carrying a ring of Lagrangian

@@ -74,6 +74,7 @@ struct Lat {
   const float* temperature;    // lb_field.T (cubic-EOS psi)
   int32_t huang;               // HuangMRTCollision: MRT with the source S Q (huang_q)
   float huang_k1[LBM_MAX_NC], huang_k2[LBM_MAX_NC], huang_g[LBM_MAX_NC];
+  uint32_t parity_xor;         // batched Shan-Chen runs: this lattice's ping-pong phase against the launch's
   lbm_neem_rule neem[LBM_MAX_NEEM];
 };
 
@@ -1021,6 +1022,7 @@ __global__ __launch_bounds__(block2_threads(ring_of(M), W)) void k_fused2(const 
 // bit-identical to the two-pass path.
 // ---------------------------------------------------------------------------
 constexpr int kScCols = 32;
+constexpr int kScThreads = 512;   // 13 x 32 nodes per tile with the ring: 82 % of the threads own a tile node
 
 template <int NT>
 using ScShape = TileShape<1, kScCols, NT>;
@@ -1040,25 +1042,20 @@ struct PsiTile {
 // component, or two with MRT: 62 VGPRs; two-component BGK / TRT would spill)
 constexpr int sc_min_waves(int coll, int nc) { return (nc == 1 || coll == LBM_MRT) ? 8 : 1; }
 
+// One Shan-Chen tile (rows i0.., columns j0..) of lattice `a` for one step;
+// psiL / solL: the block's LDS (psi of the ring-extended tile per component,
+// adhesion flags).
 template <int COLL, int NC, int S, bool BUF, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves(COLL, NC)))) void k_sc(
-    const Lat a, const Launch2 q) {
+__device__ __forceinline__ void sc_tile(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
+                                        typename Pop<S>::R* psiL, uint8_t* solL) {
   using Shape = ScShape<NT>;
   using T = typename Pop<S>::T;
   using R = typename Pop<S>::R;
-  __shared__ R psiL[NC * Shape::PN];
-  __shared__ uint8_t solL[Shape::PN];   // adhesion flags of the ring-extended tile
-  const uint32_t nt = q.tiles_i * q.tiles_j;
-  uint32_t b = blockIdx.x;
-  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
-  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  const uint32_t i0 = ti * Shape::TI, j0 = tj * kScCols;
   const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
   const uint32_t r1 = threadIdx.x;
   const uint32_t rr = r1 / (uint32_t)Shape::RJ, cc = r1 - rr * (uint32_t)Shape::RJ;
   const uint32_t qs = Shape::slot(rr, cc);
   const uint32_t pset = q.parity ^ 1u;   // the psi plane set of this step
-  const bool local = q.local_read != 0;
   const T* __restrict__ src = (const T*)a.pop[q.parity];
   const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
   R f[NC][9];
@@ -1146,6 +1143,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves
   store_macro<NC, R>(a, (R*)a.mac[pset], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
+template <int COLL, int NC, int S, bool BUF, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sc_min_waves(COLL, NC)))) void k_sc(
+    const Lat a, const Launch2 q) {
+  using Shape = ScShape<NT>;
+  using R = typename Pop<S>::R;
+  __shared__ R psiL[NC * Shape::PN];
+  __shared__ uint8_t solL[Shape::PN];   // adhesion flags of the ring-extended tile
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  sc_tile<COLL, NC, S, BUF, NT>(a, q, q.local_read != 0, ti * Shape::TI, tj * kScCols, psiL, solL);
+}
+
 // ---------------------------------------------------------------------------
 // Shan-Chen, two steps per launch.
 //
@@ -1174,18 +1185,14 @@ struct ScTwoLds {
   uint8_t sol[Shape::PN];       // adhesion flags (LBM_CODE_SC_SOLID)
 };
 
+// Two Shan-Chen steps on one tile (rows i0.., columns j0..) of lattice `a`.
 template <int COLL, int NC, bool BUF, int NT>
-__global__ __launch_bounds__(NT) void k_sc2(const Lat a, const Launch2 q) {
+__device__ __forceinline__ void sc2_tile(const Lat& a, const Launch2& q, bool local, uint32_t i0, uint32_t j0,
+                                         ScTwoLds<NC, NT>& L) {
   using Shape = TileShape<3, kScCols, NT>;
   using R = float;
   using T = typename Pop<LBM_STORE_F32>::T;
   constexpr int S = LBM_STORE_F32;
-  __shared__ ScTwoLds<NC, NT> L;
-  const uint32_t nt = q.tiles_i * q.tiles_j;
-  uint32_t b = blockIdx.x;
-  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
-  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
-  const uint32_t i0 = ti * Shape::TI, j0 = tj * kScCols;
   const uint32_t ps = a.ps, nx = a.nx, ny = a.ny;
   const uint32_t r1 = threadIdx.x;
   const uint32_t rr = r1 / (uint32_t)Shape::RJ, cc = r1 - rr * (uint32_t)Shape::RJ;
@@ -1200,7 +1207,6 @@ __global__ __launch_bounds__(NT) void k_sc2(const Lat a, const Launch2 q) {
   // newest populations share an index and the psi of the step before is in
   // the other set, as after one-step launches; inactive nodes hold equal sets
   const uint32_t pa = q.parity, pb = q.parity ^ 1u;
-  const bool local = q.local_read != 0;
   const T* __restrict__ src = (const T*)a.pop[q.parity];
   const Rsrc rsrc = make_rsrc(src, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
   R f[NC][9];
@@ -1334,6 +1340,17 @@ __global__ __launch_bounds__(NT) void k_sc2(const Lat a, const Launch2 q) {
   store_macro<NC, R>(a, (R*)a.mac[pb], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
+template <int COLL, int NC, bool BUF, int NT>
+__global__ __launch_bounds__(NT) void k_sc2(const Lat a, const Launch2 q) {
+  using Shape = TileShape<3, kScCols, NT>;
+  __shared__ ScTwoLds<NC, NT> L;
+  const uint32_t nt = q.tiles_i * q.tiles_j;
+  uint32_t b = blockIdx.x;
+  if (q.xcd_remap) b = (b & 7u) * (nt >> 3) + (b >> 3);
+  const uint32_t ti = b / q.tiles_j, tj = b - ti * q.tiles_j;
+  sc2_tile<COLL, NC, BUF, NT>(a, q, q.local_read != 0, ti * Shape::TI, tj * kScCols, L);
+}
+
 // One tile of a batched two-step launch (32 bytes): what the tile's first
 // loads need, so that the node codes are fetched together with the
 // lattice's descriptor instead of after it.
@@ -1381,6 +1398,42 @@ __global__ __launch_bounds__(block2_threads(ring_of(M), W)) __attribute__((amdgp
       tile_two_steps<LBM_MRT, NC, S, M, W, BUF>(a, q, local, tr.i0, tr.j0, tin, tout, L, tr.code, tr.nx, tr.ny);
       break;
   }
+}
+
+// Many Shan-Chen lattices, one step per launch: one block per tile of any of
+// them, from a table striped by XCD as the two-step batch's (Tile2 records);
+// one table and launch per collision model present, so that each kernel
+// keeps k_sc's register budget.  Each lattice's buffers sit at its own
+// ping-pong phase: the launch parity is offset by the lattice's parity_xor.
+// Per tile the arithmetic is k_sc's, so every lattice's fields equal its run
+// alone bit for bit.
+template <int COLL, int NC, bool BUF>
+__global__ __launch_bounds__(kScThreads) __attribute__((amdgpu_waves_per_eu(sc_min_waves(COLL, NC)))) void k_batch_sc(
+    const Lat* __restrict__ items, const Tile2* __restrict__ tiles, uint32_t stride, const Launch2 q) {
+  using Shape = ScShape<kScThreads>;
+  __shared__ float psiL[NC * Shape::PN];
+  __shared__ uint8_t solL[Shape::PN];
+  const Tile2 tr = tiles[(blockIdx.x & 7u) * stride + (blockIdx.x >> 3)];
+  if (tr.l == kEmptyTile) return;
+  const Lat& a = items[tr.l];
+  Launch2 ql = q;
+  ql.parity ^= a.parity_xor;
+  sc_tile<COLL, NC, LBM_STORE_F32, BUF, kScThreads>(a, ql, q.local_read != 0, tr.i0, tr.j0, psiL, solL);
+}
+
+// Two steps per launch for batched Shan-Chen lattices (every lattice without
+// node rules or power law): k_sc2's tile over the same kind of striped table
+// (tiles of the three-node ring's shape).
+template <int COLL, int NC, bool BUF, int NT>
+__global__ __launch_bounds__(NT) void k_batch_sc2(const Lat* __restrict__ items, const Tile2* __restrict__ tiles,
+                                                  uint32_t stride, const Launch2 q) {
+  __shared__ ScTwoLds<NC, NT> L;
+  const Tile2 tr = tiles[(blockIdx.x & 7u) * stride + (blockIdx.x >> 3)];
+  if (tr.l == kEmptyTile) return;
+  const Lat& a = items[tr.l];
+  Launch2 ql = q;
+  ql.parity ^= a.parity_xor;
+  sc2_tile<COLL, NC, BUF, NT>(a, ql, q.local_read != 0, tr.i0, tr.j0, L);
 }
 
 // Tile-grid shifts that keep every rule node's interior neighbour in its own
@@ -1852,7 +1905,6 @@ uint32_t tile_rows(int h, int w) {
 int ring_width(const lbm_lattice* lat) { return lat->n_neem > 0 ? 2 : 1; }
 
 // ---- Shan-Chen ----------------------------------------------------------------
-constexpr int kScThreads = 512;   // 13 x 32 nodes per tile with the ring: 82 % of the threads own a tile node
 using ScFn = void (*)(const Lat, const Launch2);
 template <int NC, int S, bool BUF>
 ScFn pick_sc_coll(int coll) {
@@ -2587,6 +2639,198 @@ extern "C" int lbm_batch_destroy(lbm_batch* b) {
   (void)hipFree(b->acc);
   (void)hipFree(b->tile2);
   (void)hipFree(b->shift_masks);
+  delete b;
+  return LBM_OK;
+}
+
+// ---- batched Shan-Chen lattices ------------------------------------------------
+// Descriptors and the striped tile tables (one per collision model) are built
+// once (lbm_sc_batch_create); each lattice keeps its own buffers.  A run
+// launches k_batch_sc once per step and model present.
+struct lbm_sc_batch {
+  int nc = 1;
+  int count = 0;
+  bool buf = true;               // every lattice fits 32-bit buffer offsets
+  lbm::Lat* items = nullptr;     // device descriptors
+  lbm::Tile2* tiles[3] = {nullptr, nullptr, nullptr};  // per model: 8 stripes of stride[m] entries
+  size_t stride[3] = {0, 0, 0};
+  size_t ntiles = 0;
+  bool two = false;              // two steps per launch (no lattice has node rules or power law)
+  lbm::Tile2* tiles2[3] = {nullptr, nullptr, nullptr};  // the same for the three-node-ring tiles
+  size_t stride2[3] = {0, 0, 0};
+  size_t ntiles2 = 0;
+};
+
+namespace {
+// striped upload of one tile list (8 stripes, stripe x the x-th eighth)
+int sc_tiles_upload(const std::vector<lbm::Tile2>& list, lbm::Tile2** dev, size_t* stride_out) {
+  using namespace lbm;
+  const size_t n = list.size();
+  if (n == 0) return LBM_OK;
+  const size_t stride = (n + 7) / 8;
+  if (8 * stride >= (1ull << 31)) return fail("Shan-Chen batch of %zu tiles too large", n);
+  Tile2 empty{};
+  empty.l = kEmptyTile;
+  std::vector<Tile2> lay(8 * stride, empty);
+  for (size_t x = 0; x < 8; ++x) {
+    const size_t lo = n * x / 8, hi = n * (x + 1) / 8;
+    for (size_t k = lo; k < hi; ++k) lay[x * stride + (k - lo)] = list[k];
+  }
+  *stride_out = stride;
+  int rc = hip_check(hipMalloc((void**)dev, sizeof(Tile2) * lay.size()), "hipMalloc(Shan-Chen batch tiles)");
+  if (!rc)
+    rc = hip_check(hipMemcpy(*dev, lay.data(), sizeof(Tile2) * lay.size(), hipMemcpyHostToDevice),
+                   "upload Shan-Chen batch tiles");
+  return rc;
+}
+}  // namespace
+
+extern "C" int lbm_sc_batch_destroy(lbm_sc_batch* b);
+
+extern "C" int lbm_sc_batch_create(const lbm_lattice* lats, const lbm_fields* flds, const lbm_neem_rule* neem,
+                                   const int32_t* parity_offset, int32_t count, lbm_sc_batch** out) {
+  using namespace lbm;
+  using Shape = ScShape<kScThreads>;
+  if (!out) return fail("null out");
+  *out = nullptr;
+  if (count < 1 || !lats || !flds) return fail("a Shan-Chen batch needs at least one lattice");
+  std::vector<Lat> host((size_t)count);
+  std::vector<Tile2> list[3], list2[3];
+  bool buf = true, two = true;
+  const uint32_t TI2 = lats[0].nc == 2 ? (uint32_t)TileShape<3, kScCols, sc2_threads(2)>::TI
+                                       : (uint32_t)TileShape<3, kScCols, sc2_threads(1)>::TI;
+  for (int l = 0; l < count; ++l) {
+    const lbm_lattice* L = &lats[l];
+    const lbm_neem_rule* rules = neem ? neem + (size_t)l * LBM_MAX_NEEM : nullptr;
+    int rc = validate(L, &flds[l], rules);
+    if (rc) return rc;
+    if (L->shan_chen == LBM_SC_NONE) return fail("lattice %d of a Shan-Chen batch has no Shan-Chen force", l);
+    if (L->storage != LBM_STORE_F32) return fail("lattice %d: Shan-Chen batches store FP32 populations", l);
+    if (L->nc != lats[0].nc) return fail("every lattice of a batch needs the same num_components");
+    host[l] = make_lat(L, &flds[l], rules);
+    host[l].parity_xor = parity_offset ? (uint32_t)(parity_offset[l] & 1) : 0u;
+    buf = buf && fits_buffer(L);
+    two = two && L->n_neem == 0 && L->tau_mode != LBM_TAU_POWER_LAW;
+    Tile2 t{};
+    t.code = host[l].code;
+    t.l = (uint32_t)l;
+    t.nx = (uint32_t)L->nx;
+    t.ny = (uint32_t)L->ny;
+    const uint32_t tj = ((uint32_t)L->ny + kScCols - 1) / kScCols;
+    for (uint32_t i = 0; i < ((uint32_t)L->nx + Shape::TI - 1) / Shape::TI; ++i)
+      for (uint32_t j = 0; j < tj; ++j) {
+        t.i0 = i * Shape::TI;
+        t.j0 = j * kScCols;
+        list[L->collision].push_back(t);
+      }
+    for (uint32_t i = 0; i < ((uint32_t)L->nx + TI2 - 1) / TI2; ++i)
+      for (uint32_t j = 0; j < tj; ++j) {
+        t.i0 = i * TI2;
+        t.j0 = j * kScCols;
+        list2[L->collision].push_back(t);
+      }
+  }
+  lbm_sc_batch* b = new lbm_sc_batch;
+  b->nc = lats[0].nc;
+  b->count = count;
+  b->buf = buf;
+  b->two = two;
+  int rc = hip_check(hipMalloc((void**)&b->items, sizeof(Lat) * host.size()), "hipMalloc(Shan-Chen batch items)");
+  if (!rc)
+    rc = hip_check(hipMemcpy(b->items, host.data(), sizeof(Lat) * host.size(), hipMemcpyHostToDevice),
+                   "upload Shan-Chen batch items");
+  for (int m = 0; m < 3 && !rc; ++m) {
+    b->ntiles += list[m].size();
+    rc = sc_tiles_upload(list[m], &b->tiles[m], &b->stride[m]);
+    if (!rc && two) {
+      b->ntiles2 += list2[m].size();
+      rc = sc_tiles_upload(list2[m], &b->tiles2[m], &b->stride2[m]);
+    }
+  }
+  if (rc) {
+    lbm_sc_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return LBM_OK;
+}
+
+namespace lbm {
+using ScBatchFn = void (*)(const Lat*, const Tile2*, uint32_t, const Launch2);
+template <int COLL>
+ScBatchFn pick_sc_batch(int nc, bool buf) {
+  return nc == 1 ? (buf ? k_batch_sc<COLL, 1, true> : k_batch_sc<COLL, 1, false>)
+                 : (buf ? k_batch_sc<COLL, 2, true> : k_batch_sc<COLL, 2, false>);
+}
+template <int COLL>
+ScBatchFn pick_sc_batch2(int nc, bool buf) {
+  return nc == 1 ? (buf ? k_batch_sc2<COLL, 1, true, sc2_threads(1)> : k_batch_sc2<COLL, 1, false, sc2_threads(1)>)
+                 : (buf ? k_batch_sc2<COLL, 2, true, sc2_threads(2)> : k_batch_sc2<COLL, 2, false, sc2_threads(2)>);
+}
+}  // namespace lbm
+
+extern "C" int lbm_sc_batch_run(lbm_sc_batch* b, int32_t nsteps, int32_t parity, int32_t first_local,
+                                int32_t* out_parity, void* stream) {
+  using namespace lbm;
+  if (!b) return fail("null Shan-Chen batch");
+  if (!out_parity) return fail("null out_parity");
+  if (nsteps < 0) return fail("nsteps %d < 0", nsteps);
+  if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
+  const ScBatchFn one[3] = {pick_sc_batch<LBM_BGK>(b->nc, b->buf), pick_sc_batch<LBM_TRT>(b->nc, b->buf),
+                            pick_sc_batch<LBM_MRT>(b->nc, b->buf)};
+  const ScBatchFn two[3] = {pick_sc_batch2<LBM_BGK>(b->nc, b->buf), pick_sc_batch2<LBM_TRT>(b->nc, b->buf),
+                            pick_sc_batch2<LBM_MRT>(b->nc, b->buf)};
+  hipStream_t s = (hipStream_t)stream;
+  Launch2 q;
+  memset(&q, 0, sizeof q);
+  uint32_t p = (uint32_t)parity;
+  // as lbm_run with Shan-Chen: pairs of steps first, an odd last step alone
+  const int pairs = b->two ? nsteps / 2 : 0;
+  for (int t = 0; t < pairs; ++t) {
+    q.parity = p;
+    q.local_read = (t == 0 && first_local) ? 1 : 0;
+    q.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
+    q.write_psi = (t == pairs - 1) ? 1 : 0;
+    for (int m = 0; m < 3; ++m) {
+      if (!b->tiles2[m]) continue;
+      hipLaunchKernelGGL(two[m], dim3((unsigned)(8 * b->stride2[m])), dim3(sc2_threads(b->nc)), 0, s, b->items,
+                         b->tiles2[m], (uint32_t)b->stride2[m], q);
+      int rc = hip_check(hipGetLastError(), "Shan-Chen batch two-step launch");
+      if (rc) return rc;
+    }
+    p ^= 1u;
+  }
+  const int rest = nsteps - 2 * pairs;
+  for (int t = 0; t < rest; ++t) {
+    q.parity = p;
+    q.step = t;
+    q.local_read = (t == 0 && pairs == 0 && first_local) ? 1 : 0;
+    q.write_macro = (t == rest - 1) ? 1 : 0;
+    q.write_psi = (t >= rest - 2) ? 1 : 0;
+    for (int m = 0; m < 3; ++m) {
+      if (!b->tiles[m]) continue;
+      hipLaunchKernelGGL(one[m], dim3((unsigned)(8 * b->stride[m])), dim3(kScThreads), 0, s, b->items, b->tiles[m],
+                         (uint32_t)b->stride[m], q);
+      int rc = hip_check(hipGetLastError(), "Shan-Chen batch launch");
+      if (rc) return rc;
+    }
+    p ^= 1u;
+  }
+  *out_parity = (int32_t)p;
+  return LBM_OK;
+}
+
+extern "C" int lbm_sc_batch_tiles(const lbm_sc_batch* b) {
+  return b ? (int)b->ntiles : lbm::fail("null Shan-Chen batch");
+}
+
+extern "C" int lbm_sc_batch_destroy(lbm_sc_batch* b) {
+  if (!b) return LBM_OK;
+  (void)hipFree(b->items);
+  for (int m = 0; m < 3; ++m) {
+    (void)hipFree(b->tiles[m]);
+    (void)hipFree(b->tiles2[m]);
+  }
   delete b;
   return LBM_OK;
 }

@@ -31,7 +31,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 
 class NeemRule(C.Structure):
@@ -151,6 +151,12 @@ def lib():
         "lbm_batch_residual": ([BP, I32, D, P], C.c_int),
         "lbm_batch_destroy": ([BP], C.c_int),
     })
+    sig.update({
+        "lbm_sc_batch_create": ([LP, FP, NP, C.POINTER(C.c_int32), I32, C.POINTER(C.c_void_p)], C.c_int),
+        "lbm_sc_batch_run": ([P, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
+        "lbm_sc_batch_tiles": ([P], C.c_int),
+        "lbm_sc_batch_destroy": ([P], C.c_int),
+    })
     IP = C.POINTER(IBParticles)
     sig.update({
         "lbm_ib_work_ints": ([I32, I32], C.c_int64),
@@ -185,6 +191,7 @@ def exported_symbols():
             "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2", "lbm_batch_release",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy",
+            "lbm_sc_batch_create", "lbm_sc_batch_run", "lbm_sc_batch_tiles", "lbm_sc_batch_destroy",
             "lbm_ib_work_ints", "lbm_ib_init_nodes", "lbm_ib_update", "lbm_ib_reset_force", "lbm_ib_interpolate",
             "lbm_ib_spread", "lbm_ib_mdf", "lbm_ib_particle_force"]
 

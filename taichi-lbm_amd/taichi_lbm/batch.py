@@ -558,3 +558,98 @@ def autotune_grid(runner: BatchRunner, candidates=(2, 4, 8, 16, 32), steps: int 
             best = grid
     runner.grid = best
     return best, timings
+
+
+class ShanChenBatch:
+    """Many Shan-Chen lattices advanced together: one launch per step over the
+    tiles of all of them (lbm_sc_batch_run), instead of one launch per
+    lattice and step.
+
+    ``ShanChenBatch([stepper, ...])`` takes fused Steppers of Shan-Chen
+    lattices (``Stepper(..., boundary_engine_sc=...)``, FP32 storage, the
+    same num_components; collision models, sizes, rules and psi objects may
+    differ).  ``run(n)`` advances every lattice by n steps and leaves each
+    LBField exactly as its own ``Stepper.run(n)`` would: the per-tile step is
+    the one-lattice kernel's, so the fields are equal bit for bit.  Each
+    lattice keeps its own buffers; reading one (``lb_field.vel.to_numpy()``)
+    between runs is allowed."""
+
+    def __init__(self, steppers):
+        self.steppers = list(steppers)
+        if not self.steppers:
+            raise ValueError("a Shan-Chen batch needs at least one lattice")
+        nc = self.steppers[0].lbf.nc
+        for st in self.steppers:
+            if st.sc is None:
+                raise ValueError("every lattice of a ShanChenBatch needs a Shan-Chen force")
+            if st.storage != H.STORE_F32:
+                raise NotImplementedError("Shan-Chen batches store FP32 populations")
+            if st.lbf.nc != nc:
+                raise ValueError("every lattice of a ShanChenBatch needs the same num_components")
+        self.h = None
+        self._key = None
+
+    def _handle(self, structs, offsets):
+        """The batch handle for these buffers and relative parities (rebuilt
+        only when a lattice's buffers or ping-pong phase changed)."""
+        key = tuple((tuple(F.f), tuple(F.macro), tuple(F.psi), F.tau, F.code, F.temperature, o)
+                    for (L, F), o in zip(structs, offsets))
+        n = len(structs)
+        lats = (H.Lattice * n)()
+        flds = (H.Fields * n)()
+        neem = (H.NeemRule * (n * H.MAX_NEEM))()
+        for l, ((L, F), st) in enumerate(zip(structs, self.steppers)):
+            ctypes.memmove(ctypes.byref(lats[l]), ctypes.byref(L), ctypes.sizeof(H.Lattice))
+            ctypes.memmove(ctypes.byref(flds[l]), ctypes.byref(F), ctypes.sizeof(H.Fields))
+            for e in range(L.n_neem):
+                ctypes.memmove(ctypes.byref(neem[l * H.MAX_NEEM + e]), ctypes.byref(st.neem[e]),
+                               ctypes.sizeof(H.NeemRule))
+        # parameters (relaxation, forces, rules) are part of the descriptors too
+        key = key + (bytes(lats), bytes(neem))
+        if self.h is not None and key == self._key:
+            return self.h
+        self.close()
+        off = (ctypes.c_int32 * n)(*offsets)
+        h = ctypes.c_void_p()
+        H.check(H.lib().lbm_sc_batch_create(lats, flds, neem, off, n, ctypes.byref(h)), "lbm_sc_batch_create")
+        self.h, self._key = h, key
+        return h
+
+    def tiles(self) -> int:
+        return H.lib().lbm_sc_batch_tiles(self.h) if self.h is not None else 0
+
+    def run(self, nsteps: int):
+        """Advances every lattice by nsteps steps (asynchronous on the current stream)."""
+        nsteps = int(nsteps)
+        if nsteps <= 0:
+            return
+        states = [st.lbf.state() for st in self.steppers]
+        if len({s["f_valid"] for s in states}) > 1:
+            # some lattices were read back: run the others' pending boundary
+            # pass too, so that every first step reads f in place (a readback
+            # does not change a lattice's later steps)
+            for st in self.steppers:
+                st.lbf.materialize()
+        begun = [st._begin(nsteps) for st in self.steppers]
+        first_local = begun[0][4]
+        p0 = begun[0][3]
+        offsets = [b[3] ^ p0 for b in begun]
+        h = self._handle([(b[0], b[1]) for b in begun], offsets)
+        out = ctypes.c_int32(-1)
+        H.check(H.lib().lbm_sc_batch_run(h, nsteps, p0, first_local, ctypes.byref(out), stream()),
+                "lbm_sc_batch_run")
+        for st, s, off, (L, F, tau_mode, _, _) in zip(self.steppers, states, offsets, begun):
+            s["f2_idx"] = int(out.value) ^ off      # a launch of two steps flips the buffers once
+            st._end(nsteps, L, F, tau_mode)
+
+    def close(self):
+        if self.h is not None:
+            H.lib().lbm_sc_batch_destroy(self.h)
+            self.h = None
+            self._key = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -214,12 +214,15 @@ class Stepper:
                 return
         self._run(nsteps)
 
-    def _run(self, nsteps: int):
+    def _begin(self, nsteps: int):
+        """Prepares a run of nsteps: compiles the codes if the geometry changed,
+        encodes 16-bit storage if needed; returns the structs, the tau mode,
+        the parity of the buffer the run reads and whether its first step
+        reads f in place."""
         lbf = self.lbf
         if self._compiled_version != lbf.geometry_version:
             self._compile()
-        shifted = self.storage != H.STORE_F32
-        if shifted and lbf._pop_owner is not self:
+        if self.storage != H.STORE_F32 and lbf._pop_owner is not self:
             self._encode16()
         st = lbf.state()
         tau_mode = self._tau_mode()
@@ -232,6 +235,12 @@ class Stepper:
             # a launch writes its psi into plane set parity ^ 1; a run of two
             # or more leaves the psi of its last two steps in both sets
             self.sc.keep_previous(parity ^ 1)
+        return L, F, tau_mode, parity, first_local
+
+    def _run(self, nsteps: int):
+        lbf = self.lbf
+        L, F, tau_mode, parity, first_local = self._begin(nsteps)
+        st = lbf.state()
         if self.blocked and self.no_alloc:
             out = ctypes.c_int32(-1)
             H.check(H.lib().lbm_run_blocked_noalloc(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity,
@@ -254,6 +263,15 @@ class Stepper:
             H.check(H.lib().lbm_run(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity, first_local,
                                     stream()), "lbm_run")
             st["f2_idx"] = (parity + nsteps) % 2
+        self._end(nsteps, L, F, tau_mode)
+
+    def _end(self, nsteps: int, L, F, tau_mode):
+        """After a run that left the newest f2 in buffer state["f2_idx"]: the
+        LBField's lazy state (boundary pass and pressure observable of a
+        readback)."""
+        lbf = self.lbf
+        st = lbf.state()
+        shifted = self.storage != H.STORE_F32
         st["f_valid"] = False
         lbf._pop_owner = self if shifted else None
         if tau_mode == H.TAU_POWER_LAW:
