@@ -608,6 +608,21 @@ def ib_leg(NX=2048, NY=1024, n_particles=256, steps=200, mdf=3):
         coll.apply(lbf)
         bce.apply_boundary_conditions(lbf)
 
+    from taichi_lbm.stepper import Stepper
+    fused = Stepper(lbf, macro, coll, bce, ib_engine=ibe, sphere_field=sp)
+
+    def fused_step(t):
+        pos, vel, ang = cases.ib_particle_state(case, t)
+        coupler.SphereInfoUpdata(sp, pos, vel, ang)
+        fused.run(1)
+        ibe.ComputeParticleForce(sp)
+
+    def fused_run(t0, n):
+        # particles held in place over n steps: one lbm_ib_run launch sequence
+        pos, vel, ang = cases.ib_particle_state(case, t0)
+        coupler.SphereInfoUpdata(sp, pos, vel, ang)
+        fused.run(n)
+
     for t in range(10):
         step(t)
     torch.cuda.synchronize()
@@ -623,6 +638,20 @@ def ib_leg(NX=2048, NY=1024, n_particles=256, steps=200, mdf=3):
     e1.record(s)
     torch.cuda.synchronize()
     ib_us = e0.elapsed_time(e1) / steps * 1e3
+    for t in range(10):
+        fused_step(t)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for t in range(steps):
+        fused_step(t)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms_fused = e0.elapsed_time(e1) / steps
+    e0.record(s)
+    fused_run(0, steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms_fixed = e0.elapsed_time(e1) / steps
     markers = int(sp.IBInsiderCount[None])
     u = lbf.vel.to_numpy()
     force = coupler.GetSphereForce(sp)
@@ -630,6 +659,10 @@ def ib_leg(NX=2048, NY=1024, n_particles=256, steps=200, mdf=3):
         "steps": steps + 10, "ms_per_step": round(ms, 4), "mlups": round(NX * NY / (ms * 1e-3) / 1e6, 1),
         "ib_us_per_step": round(ib_us, 1), "markers": markers, "mdf_iterations": mdf,
         "path": "granular sequence (one HIP kernel per reference kernel) with the IB kernels of csrc/lbm_ibm.hip",
+        "fused_ms_per_step": round(ms_fused, 4), "fused_mlups": round(NX * NY / (ms_fused * 1e-3) / 1e6, 1),
+        "fused_path": "Stepper(ib_engine, sphere_field): lbm_ib_run, particles moved every step",
+        "fused_fixed_ms_per_step": round(ms_fixed, 4),
+        "fused_fixed_path": "the same with the particles held over one run of all steps",
         "finite": bool(np.isfinite(u).all() and np.isfinite(force).all()),
         "max_abs_particle_force": float(np.abs(force).max())}}
     del case, sp

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 14
+#define LBM_ABI_VERSION 15
 
 /* status codes */
 #define LBM_OK 0
@@ -486,6 +486,19 @@ int lbm_ib_spread(const lbm_lattice* lat, const lbm_gfields* g, const lbm_ib_par
 /* IBEngine.MultiDirectForcingLoop, ibm.py:248-253: reset_force, then
  * `iterations` x (interpolate, spread).                                    */
 int lbm_ib_mdf(const lbm_lattice* lat, const lbm_gfields* g, const lbm_ib_particles* p, int32_t iterations,
+               void* stream);
+/* nsteps lbm_solve() steps of a lattice with immersed-boundary particles,
+ * IBEngine.MultiDirectForcingLoop (`iterations`) between the velocity and
+ * the collision of every step (ibm.py:248-253), fused: per step the
+ * stencil nodes of the marker list get the step's density and velocity in
+ * `scratch` (vel 2, rho nc, total_rho 1, body_force 2*nc planes of
+ * lat->plane_stride), the forcing loop runs on them, one fused step runs
+ * every node, and the stencil nodes are stepped again with the spread force.
+ * The marker list is the caller's last lbm_ib_update (particles do not move
+ * within a call).  FP32 storage, no node rules, Shan-Chen or power law.
+ * parity / first_local / *out_parity as lbm_run_blocked.                  */
+int lbm_ib_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_ib_particles* p, const lbm_gfields* scratch,
+               int32_t iterations, int32_t nsteps, int32_t parity, int32_t first_local, int32_t* out_parity,
                void* stream);
 /* IBEngine.ComputeParticleForce, ibm.py:171-185: each sphere with markers
  * in the list of the last lbm_ib_update gets force = sum of their forces and

@@ -355,6 +355,24 @@ struct PsiNone {
   __device__ __forceinline__ bool solid(int) const { return false; }
 };
 
+// Immersed-boundary correction of a node's force (no Shan-Chen): the body
+// force that multi-direct forcing spread onto the node (nc x 2 planes), added
+// to the node's own force before its velocity is taken (ibm.py:229-248: the
+// spread adds F/(2 rho_tot) to u and F rho_c/rho_tot to each component's
+// force, which is the velocity's half-force term of the added force).
+struct IbForce {
+  const float* df;
+  uint32_t ps, n;
+  __device__ __forceinline__ float at(int, int) const { return 0.0f; }
+  __device__ __forceinline__ bool solid(int) const { return false; }
+  __device__ __forceinline__ float fx(int c) const { return df[(size_t)(2 * c) * ps + n]; }
+  __device__ __forceinline__ float fy(int c) const { return df[(size_t)(2 * c + 1) * ps + n]; }
+};
+template <class P, class = void>
+struct has_extra_force : std::false_type {};
+template <class P>
+struct has_extra_force<P, std::void_t<decltype(&P::fx)>> : std::true_type {};
+
 // S1..S6 of one step at node n = (i, j).  state_in is the per-node relaxation
 // state (BGK tau, TRT tau_sym, MRT 1/tau) for LBM_TAU_FIELD / _POWER_LAW.
 // Shan-Chen reads psi of this step through `psi` (PsiPlanes and the like).
@@ -439,13 +457,17 @@ __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t 
   }
 
   // S3 force_density (lbm.py:215-220)
-  const bool force = a.force || a.sc != LBM_SC_NONE;
+  const bool force = a.force || a.sc != LBM_SC_NONE || has_extra_force<PS>::value;
   const R inv_rtot = rcp(o.rtot);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const R frac = o.rho[c] * inv_rtot;
     o.Fx[c] = o.scx[c] + frac * (R)a.gx;
     o.Fy[c] = o.scy[c] + frac * (R)a.gy;
+    if constexpr (has_extra_force<PS>::value) {
+      o.Fx[c] += (R)psi.fx(c);
+      o.Fy[c] += (R)psi.fy(c);
+    }
   }
 
   // S4 velocity (lbm.py:223-237)
@@ -629,6 +651,104 @@ __global__ __launch_bounds__(kBlock) void k_fused(const Lat a, const Launch q) {
   const uint32_t n = logical_block(q) * kBlock + threadIdx.x;
   if (n >= a.N) return;
   node_update<COLL, NC, S, MODE>(a, q, n, q.local_read != 0);
+}
+
+// ---------------------------------------------------------------------------
+// Immersed-boundary coupling in the fused step.  The reference runs multi-
+// direct forcing between MacroscopicEngine.velocity and the collision
+// (ibm.py:248-253), i.e. inside one step.  Here a step is: (0) at every
+// stencil node of the marker list, the step's density, force and velocity
+// from the pulled populations (step_math: what the fused step computes
+// there) into scratch planes, the spread force cleared; (1) multi-direct
+// forcing on the scratch planes (lbm_ib_mdf); (2) the fused step on every
+// node; (3) at the stencil nodes again, the step recomputed with the spread
+// force added, over the fused step's result.  Nodes touched by several
+// markers are computed by each with the same inputs and write the same
+// values.  Lattices with node rules, Shan-Chen or a power law do not take
+// this path (their step at a node needs more than its own pulls).
+// ---------------------------------------------------------------------------
+struct IbStencil {
+  const int2* group;
+  const int32_t* count;
+  const float* node_pos;   // 2 planes of cap
+  uint32_t cap, K;
+  int32_t max_number;
+  float* vel;              // scratch: 2 planes (u of the step at stencil nodes)
+  float* rho;              // nc planes
+  float* total_rho;        // 1 plane
+  float* body_force;       // nc * 2 planes (the spread force)
+};
+
+template <int COLL, int NC, int MODE>   // MODE 0: macros into the scratch planes; 1: fix-up
+__global__ __launch_bounds__(256) void k_ib_stencil(const Lat a, const IbStencil ib, uint32_t parity, int32_t local,
+                                                    int32_t write_macro) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ib.cap || (int32_t)t >= *ib.count) return;
+  const int2 mn = ib.group[t];
+  if (mn.x < 0 || mn.x >= ib.max_number || mn.y < 0 || (uint32_t)mn.y >= ib.K) return;
+  const uint32_t q = (uint32_t)mn.x * ib.K + (uint32_t)mn.y;
+  const float px = fminf(fmaxf(ib.node_pos[q], -2.0f), (float)a.nx + 1.0f);
+  const float py = fminf(fmaxf(ib.node_pos[ib.cap + q], -2.0f), (float)a.ny + 1.0f);
+  const int x0 = (int)floorf(px), x1 = (int)ceilf(px), y0 = (int)floorf(py), y1 = (int)ceilf(py);
+  const uint32_t ps = a.ps;
+  const float* __restrict__ src = (const float*)a.pop[parity];
+  for (int ix = x0; ix <= x1; ++ix) {
+    if (ix < 0 || ix >= (int)a.nx) continue;
+    for (int iy = y0; iy <= y1; ++iy) {
+      if (iy < 0 || iy >= (int)a.ny) continue;
+      const uint32_t i = (uint32_t)ix, j = (uint32_t)iy, n = i * a.ny + j;
+      const uint32_t code = a.code[n];
+      if (!(code & LBM_CODE_ACTIVE)) {
+        if (MODE == 0) {   // as the granular fields outside the fluid group
+          ib.vel[n] = 0.0f;
+          ib.vel[ps + n] = 0.0f;
+          ib.total_rho[n] = 0.0f;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            ib.rho[(size_t)c * ps + n] = 0.0f;
+            ib.body_force[(size_t)(2 * c) * ps + n] = 0.0f;
+            ib.body_force[(size_t)(2 * c + 1) * ps + n] = 0.0f;
+          }
+        }
+        continue;
+      }
+      float f[NC][9];
+      if (local) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) f[c][k] = src[(size_t)(c * 9 + k) * ps + n];
+      } else {
+        gather<NC, LBM_STORE_F32>(src, ps, Nbr(i, j, a.nx, a.ny), code & LBM_CODE_BOUNCE_MASK, f);
+      }
+      float st_in[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+        st_in[c] = a.tau_mode == LBM_TAU_FIELD ? ((const float*)a.tau)[(size_t)c * ps + n] : 0.0f;
+      StepOut<NC, float> o;
+      if constexpr (MODE == 0) {
+        step_math<COLL, NC, float>(a, PsiNone<float>{}, n, i, j, f, st_in, o);
+        ib.vel[n] = o.ux;
+        ib.vel[ps + n] = o.uy;
+        ib.total_rho[n] = o.rtot;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          ib.rho[(size_t)c * ps + n] = o.rho[c];
+          ib.body_force[(size_t)(2 * c) * ps + n] = 0.0f;
+          ib.body_force[(size_t)(2 * c + 1) * ps + n] = 0.0f;
+        }
+      } else {
+        step_math<COLL, NC, float>(a, IbForce{ib.body_force, ps, n}, n, i, j, f, st_in, o);
+        float* dst = (float*)a.pop[parity ^ 1u];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int k = 0; k < 9; ++k) dst[(size_t)(c * 9 + k) * ps + n] = o.f[c][k];
+        store_macro<NC, float>(a, (float*)a.mac[parity ^ 1u], n, o, (code & LBM_CODE_KEEP_MACRO) != 0,
+                               write_macro != 0);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2099,6 +2219,24 @@ int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_ru
   return LBM_OK;
 }
 
+// IB-coupled fused steps (k_ib_stencil around one-step launches).
+using IbStencilFn = void (*)(const Lat, const IbStencil, uint32_t, int32_t, int32_t);
+template <int MODE>
+IbStencilFn pick_ib_stencil(int nc, int coll) {
+  if (nc == 1) {
+    switch (coll) {
+      case LBM_BGK: return k_ib_stencil<LBM_BGK, 1, MODE>;
+      case LBM_TRT: return k_ib_stencil<LBM_TRT, 1, MODE>;
+      default: return k_ib_stencil<LBM_MRT, 1, MODE>;
+    }
+  }
+  switch (coll) {
+    case LBM_BGK: return k_ib_stencil<LBM_BGK, 2, MODE>;
+    case LBM_TRT: return k_ib_stencil<LBM_TRT, 2, MODE>;
+    default: return k_ib_stencil<LBM_MRT, 2, MODE>;
+  }
+}
+
 // Workspace of lbm_run_blocked_noalloc: the second relaxation-state buffer of
 // two-step power-law runs (nc planes of plane_stride floats), a fixed device
 // array instead of a caller's buffer.  Every GPU holds its own copy, at its
@@ -2137,6 +2275,63 @@ extern "C" int lbm_run_blocked(const lbm_lattice* lat, const lbm_fields* fld, co
                                int32_t nsteps, int32_t parity, int32_t first_local, int32_t tile_cols,
                                void* tau_next, int32_t* out_parity, void* stream) {
   return lbm::run_blocked(lat, fld, neem, nsteps, parity, first_local, tile_cols, tau_next, out_parity, stream);
+}
+
+extern "C" int lbm_ib_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_ib_particles* p,
+                          const lbm_gfields* scratch, int32_t iterations, int32_t nsteps, int32_t parity,
+                          int32_t first_local, int32_t* out_parity, void* stream) {
+  using namespace lbm;
+  int rc = validate(lat, fld, nullptr);
+  if (rc) return rc;
+  if (!p || !scratch || !out_parity) return fail("null particle set, scratch fields or out_parity");
+  if (lat->n_neem != 0 || lat->shan_chen != LBM_SC_NONE || lat->tau_mode == LBM_TAU_POWER_LAW ||
+      lat->storage != LBM_STORE_F32)
+    return fail("IB-coupled fused steps need FP32 storage without node rules, Shan-Chen or a power law");
+  if (nsteps < 0) return fail("nsteps %d < 0", nsteps);
+  if (iterations < 0) return fail("MDFIteration = %d", iterations);
+  if (parity != 0 && parity != 1) return fail("parity must be 0 or 1");
+  if (p->max_number < 1 || p->max_num_ib < 1 || !p->group || !p->count || !p->node_pos)
+    return fail("bad particle set");
+  const uint64_t cap = (uint64_t)p->max_number * (uint64_t)p->max_num_ib;
+  if (cap >= (1ull << 31)) return fail("particle capacity too large");
+  if (!scratch->vel || !scratch->rho || !scratch->total_rho || !scratch->body_force)
+    return fail("IB scratch planes missing (vel, rho, total_rho, body_force)");
+  hipStream_t s = (hipStream_t)stream;
+  const Lat a = make_lat(lat, fld, nullptr);
+  IbStencil ib;
+  ib.group = (const int2*)p->group;
+  ib.count = p->count;
+  ib.node_pos = p->node_pos;
+  ib.cap = (uint32_t)cap;
+  ib.K = (uint32_t)p->max_num_ib;
+  ib.max_number = p->max_number;
+  ib.vel = scratch->vel;
+  ib.rho = scratch->rho;
+  ib.total_rho = scratch->total_rho;
+  ib.body_force = scratch->body_force;
+  const IbStencilFn macro = pick_ib_stencil<0>(lat->nc, lat->collision);
+  const IbStencilFn fixup = pick_ib_stencil<1>(lat->nc, lat->collision);
+  const KernelFn step = pick(lat->nc, lat->storage, lat->collision, MODE_STEP);
+  const dim3 gib((unsigned)((cap + 255) / 256)), bib(256);
+  Launch q = make_launch(a.N);
+  uint32_t par = (uint32_t)parity;
+  for (int t = 0; t < nsteps; ++t) {
+    const int32_t local = (t == 0 && first_local) ? 1 : 0;
+    const int32_t last = (t == nsteps - 1) ? 1 : 0;
+    hipLaunchKernelGGL(macro, gib, bib, 0, s, a, ib, par, local, last);
+    if ((rc = hip_check(hipGetLastError(), "IB stencil launch"))) return rc;
+    if ((rc = lbm_ib_mdf(lat, scratch, p, iterations, stream))) return rc;
+    q.parity = par;
+    q.step = t;
+    q.local_read = local;
+    q.write_macro = last;
+    if ((rc = launch(step, a, q, s))) return rc;
+    hipLaunchKernelGGL(fixup, gib, bib, 0, s, a, ib, par, local, last);
+    if ((rc = hip_check(hipGetLastError(), "IB fix-up launch"))) return rc;
+    par ^= 1u;
+  }
+  *out_parity = (int32_t)par;
+  return LBM_OK;
 }
 
 extern "C" int lbm_run_blocked_noalloc(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,

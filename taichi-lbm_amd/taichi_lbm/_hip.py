@@ -31,7 +31,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 
 class NeemRule(C.Structure):
@@ -167,6 +167,7 @@ def lib():
         "lbm_ib_spread": ([LP, GP, IP, P], C.c_int),
         "lbm_ib_mdf": ([LP, GP, IP, I32, P], C.c_int),
         "lbm_ib_particle_force": ([IP, P], C.c_int),
+        "lbm_ib_run": ([LP, FP, IP, GP, I32, I32, I32, I32, C.POINTER(C.c_int32), P], C.c_int),
     })
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -193,7 +194,7 @@ def exported_symbols():
             "lbm_graph_destroy", "lbm_batch_residual", "lbm_batch_destroy",
             "lbm_sc_batch_create", "lbm_sc_batch_run", "lbm_sc_batch_tiles", "lbm_sc_batch_destroy",
             "lbm_ib_work_ints", "lbm_ib_init_nodes", "lbm_ib_update", "lbm_ib_reset_force", "lbm_ib_interpolate",
-            "lbm_ib_spread", "lbm_ib_mdf", "lbm_ib_particle_force"]
+            "lbm_ib_spread", "lbm_ib_mdf", "lbm_ib_particle_force", "lbm_ib_run"]
 
 
 def check(rc: int, what: str = ""):

@@ -41,7 +41,7 @@ def _same_nodes(g1, g2, NY) -> bool:
 class Stepper:
     def __init__(self, lb_field, macroscopic_engine, collision_engine, boundary_engine, power_law=False,
                  boundary_engine_sc=None, advance_time=False, storage="f32", blocked=True, tile_cols=0,
-                 no_alloc=False):
+                 no_alloc=False, ib_engine=None, sphere_field=None):
         """blocked: run two steps per launch where the lattice allows it
         (lbm_run_blocked; bit-identical to one step per launch), on tiles of
         tile_cols = 16, 32 or 64 columns (0: the library's default; "auto":
@@ -59,7 +59,14 @@ class Stepper:
         decodes them into the LBField's FP32 fields (active nodes).  Runs
         without a readback in between equal one run; a readback rounds the
         post-boundary populations, which the next run starts from, to the
-        storage format."""
+        storage format.  ib_engine, sphere_field: immersed-boundary particles
+        (ibm.py IBEngine, SphereIB) coupled into every step, as a case's
+        lbm_solve() with ib_engine.MultiDirectForcingLoop(sphere_field,
+        lb_field) between MacroscopicEngine.velocity and the collision
+        (lbm_ib_run: one fused step per launch plus the forcing at the
+        markers' stencil nodes); the markers are the last SphereIBUpdate's
+        (IBDEMCouplerEngine.SphereInfoUpdata before each run moves them).
+        FP32 storage, no node rules, Shan-Chen or power law."""
         self.lbf = lb_field
         self.blocked = bool(blocked)
         self.no_alloc = bool(no_alloc)
@@ -78,6 +85,12 @@ class Stepper:
         self.storage = _STORAGE[storage]
         self._pop16 = None       # (2, nc*9, ps) shifted 16-bit populations (storage f16 / bf16)
         self.sc = lb_field.sc_field
+        self.ib_engine, self.sphere_field = ib_engine, sphere_field
+        self._ib_scratch = None
+        if (ib_engine is None) != (sphere_field is None):
+            raise ValueError("IB coupling needs both ib_engine and sphere_field")
+        if ib_engine is not None and (self.sc is not None or self.power_law or storage != "f32"):
+            raise NotImplementedError("IB-coupled fused steps need FP32 storage without Shan-Chen or a power law")
         self._psi_fixed = []
         if self.sc is not None:
             if boundary_engine_sc is None:
@@ -106,6 +119,9 @@ class Stepper:
         for e, (kind, d, ux, uy, rho) in enumerate(rules):
             arr[e].kind, arr[e].direction, arr[e].ux, arr[e].uy, arr[e].rho = kind, d, ux, uy, rho
         self.neem = arr
+        if getattr(self, "ib_engine", None) is not None and rules:
+            raise NotImplementedError("IB-coupled fused steps need a boundary list without node rules "
+                                      "(NEEM, VelocityBB, OpenNeumann, ...): use the granular methods")
         self._compiled_version = lbf.geometry_version
 
     def _tau_mode(self):
@@ -241,7 +257,23 @@ class Stepper:
         lbf = self.lbf
         L, F, tau_mode, parity, first_local = self._begin(nsteps)
         st = lbf.state()
-        if self.blocked and self.no_alloc:
+        if self.ib_engine is not None:
+            lbf = self.lbf
+            if self._ib_scratch is None:
+                t = torch()
+                self._ib_scratch = t.zeros((3 + 3 * lbf.nc, lbf.ps), dtype=t.float32, device=device())
+            sc = self._ib_scratch
+            G = H.GFields()
+            G.vel = H.ptr(sc[0])
+            G.total_rho = H.ptr(sc[2])
+            G.rho = H.ptr(sc[3])
+            G.body_force = H.ptr(sc[3 + lbf.nc])
+            out = ctypes.c_int32(-1)
+            H.check(H.lib().lbm_ib_run(ctypes.byref(L), ctypes.byref(F), ctypes.byref(self.sphere_field.abi()),
+                                       ctypes.byref(G), self.ib_engine.MDFIteration, nsteps, parity, first_local,
+                                       ctypes.byref(out), stream()), "lbm_ib_run")
+            st["f2_idx"] = int(out.value)
+        elif self.blocked and self.no_alloc:
             out = ctypes.c_int32(-1)
             H.check(H.lib().lbm_run_blocked_noalloc(ctypes.byref(L), ctypes.byref(F), self.neem, nsteps, parity,
                                                     first_local, self.tile_cols, ctypes.byref(out), stream()),

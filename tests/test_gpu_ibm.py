@@ -182,3 +182,62 @@ def test_ib_coupled_granular_run_matches_oracle(coll):
     _close("sphere torque", coupler.GetSphereTorque(sp), o.torque[:n], atol=1e-4, rtol=1e-3)
     # the particles push the fluid: the coupled run differs from one without particles
     assert np.abs(lbf.vel.to_numpy()).max() > 1e-3
+
+
+@pytest.mark.parametrize("coll", ["bgk", "mrt"])
+def test_ib_coupled_fused_stepper_matches_granular(coll):
+    """The fused IB-coupled step (Stepper(..., ib_engine, sphere_field):
+    lbm_ib_run) against the granular sequence with MultiDirectForcingLoop
+    between velocity and collide, on the same moving particles: the same
+    discrete problem, so the fields agree within the FP32 parity bounds
+    (the spread force enters the fused step as a force term, so its sums
+    round differently)."""
+    from taichi_lbm.stepper import Stepper
+    print(f"case IB-coupled fused vs granular 96x64 {coll}, 30 steps", flush=True)
+
+    def make():
+        case = cases.ib_particles(NX=96, NY=64, n_particles=3, radius=(5.0, 8.0), u_particle=0.02,
+                                  collision=coll, g=(2e-6, 0.0), seed=4)
+        cases.initialise(case)
+        s = case["spheres"]
+        n = len(s["radius"])
+        case["sp"].init_Balls(n, np.arange(n), s["pos"], s["vel"], np.zeros((n, 2), np.float32),
+                              np.zeros(n, np.float32), s["radius"], s["angle"])
+        case["sp"].init_IB_nodes()
+        return case
+
+    g, fz = make(), make()
+    ibe_g, ibe_f = IBEngine(2), IBEngine(2)
+    cg, cf = IBDEMCouplerEngine(g["lbf"]), IBDEMCouplerEngine(fz["lbf"])
+    st = Stepper(fz["lbf"], fz["macro"], fz["coll"], fz["bce"], ib_engine=ibe_f, sphere_field=fz["sp"])
+    for step in range(30):
+        pos, pvel, ang = cases.ib_particle_state(g, step)
+        g["macro"].density(g["lbf"])
+        g["macro"].force_density(g["lbf"])
+        g["macro"].velocity(g["lbf"])
+        cg.SphereInfoUpdata(g["sp"], pos, pvel, ang)
+        ibe_g.MultiDirectForcingLoop(g["sp"], g["lbf"])
+        ibe_g.ComputeParticleForce(g["sp"])
+        g["coll"].apply(g["lbf"])
+        g["bce"].apply_boundary_conditions(g["lbf"])
+        cf.SphereInfoUpdata(fz["sp"], pos, pvel, ang)
+        st.run(1)
+        ibe_f.ComputeParticleForce(fz["sp"])
+        if step == 14:       # a readback in between
+            _ = fz["lbf"].vel.to_numpy()
+    for name in ("f", "f2", "rho", "vel"):
+        check(name, getattr(fz["lbf"], name).to_numpy(), getattr(g["lbf"], name).to_numpy(), growth=3.0)
+    _close("sphere force", cf.GetSphereForce(fz["sp"]), cg.GetSphereForce(g["sp"]), atol=1e-5, rtol=1e-3)
+    _close("sphere torque", cf.GetSphereTorque(fz["sp"]), cg.GetSphereTorque(g["sp"]), atol=1e-4, rtol=1e-3)
+    # without the particles the flow would stay uniform: the coupling acted
+    assert np.abs(fz["lbf"].vel.to_numpy()).max() > 1e-3
+
+
+def test_ib_fused_stepper_refuses_what_it_cannot_run():
+    from taichi_lbm.stepper import Stepper
+    case = cases.ib_particles(NX=40, NY=32, n_particles=1, radius=(3.0, 4.0))
+    with pytest.raises(NotImplementedError):
+        Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True, ib_engine=IBEngine(1),
+                sphere_field=case["sp"])
+    with pytest.raises(ValueError):
+        Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], ib_engine=IBEngine(1))
