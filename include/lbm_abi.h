@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 15
+#define LBM_ABI_VERSION 16
 
 /* status codes */
 #define LBM_OK 0
@@ -89,7 +89,9 @@ enum {
   LBM_RULE_OPEN_NEUMANN = 4, /* OpenNeumann.apply, :321-334 (unknowns := f(x_nb)) */
   LBM_RULE_PRESSURE_ABC = 5, /* PressureABC.apply, :190-205 (unknowns := -f + 2 w rho_value (...), *
                               * u_w = 1.5 u(x) - 0.5 u(x_nb))                                      */
-  LBM_RULE_VELOCITY_EQ = 6   /* VelocityEquilibrium.apply, :135-145 (all 9 := feq(u_w, rho_w))     */
+  LBM_RULE_VELOCITY_EQ = 6,  /* VelocityEquilibrium.apply, :135-145 (all 9 := feq(u_w, rho_w))     */
+  LBM_RULE_OPEN_CONVECTIVE1 = 7 /* OpenConvective1order.apply, :343-359: unknowns := (f + l f(x_nb)) / *
+                                 * (1 + l), l = |u(x_nb)|, then density and velocity                  */
 };
 
 /* Where the relaxation time comes from.  The per-node state (lbm_fields.tau,
@@ -382,7 +384,10 @@ enum {
   LBM_BC_VELOCITY_BB = 6,    /* VelocityBB.apply, :85-102 (additive, rule.ux/uy, direction) */
   LBM_BC_OPEN_NEUMANN = 7,   /* OpenNeumann.apply, :321-334 (copy unknowns, then density and velocity) */
   LBM_BC_PRESSURE_ABC = 8,   /* PressureABC.apply, :190-205 (rule.rho, direction)               */
-  LBM_BC_VELOCITY_EQ = 9     /* VelocityEquilibrium.apply, :135-145 (rule.ux/uy, direction)     */
+  LBM_BC_VELOCITY_EQ = 9,    /* VelocityEquilibrium.apply, :135-145 (rule.ux/uy, direction)     */
+  LBM_BC_OPEN_CONVECTIVE1 = 10, /* OpenConvective1order.apply, :343-359 (direction)                */
+  LBM_BC_OPEN_EXTRAPOLATION = 11 /* OpenExtrapolation.apply, :399-413: unknowns := 2 f(x - c_d)   *
+                                  * - f(x - 2 c_d) (direction)                                    */
 };
 int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                    const int32_t* group, int32_t count, void* stream);

@@ -615,12 +615,26 @@ class BC:
                 for k in UNKNOWN[d]:
                     cv = dt.type(C[k, 0]) * uw[0] + dt.type(C[k, 1]) * uw[1]
                     lat.f[I, J, c, k] = lat.f[I, J, c, k] + 6 * w[k] * rho_w * cv
-        elif self.kind == "open_neumann":    # :321-334, OpenBoundary.density/vel :274-298
+        elif self.kind == "open_extrapolation":   # :399-413, density / vel not recomputed (:412-413)
             d = self.direction
             I2, J2 = I - C[d, 0], J - C[d, 1]
+            I3, J3 = I2 - C[d, 0], J2 - C[d, 1]
             for c in range(lat.nc):
                 for k in UNKNOWN[d]:
-                    lat.f[I, J, c, k] = lat.f[I2, J2, c, k]
+                    lat.f[I, J, c, k] = 2 * lat.f[I2, J2, c, k] - lat.f[I3, J3, c, k]
+        elif self.kind in ("open_neumann", "open_convective1"):   # :321-334 / :343-359, density/vel :274-298
+            d = self.direction
+            I2, J2 = I - C[d, 0], J - C[d, 1]
+            if self.kind == "open_neumann":
+                for c in range(lat.nc):
+                    for k in UNKNOWN[d]:
+                        lat.f[I, J, c, k] = lat.f[I2, J2, c, k]
+            else:
+                U = lat.vel[I2, J2]
+                lam = np.sqrt(U[:, 0] * U[:, 0] + U[:, 1] * U[:, 1])     # lambdda = 1. * |u(x - c_d)|
+                for c in range(lat.nc):
+                    for k in UNKNOWN[d]:
+                        lat.f[I, J, c, k] = (lat.f[I, J, c, k] + lam * lat.f[I2, J2, c, k]) / (1.0 + lam)
             tot = np.zeros(len(I), dt)
             for c in range(lat.nc):
                 r = np.zeros(len(I), dt)

@@ -238,16 +238,30 @@ __device__ void apply_rule(const Lat& a, const V& v, const lbm_neem_rule& r, con
     }
     return;
   }
-  if (r.kind == LBM_RULE_OPEN_NEUMANN) {
-    // BoundaryCondition.py:327-334: unknowns := f(x - c_d), then density and
-    // velocity of the node (OpenBoundary.density / vel, :274-298)
+  if (r.kind == LBM_RULE_OPEN_NEUMANN || r.kind == LBM_RULE_OPEN_CONVECTIVE1) {
+    // BoundaryCondition.py:327-334: unknowns := f(x - c_d); :349-359:
+    // unknowns := (f + l f(x - c_d)) / (1 + l), l = |u(x - c_d)| of the
+    // step's velocity pass; then density and velocity of the node
+    // (OpenBoundary.density / vel, :274-298)
+    if (r.kind == LBM_RULE_OPEN_NEUMANN) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
+      for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        const R g = neem_unknown(d, k) ? v.pop(y, bm_y, c, k) : f[c][k];
-        f[c][k] = g;
-      }
+        for (int k = 0; k < 9; ++k) {
+          const R g = neem_unknown(d, k) ? v.pop(y, bm_y, c, k) : f[c][k];
+          f[c][k] = g;
+        }
+    } else {
+      const R ux2 = v.macro(y, NC + 1), uy2 = v.macro(y, NC + 2);
+      const R lam = sqrt(ux2 * ux2 + uy2 * uy2);
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const R g = neem_unknown(d, k) ? (f[c][k] + lam * v.pop(y, bm_y, c, k)) / (R(1) + lam) : f[c][k];
+          f[c][k] = g;
+        }
+    }
     if (mac_override) {
       R rtot = R(0), vx = R(0), vy = R(0);
 #pragma unroll
@@ -1732,7 +1746,7 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
   if (lat->n_neem > 0 && !neem) return fail("missing NEEM rule table");
   for (int r = 0; r < lat->n_neem; ++r) {
     if (neem[r].direction < 1 || neem[r].direction > 4) return fail("NEEM rule %d: direction %d", r, neem[r].direction);
-    if (neem[r].kind < LBM_NEEM_VELOCITY || neem[r].kind > LBM_RULE_VELOCITY_EQ)
+    if (neem[r].kind < LBM_NEEM_VELOCITY || neem[r].kind > LBM_RULE_OPEN_CONVECTIVE1)
       return fail("node rule %d: kind %d", r, neem[r].kind);
   }
   if (!fld->f[0] || !fld->f[1] || !fld->macro[0] || !fld->macro[1] || !fld->code)

@@ -341,13 +341,36 @@ __global__ void k_boundary(const G a) {
           float* fp = P(a.g.f, c * 9 + k, a.ps) + n;
           *fp = -*fp + 2.0f * wgt(k) * a.rule.rho * (1.0f + 4.5f * cu * cu - 1.5f * uw2);
         }
-  } else if (a.kind == LBM_BC_OPEN_NEUMANN) {
-    // copy the unknowns from x - c_d, then OpenBoundary.density / vel at x (:274-298)
+  } else if (a.kind == LBM_BC_OPEN_EXTRAPOLATION) {
+    // unknowns := 2 f(x - c_d) - f(x - 2 c_d); density and velocity untouched
+    // (commented out in the reference, :412-413)
     const int d = a.rule.direction;
     const uint32_t n2 = (uint32_t)((int)i - cx(d)) * ny + (uint32_t)((int)j - cy(d));
+    const uint32_t n3 = (uint32_t)((int)i - 2 * cx(d)) * ny + (uint32_t)((int)j - 2 * cy(d));
     for (uint32_t c = 0; c < a.nc; ++c)
       for (int k = 0; k < 9; ++k)
-        if (neem_unknown(d, k)) P(a.g.f, c * 9 + k, a.ps)[n] = P(a.g.f, c * 9 + k, a.ps)[n2];
+        if (neem_unknown(d, k))
+          P(a.g.f, c * 9 + k, a.ps)[n] = 2.0f * P(a.g.f, c * 9 + k, a.ps)[n2] - P(a.g.f, c * 9 + k, a.ps)[n3];
+  } else if (a.kind == LBM_BC_OPEN_NEUMANN || a.kind == LBM_BC_OPEN_CONVECTIVE1) {
+    // OpenNeumann: copy the unknowns from x - c_d; OpenConvective1order:
+    // unknowns := (f + l f(x - c_d)) / (1 + l), l = |u(x - c_d)|; then
+    // OpenBoundary.density / vel at x (:274-298)
+    const int d = a.rule.direction;
+    const uint32_t n2 = (uint32_t)((int)i - cx(d)) * ny + (uint32_t)((int)j - cy(d));
+    if (a.kind == LBM_BC_OPEN_NEUMANN) {
+      for (uint32_t c = 0; c < a.nc; ++c)
+        for (int k = 0; k < 9; ++k)
+          if (neem_unknown(d, k)) P(a.g.f, c * 9 + k, a.ps)[n] = P(a.g.f, c * 9 + k, a.ps)[n2];
+    } else {
+      const float ux2 = a.g.vel[n2], uy2 = a.g.vel[ps + n2];
+      const float lam = sqrtf(ux2 * ux2 + uy2 * uy2);
+      for (uint32_t c = 0; c < a.nc; ++c)
+        for (int k = 0; k < 9; ++k)
+          if (neem_unknown(d, k)) {
+            float* fp = P(a.g.f, c * 9 + k, a.ps) + n;
+            *fp = (*fp + lam * P(a.g.f, c * 9 + k, a.ps)[n2]) / (1.0f + lam);
+          }
+    }
     float tot = 0.0f;
     for (uint32_t c = 0; c < a.nc; ++c) {
       float r = 0.0f;
@@ -669,7 +692,7 @@ extern "C" int lbm_g_collide(const lbm_lattice* lat, const lbm_gfields* g, const
 extern "C" int lbm_g_boundary(const lbm_lattice* lat, const lbm_gfields* g, int32_t kind, const lbm_neem_rule* rule,
                               const int32_t* group, int32_t count, void* stream) {
   GRANULAR_PROLOGUE
-  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_VELOCITY_EQ) return fail("bad boundary kind %d", kind);
+  if (kind < LBM_BC_PERIODIC || kind > LBM_BC_OPEN_EXTRAPOLATION) return fail("bad boundary kind %d", kind);
   if ((kind == LBM_BC_INSIDE || kind == LBM_BC_BOUNCEBACK) && !g->neighbor)
     return fail("boundary kind %d needs the neighbour table", kind);
   if (kind >= LBM_BC_VELOCITY_NEEM) {

@@ -139,17 +139,19 @@ def check_granular_bounds(kind, nodes, NX, NY, direction=None, table=None, name=
             m, k = np.argwhere(bad)[0]
             raise H.LBMError(f"{name}: neighbour table entry {tuple(nb[m, k])} of node {tuple(nodes[m])}, "
                              f"population {k}, is neither -1 nor a node of the lattice")
-    elif kind in (H.BC_VELOCITY_NEEM, H.BC_PRESSURE_NEEM, H.BC_OPEN_NEUMANN, H.BC_PRESSURE_ABC, H.BC_VELOCITY_EQ):
+    elif kind in (H.BC_VELOCITY_NEEM, H.BC_PRESSURE_NEEM, H.BC_OPEN_NEUMANN, H.BC_PRESSURE_ABC, H.BC_VELOCITY_EQ,
+                  H.BC_OPEN_CONVECTIVE1, H.BC_OPEN_EXTRAPOLATION):
         if direction not in (1, 2, 3, 4):
             raise H.LBMError(f"{name}: direction {direction} (1..4)")
         if kind == H.BC_VELOCITY_EQ:     # reads its own node only
             return
-        I2, J2 = I - _C[direction, 0], J - _C[direction, 1]
-        out = (I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)
-        if out.any():
-            m = int(np.argmax(out))
-            raise H.LBMError(f"{name}: node {tuple(nodes[m])} reads its interior neighbour "
-                             f"{(int(I2[m]), int(J2[m]))} outside the lattice (direction {direction})")
+        for s in ((1, 2) if kind == H.BC_OPEN_EXTRAPOLATION else (1,)):   # x - c_d (and x - 2 c_d)
+            I2, J2 = I - s * _C[direction, 0], J - s * _C[direction, 1]
+            out = (I2 < 0) | (I2 >= NX) | (J2 < 0) | (J2 >= NY)
+            if out.any():
+                m = int(np.argmax(out))
+                raise H.LBMError(f"{name}: node {tuple(nodes[m])} reads its interior neighbour "
+                                 f"{(int(I2[m]), int(J2[m]))} outside the lattice (direction {direction})")
 
 
 class BoundaryCondition:
@@ -365,17 +367,56 @@ class FluidBoundary(BoundaryCondition):
     kind = "fluid"
 
 
-def _out_of_scope(name, where):
+class OpenConvective1order(PlaneBoundary):
+    """BoundaryCondition.py:337-359: convective outlet -- the unknown
+    populations relax towards x - c_d's, f := (f + l f(x - c_d)) / (1 + l)
+    with l = |u(x - c_d)| of the step's velocity pass, then density and
+    velocity at x (fused and granular)."""
+
+    kind = "open_convective1"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = H.RULE_OPEN_CONVECTIVE1, int(self.direction)
+        return r
+
+    def fused_spec(self, name=""):
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), name=name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_OPEN_CONVECTIVE1, self.rule())
+
+
+class OpenExtrapolation(PlaneBoundary):
+    """BoundaryCondition.py:394-413: the unknown populations extrapolated
+    linearly from x - c_d and x - 2 c_d (density and velocity left as they
+    are, as in the reference).  Granular only: the fused step's node rules
+    read one node inward, this one reads two."""
+
+    kind = "open_extrapolation"
+
+    def rule(self):
+        r = H.NeemRule()
+        r.kind, r.direction = 0, int(self.direction)
+        return r
+
+    def fused_spec(self, name=""):
+        return BCSpec(self.kind, self.group.nodes, int(self.direction), name=name)
+
+    def apply(self, lb_field):
+        self._granular(lb_field, H.BC_OPEN_EXTRAPOLATION, self.rule())
+
+
+def _out_of_scope(name, where, why):
     def init(self, *a, **k):
-        raise NotImplementedError(
-            f"{name} ({where}) is not part of the MI355X hot path; supported boundaries: PeriodicAllBoundary, "
-            "InsideBoundary, BounceBackWall, VelocityNEEM, PressureNEEM, VelocityBB, OpenNeumann, FluidBoundary")
+        raise NotImplementedError(f"{name} ({where}) is not built: {why}")
     return type(name, (BoundaryCondition,), {"__init__": init})
 
 
-OpenConvective1order = _out_of_scope("OpenConvective1order", "BoundaryCondition.py:337-359")
-OpenConvective2order = _out_of_scope("OpenConvective2order", "BoundaryCondition.py:362-390")
-OpenExtrapolation = _out_of_scope("OpenExtrapolation", "BoundaryCondition.py:394-415")
+OpenConvective2order = _out_of_scope(
+    "OpenConvective2order", "BoundaryCondition.py:362-390",
+    "it needs ShanChenForceC1's psi and rewrites the interior neighbour's force and velocity, and its three "
+    "passes over the group run as parallel loop iterations in the reference (a race)")
 
 
 class BoundaryEngine:

@@ -51,6 +51,7 @@ class BCSpec:
     """Host description of one boundary-list entry."""
     kind: str                 # fluid | periodic | inside | bounceback | velocity_neem | pressure_neem
                               # | velocity_bb | open_neumann | pressure_abc | velocity_eq
+                              # | open_convective1 | open_extrapolation (granular only)
     nodes: np.ndarray         # (m, 2) int (i, j)
     direction: int = 0
     ux: float = 0.0
@@ -109,7 +110,8 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     neem_nodes: List[Tuple[np.ndarray, np.ndarray]] = []   # (nodes, interior neighbours or None)
     rule_kinds = {"velocity_neem": H.NEEM_VELOCITY, "pressure_neem": H.NEEM_PRESSURE,
                   "velocity_bb": H.RULE_VELOCITY_BB, "open_neumann": H.RULE_OPEN_NEUMANN,
-                  "pressure_abc": H.RULE_PRESSURE_ABC, "velocity_eq": H.RULE_VELOCITY_EQ}
+                  "pressure_abc": H.RULE_PRESSURE_ABC, "velocity_eq": H.RULE_VELOCITY_EQ,
+                  "open_convective1": H.RULE_OPEN_CONVECTIVE1}
     nb_ok = (neighbor.reshape(N, 9) if neighbor.ndim == 3 else neighbor.reshape(N, 9, 2)[:, :, 0] != -1)
 
     for bc in bcs:
@@ -199,7 +201,7 @@ def compile_codes(NX: int, NY: int, active_nodes: np.ndarray, bcs: Sequence[BCSp
     for e, (flat, nbf) in enumerate(neem_nodes):
         kind = neem_rules[e][0]
         code[flat] |= np.uint16((e + 1) << H.CODE_NEEM_SHIFT)
-        if kind in (H.NEEM_VELOCITY, H.NEEM_PRESSURE, H.RULE_PRESSURE_ABC):
+        if kind in (H.NEEM_VELOCITY, H.NEEM_PRESSURE, H.RULE_PRESSURE_ABC, H.RULE_OPEN_CONVECTIVE1):
             code[nbf] |= np.uint16(H.CODE_KEEP_MACRO)        # neighbour's rho/u of the previous step
         if kind in (H.NEEM_PRESSURE, H.RULE_VELOCITY_BB, H.RULE_PRESSURE_ABC):
             code[flat] |= np.uint16(H.CODE_KEEP_MACRO)       # own rho (ABC: u) of the previous step

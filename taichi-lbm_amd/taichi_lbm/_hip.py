@@ -19,6 +19,8 @@ NEEM_VELOCITY, NEEM_PRESSURE, RULE_VELOCITY_BB, RULE_OPEN_NEUMANN, RULE_PRESSURE
 BC_PERIODIC, BC_INSIDE, BC_BOUNCEBACK, BC_VELOCITY_NEEM, BC_PRESSURE_NEEM, BC_VELOCITY_BB, BC_OPEN_NEUMANN = \
     1, 2, 3, 4, 5, 6, 7
 BC_PRESSURE_ABC, BC_VELOCITY_EQ = 8, 9
+BC_OPEN_CONVECTIVE1, BC_OPEN_EXTRAPOLATION = 10, 11
+RULE_OPEN_CONVECTIVE1 = 7
 
 CODE_BOUNCE_MASK = 0x00FF
 CODE_ACTIVE = 0x0100
@@ -31,7 +33,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 
 class NeemRule(C.Structure):

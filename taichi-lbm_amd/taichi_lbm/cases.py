@@ -490,12 +490,14 @@ def cavity(N=256, power_n=1.0, Re=10.0, u_lid=5e-3, collision="mrt", magic=0.25,
 
 
 def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt", tau_bulk=None,
-                           name="cylinder_ref"):
+                           name="cylinder_ref", outlet="neumann"):
     """The boundary list of cases/Circular cylinder.py:72-133 with bounds-checked
     predicates: cylinder at (3/8 NX - 1/2, NY/2 - 1/2), radius NY/10 - 1/2;
     periodic streaming; bounce-back on the first/last columns and around the
     cylinder; VelocityBB inlet (d=3) on column 0; OpenNeumann outlet (d=1) on
-    the last column.  (The reference's wall predicate reads the mask out of
+    the last column (outlet="convective1" | "extrapolation": the
+    OpenConvective1order / OpenExtrapolation outlet of :337-413 there
+    instead).  (The reference's wall predicate reads the mask out of
     range at the top/bottom rows; here those reads count as fluid, i.e. the
     channel is periodic in y as the streaming is.)"""
     from .BoundaryCondition import OpenNeumann, VelocityBB
@@ -519,7 +521,10 @@ def cylinder_reference_bcs(NX=600, NY=160, tau=0.62, u_in=0.0173, collision="mrt
     inlet = VelocityBB(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == 0)), (u_in, 0.0), 3)
     inlet.precompute(clf)
     bce.add_boundary_condition("inlet", inlet)
-    outlet = OpenNeumann(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1)), 1)
+    from .BoundaryCondition import OpenConvective1order, OpenExtrapolation
+    out_cls = {"neumann": OpenNeumann, "convective1": OpenConvective1order,
+               "extrapolation": OpenExtrapolation}[outlet]
+    outlet = out_cls(BoundarySpec(lambda I, J: (m[I, J] == 1) & (I == NX - 1)), 1)
     outlet.precompute(clf)
     bce.add_boundary_condition("outlet", outlet)
     lbf.neighbor_classify()

@@ -509,6 +509,50 @@ def test_open_neumann_copies_unknowns_and_recomputes_moments():
     assert np.allclose(lat.vel[4], m / lat.rho[4, :, 0][:, None])
 
 
+def test_open_convective1_relaxes_unknowns_towards_the_neighbour():
+    # OpenConvective1order (BoundaryCondition.py:343-359): with l = |u(x - c_d)|
+    # the unknowns become (f + l f_nb) / (1 + l): l = 0 keeps them; u_nb =
+    # (0.3, 0.4) gives l = 0.5, i.e. (2 f + f_nb) / 3; only the unknowns move
+    # and density / velocity at x follow from the new populations
+    for d, col, nb in ((1, 4, 3), (3, 0, 1)):
+        lat, grp = _lat_all_fluid(5, 4)
+        rng = np.random.default_rng(5 + d)
+        lat.f[...] = 0.1 + 0.01 * rng.standard_normal(lat.f.shape)
+        line = np.array([[col, j] for j in range(4)])
+        before = lat.f.copy()
+        lat.vel[nb] = 0.0
+        O.BC("open_convective1", line, d).apply(lat)
+        assert np.array_equal(lat.f, before)
+        lat.f[...] = before
+        lat.vel[nb] = (0.3, 0.4)
+        O.BC("open_convective1", line, d).apply(lat)
+        unk = O.UNKNOWN[d]
+        known = [k for k in range(9) if k not in unk]
+        assert np.allclose(lat.f[col, :, 0, unk], (2 * before[col, :, 0, unk] + before[nb, :, 0, unk]) / 3, atol=1e-15)
+        assert np.array_equal(lat.f[col, :, 0, known], before[col, :, 0, known])
+        assert np.allclose(lat.rho[col, :, 0], lat.f[col, :, 0, :].sum(-1))
+        m = lat.f[col, :, 0, :] @ O.C.astype(f64)
+        assert np.allclose(lat.vel[col], m / lat.rho[col, :, 0][:, None])
+
+
+def test_open_extrapolation_continues_linear_profiles():
+    # OpenExtrapolation (BoundaryCondition.py:399-413): 2 f(x - c_d) - f(x - 2 c_d)
+    # continues a population profile that is linear along c_d exactly; the
+    # known populations and density / velocity are left as they were
+    lat, grp = _lat_all_fluid(6, 5)
+    i = np.arange(6)[:, None, None]
+    alpha = np.linspace(0.1, 0.3, 9)[None, None, :]
+    beta = np.linspace(-0.02, 0.015, 9)[None, None, :]
+    lat.f[:, :, 0, :] = alpha + beta * i + np.zeros((6, 5, 9))
+    lat.f[5] = 7.0                                           # outlet column garbage
+    lat.rho[5], lat.vel[5] = 3.0, 4.0
+    O.BC("open_extrapolation", np.array([[5, j] for j in range(5)]), 1).apply(lat)
+    unk = O.UNKNOWN[1]
+    assert np.allclose(lat.f[5, :, 0, unk], (alpha + 5 * beta)[0, 0, unk][:, None], atol=1e-15)
+    assert np.all(lat.f[5, :, 0, [k for k in range(9) if k not in unk]] == 7.0)
+    assert np.all(lat.rho[5] == 3.0) and np.all(lat.vel[5] == 4.0)
+
+
 # ---------------------------------------------------------------------------
 # Two-component Shan-Chen force, pressure observables, the nc = 2 force split
 # and barycentric velocity, the table stream (InsideBoundary)

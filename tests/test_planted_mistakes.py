@@ -171,8 +171,38 @@ def _inside_wraps_periodically(self, lat):
         self.kind = keep
 
 
+def _convective_with_own_velocity(self, lat):
+    # l = |u(x)| of the outlet node in place of its interior neighbour's
+    if self.kind != "open_convective1":
+        return _orig_bc_apply(self, lat)
+    I, J = O.group_ij(self.group)
+    d = self.direction
+    keep = lat.vel[I - O.C[d, 0], J - O.C[d, 1]].copy()
+    lat.vel[I - O.C[d, 0], J - O.C[d, 1]] = lat.vel[I, J]
+    try:
+        _orig_bc_apply(self, lat)
+    finally:
+        lat.vel[I - O.C[d, 0], J - O.C[d, 1]] = keep
+
+
+def _extrapolation_from_own_node(self, lat):
+    # 2 f(x - c_d) - f(x) in place of 2 f(x - c_d) - f(x - 2 c_d)
+    if self.kind != "open_extrapolation":
+        return _orig_bc_apply(self, lat)
+    I, J = O.group_ij(self.group)
+    d = self.direction
+    I2, J2 = I - O.C[d, 0], J - O.C[d, 1]
+    for c in range(lat.nc):
+        for k in O.UNKNOWN[d]:
+            lat.f[I, J, c, k] = 2 * lat.f[I2, J2, c, k] - lat.f[I, J, c, k]
+
+
 # name -> (object, attribute, planted value, pins that must fail)
 MUTATIONS_SC = {
+    "convective_outlet_own_velocity": (O.BC, "apply", _convective_with_own_velocity,
+                                       [kat.test_open_convective1_relaxes_unknowns_towards_the_neighbour]),
+    "extrapolation_from_own_node": (O.BC, "apply", _extrapolation_from_own_node,
+                                    [kat.test_open_extrapolation_continues_linear_profiles]),
     "pressure2_cross_term_unscaled": (O, "pressure", _pressure_cross_term_unscaled, [kat.test_pressure2_uniform_states]),
     "pressure1_half_dropped": (O, "pressure", _pressure1_without_half,
                                [kat.test_pressure1_equal_at_reference_coexistence_densities]),
