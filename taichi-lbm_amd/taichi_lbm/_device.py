@@ -32,8 +32,16 @@ def stream() -> int:
 
 
 def plane_stride(n: int) -> int:
-    """Planes start on 256-byte boundaries."""
-    return (n + 63) // 64 * 64
+    """Elements from one plane of a field to the next: planes start on
+    256-byte boundaries (64 elements), an odd number of 256-byte blocks
+    apart.  At a power-of-two stride (the 8192^2 lattice: 2^26 elements) the
+    nine population planes of a node fall on the same memory channel and
+    cache sets; one block more made the two-step MRT step 4 % faster
+    (0.593 against 0.619 ms/step, means of two runs each)."""
+    ps = (n + 63) // 64 * 64
+    if (ps // 64) % 2 == 0:
+        ps += 64
+    return ps
 
 
 class _Scalar:
