@@ -44,7 +44,7 @@ def build_case(N, collision, seed):
     return case
 
 
-def time_gpu(case, steps, warmup, storage="f32"):
+def time_gpu(case, steps, warmup, storage="f32", rounding="nearest"):
     import torch
     from taichi_lbm import cases
     from taichi_lbm.stepper import Stepper
@@ -54,7 +54,7 @@ def time_gpu(case, steps, warmup, storage="f32"):
     # steps are ordinary steps of the run; the fields do not depend on the
     # width), so the timed kernel is the tuned one whatever --warmup is
     st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=True, tile_cols="auto",
-                 storage=storage)
+                 storage=storage, rounding=rounding)
     st.run(Stepper.tune_steps())
     st.run(warmup)
     torch.cuda.synchronize()
@@ -402,6 +402,17 @@ def serving_leg(n_req, rate, seed, max_steps, tol, tick, storages=("f32", "f16",
         for s in storages:   # two steps per launch, as the server runs them
             us, _, _ = solve_alone(specs[q], short_steps, 0.0, tick, s, blocked=True)
             err_short[s].append(float(np.abs(us - u64s).max()))
+    # the same converged requests with round-to-nearest 16-bit stores (the
+    # server rounds stochastically)
+    err_near = {s: [] for s in storages if s != "f32"}
+    for q in sorted(conv)[:n_fp64]:
+        nst = results["f32"][q][1]
+        u64, _, _ = solve_alone(specs[q], nst, 0.0, tick, "f64")
+        for s in err_near:
+            un, _, _ = solve_alone(specs[q], nst, 0.0, tick, s, blocked=True, rounding="nearest")
+            err_near[s].append(float(np.abs(un - u64).max()))
+    out["rounding_16bit"] = "stochastic"
+    out["max_abs_u_err_vs_fp64_converged_nearest"] = {s: max(v) if v else None for s, v in err_near.items()}
     out["max_abs_u_err_vs_fp64"] = {s: max(v) if v else None for s, v in err.items()}
     out["max_abs_u_err_vs_fp64_converged"] = {s: max(v) if v else None for s, v in err_conv.items()}
     out["precision_requests"] = per_request
@@ -764,14 +775,19 @@ def main():
         # the same MRT lattice with shifted 16-bit populations (FP32 arithmetic):
         # 36 B of populations + 8 B of power-law state per fluid node-step
         lbf = case["lbf"]
-        for storage in ("f16", "bf16"):
+        for storage, rounding in (("f16", "nearest"), ("bf16", "nearest"), ("f16", "stochastic"),
+                                  ("bf16", "stochastic")):
             lbf._st = None
             lbf._finisher = None
             lbf._pop_owner = None
             case["coll"]._state = None
             torch.cuda.empty_cache()
-            ms_s, red_s = time_gpu(case, args.steps, args.warmup, storage=storage)
-            log(f"MRT {storage}: {ms_s:.4f} ms/step")
+            ms_s, red_s = time_gpu(case, args.steps, args.warmup, storage=storage, rounding=rounding)
+            log(f"MRT {storage} ({rounding} rounding): {ms_s:.4f} ms/step")
+            if rounding == "stochastic":
+                extra[f"{storage}_stochastic_ms_per_step"] = round(ms_s, 5)
+                extra[f"{storage}_stochastic_nonfinite"] = red_s["nonfinite"]
+                continue
             extra[f"{storage}_ms_per_step"] = round(ms_s, 5)
             extra[f"{storage}_tile_cols"] = red_s["tile_cols"]
             extra[f"{storage}_mlups"] = round(n_fluid / (ms_s * 1e-3) / 1e6, 1)

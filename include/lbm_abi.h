@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 16
+#define LBM_ABI_VERSION 17
 
 /* status codes */
 #define LBM_OK 0
@@ -49,6 +49,13 @@ enum {
   LBM_STORE_F16_SHIFT = 1, /* f - w_k, IEEE half; arithmetic in fp32 */
   LBM_STORE_BF16_SHIFT = 2, /* f - w_k, bfloat16; arithmetic in fp32 */
   LBM_STORE_F64 = 3         /* f itself, fp64; arithmetic in fp64 (high-precision path) */
+};
+
+/* rounding of a step's populations to FP16 / BF16 storage (lbm_lattice.rounding) */
+enum {
+  LBM_ROUND_NEAREST = 0,    /* round to nearest even */
+  LBM_ROUND_STOCHASTIC = 1  /* gfx950 stochastic conversions: the expected stored value is the
+                               FP32 one, so steady flows do not accumulate a bias */
 };
 
 /* Shan-Chen variants (ExternalForce.py:55-145) */
@@ -152,6 +159,13 @@ typedef struct lbm_lattice {
   float huang_k1[LBM_MAX_NC];  /* k1                                               */
   float huang_k2[LBM_MAX_NC];  /* k2 = epsilon / (-8) - k1 (:462)                  */
   float huang_g[LBM_MAX_NC];   /* G of the component with itself (:463)            */
+  uint32_t round_step;         /* steps the lattice has taken since its initial state *
+                                * (mod 2^32): with FP16 / BF16 storage the stores of *
+                                * step t are rounded stochastically, seeded by t and *
+                                * the node, so a lattice rounds the same in split     *
+                                * runs and in a batch; no counterpart in the          *
+                                * reference (its fields are FP32)                     */
+  int32_t rounding;            /* LBM_ROUND_*: FP16 / BF16 stores (ignored otherwise) */
   int32_t pad1;
 } lbm_lattice;
 
@@ -263,6 +277,11 @@ int lbm_batch_update(lbm_batch* b, int32_t index, const lbm_lattice* lat, const 
  * this lattice's populations are post-stream f read in place (-1: never);
  * vel_old: 2 planes for lbm_batch_residual (may be NULL).                  */
 int lbm_batch_set(lbm_batch* b, int32_t index, int32_t active, int32_t start_step, float* vel_old);
+/* The step count of lattice `index` (lbm_lattice.round_step of a batch: kept
+ * in device memory, advanced by every run and graph replay of the batch, 0
+ * at lbm_batch_create); set it when a slot takes a new lattice.  Ordered on
+ * `stream`.  No counterpart in the reference (16-bit storage).             */
+int lbm_batch_set_round_step(lbm_batch* b, int32_t index, uint32_t step, void* stream);
 /* Rebuild the tile list and upload descriptors (synchronous on `stream`).  A
  * graph captured earlier stays valid while the tile count fits the table
  * capacity of its capture (lbm_batch_capacity).                           */

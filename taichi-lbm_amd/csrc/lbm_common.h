@@ -97,10 +97,58 @@ __device__ __forceinline__ Rsrc make_rsrc(const void* base, uint32_t bytes) {
 }
 constexpr int kNonTemporal = 2;   // cache-policy bits of a streaming store
 
+// ---- stochastic rounding of 16-bit stores ----------------------------------
+// A step's post-collision populations are rounded to FP16 / BF16 with
+// gfx950's stochastic conversions (v_cvt_sr_f16_f32 adds seed >> 19 to the
+// FP32 bits before truncating to FP16, v_cvt_sr_bf16_f32 adds seed >> 16
+// before truncating to BF16): the expected stored value is the FP32 value, so
+// a steady flow's rounding does not push the same way every step.  Round to
+// nearest even at every store left errors of 2e-5 (FP16) and 4e-4 (BF16) in
+// the steady velocity of a force-driven channel, against 2e-6 and 2e-5 with
+// these seeds.  The seed is a hash of the node's row and column and of the
+// lattice's step count, so a lattice rounds the same alone, in a batch and
+// across split runs.
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+// The step key of the populations a launch stores: `step` counts the steps
+// of the lattice from its initial state up to and including the stored one.
+__host__ __device__ __forceinline__ uint32_t round_key(uint32_t step) {
+  return mix32(step * 0x9e3779b9u + 0x632be5abu);
+}
+// 64 random bits per node and step; direction k takes the 32-bit window at
+// bit 4k (the conversions read a window's top bits).
+struct SrSeed {
+  uint32_t lo, hi;
+  __device__ __forceinline__ SrSeed(uint32_t i, uint32_t j, uint32_t key) {
+    lo = mix32(((i << 16) + j) ^ key);
+    hi = mix32(lo ^ 0x85ebca6bu);
+  }
+  __device__ __forceinline__ uint32_t bits(int k) const {
+    return k < 8 ? __builtin_amdgcn_alignbit(hi, lo, 4 * k) : hi;
+  }
+};
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf162_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint16_t f16_sr(float x, uint32_t seed) {
+  return __builtin_bit_cast(uint16_t, __builtin_amdgcn_cvt_sr_f16_f32((half2_t){0, 0}, x, seed, false)[0]);
+}
+__device__ __forceinline__ uint16_t bf16_sr(float x, uint32_t seed) {
+  return __builtin_bit_cast(uint16_t, __builtin_amdgcn_cvt_sr_bf16_f32((bf162_t){0, 0}, x, seed, false)[0]);
+}
+
 // ---- population storage ---------------------------------------------------
 // Pop<S>::T is the stored type, Pop<S>::R the arithmetic type.  Shifted
 // formats store g = f - w_k so that the stored magnitude is the deviation
-// from rest; their arithmetic is fp32.
+// from rest; their arithmetic is fp32.  st / stb round to nearest even (a
+// re-store of loaded values: exact); st_sr / stb_sr store a step's new
+// populations with stochastic rounding (seed from SrSeed::bits; FP32 and
+// FP64 ignore it).
 template <int S> struct Pop;
 template <> struct Pop<LBM_STORE_F32> {
   using T = float;
@@ -115,6 +163,10 @@ template <> struct Pop<LBM_STORE_F32> {
   static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int, R v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, (int)soff, kNonTemporal);
   }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, int k, R v, uint32_t) { st(p, i, k, v); }
+  static __device__ __forceinline__ void stb_sr(Rsrc r, uint32_t voff, uint32_t soff, int k, R v, uint32_t) {
+    stb(r, voff, soff, k, v);
+  }
 };
 template <> struct Pop<LBM_STORE_F64> {
   using T = double;
@@ -122,6 +174,7 @@ template <> struct Pop<LBM_STORE_F64> {
   static __device__ __forceinline__ R ld(const T* p, uint32_t i, int) { return p[i]; }
   static __device__ __forceinline__ void st(T* p, uint32_t i, int, R v) { __builtin_nontemporal_store(v, p + i); }
   static __device__ __forceinline__ void st_cached(T* p, uint32_t i, int, R v) { p[i] = v; }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, int k, R v, uint32_t) { st(p, i, k, v); }
 };
 template <> struct Pop<LBM_STORE_F16_SHIFT> {
   using T = _Float16;
@@ -138,6 +191,12 @@ template <> struct Pop<LBM_STORE_F16_SHIFT> {
   static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int k, R v) {
     const uint16_t b = __builtin_bit_cast(uint16_t, (T)(v - wgt<float>(k)));
     __builtin_amdgcn_raw_buffer_store_b16(b, r, (int)voff, (int)soff, kNonTemporal);
+  }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, int k, R v, uint32_t seed) {
+    __builtin_nontemporal_store(__builtin_bit_cast(T, f16_sr(v - wgt<float>(k), seed)), p + i);
+  }
+  static __device__ __forceinline__ void stb_sr(Rsrc r, uint32_t voff, uint32_t soff, int k, R v, uint32_t seed) {
+    __builtin_amdgcn_raw_buffer_store_b16(f16_sr(v - wgt<float>(k), seed), r, (int)voff, (int)soff, kNonTemporal);
   }
 };
 template <> struct Pop<LBM_STORE_BF16_SHIFT> {
@@ -163,6 +222,12 @@ template <> struct Pop<LBM_STORE_BF16_SHIFT> {
   }
   static __device__ __forceinline__ void stb(Rsrc r, uint32_t voff, uint32_t soff, int k, R v) {
     __builtin_amdgcn_raw_buffer_store_b16(enc(v - wgt<float>(k)), r, (int)voff, (int)soff, kNonTemporal);
+  }
+  static __device__ __forceinline__ void st_sr(T* p, uint32_t i, int k, R v, uint32_t seed) {
+    __builtin_nontemporal_store(bf16_sr(v - wgt<float>(k), seed), p + i);
+  }
+  static __device__ __forceinline__ void stb_sr(Rsrc r, uint32_t voff, uint32_t soff, int k, R v, uint32_t seed) {
+    __builtin_amdgcn_raw_buffer_store_b16(bf16_sr(v - wgt<float>(k), seed), r, (int)voff, (int)soff, kNonTemporal);
   }
 };
 

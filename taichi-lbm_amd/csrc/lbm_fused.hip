@@ -75,6 +75,9 @@ struct Lat {
   int32_t huang;               // HuangMRTCollision: MRT with the source S Q (huang_q)
   float huang_k1[LBM_MAX_NC], huang_k2[LBM_MAX_NC], huang_g[LBM_MAX_NC];
   uint32_t parity_xor;         // batched Shan-Chen runs: this lattice's ping-pong phase against the launch's
+  uint32_t stochastic;         // 16-bit storage: stochastic rounding of the steps' stores
+  uint32_t round_base;         // steps the lattice has taken before this call (16-bit stochastic rounding)
+  const uint32_t* round_ctr;   // batches: the same count in device memory (advanced by k_round_tick)
   lbm_neem_rule neem[LBM_MAX_NEEM];
 };
 
@@ -94,6 +97,7 @@ struct Launch {
   int32_t write_macro;   // write every node's macro fields (last step of a run)
   int32_t xcd_remap;
   uint32_t nblocks;
+  uint32_t round_step;   // steps of the call up to and including the one this launch stores
 };
 
 enum { MODE_STEP = 0, MODE_FINISH = 1, MODE_PSI = 2 };
@@ -344,6 +348,45 @@ struct StepOut {
   R gam[NC];                  // squared shear rate (power law)
   R state[NC];                // new relaxation state (power law)
 };
+
+// A node's new populations to pop planes (BUF: buffer stores).  With 16-bit
+// storage and a.stochastic the step's stores round stochastically, seeded by
+// the node's row and column and the lattice's step count at the stored step
+// (round_step: steps of this call up to it); else to nearest even.
+template <int NC, int S, bool BUF>
+__device__ __forceinline__ void store_pops(const Lat& a, uint32_t round_step, typename Pop<S>::T* __restrict__ dst,
+                                           Rsrc rdst, uint32_t ps, uint32_t n, uint32_t gi, uint32_t gj,
+                                           const typename Pop<S>::R (&f)[NC][9]) {
+  using T = typename Pop<S>::T;
+  constexpr bool sr = S == LBM_STORE_F16_SHIFT || S == LBM_STORE_BF16_SHIFT;
+  if (sr && a.stochastic) {
+    const uint32_t base = a.round_ctr ? *a.round_ctr : a.round_base;
+    const SrSeed sd(gi, gj, round_key(base + round_step));
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const uint32_t seed = sd.bits(k) ^ (0x9e3779b9u * (uint32_t)c);
+        if constexpr (BUF)
+          Pop<S>::stb_sr(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, f[c][k],
+                         seed);
+        else
+          Pop<S>::st_sr(dst + (size_t)(c * 9 + k) * ps, n, k, f[c][k], seed);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      if constexpr (BUF)
+        Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, f[c][k]);
+      else
+        Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, f[c][k]);
+    }
+  }
+}
 
 // Where a step reads Shan-Chen psi of the step at x + c_k (k = 0: x itself):
 // psi.at(c, k) for component c.  PsiPlanes: the psi plane set `set` in HBM,
@@ -651,12 +694,10 @@ __device__ __forceinline__ void node_update(const Lat& a, const Launch& q, uint3
   StepOut<NC, R> o;
   step_math<COLL, NC, R>(a, PsiPlanes<R>{a, p ^ 1u, i, j}, n, i, j, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[p ^ 1u];
+  store_pops<NC, S, false>(a, q.round_step, dst, make_rsrc(dst, 0u), ps, n, i, j, o.f);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
+  for (int c = 0; c < NC; ++c)
     if (a.tau_mode == LBM_TAU_POWER_LAW) ((R*)a.tau)[(size_t)c * ps + n] = o.state[c];
-  }
   store_macro<NC, R>(a, (R*)a.mac[p ^ 1u], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
 }
 
@@ -874,6 +915,7 @@ struct Launch2 {
   void* tau_out;              // single lattice, power law: state after step n+1
   uint32_t tau_flip;          // batches, power law: read tau2, write tau (else the reverse)
   int32_t write_psi;          // Shan-Chen: keep this step's psi in HBM (the last two steps of a run)
+  uint32_t round_step;        // steps of the call up to and including the one this launch stores
 };
 
 // Phase A inputs of one ring node: gathered populations and relaxation state.
@@ -1065,17 +1107,10 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
       st_in[c] = pl ? L.st[c][x.q] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
     StepOut<NC, R> o;
     step_math<COLL, NC, R>(a, PsiNone<R>{}, n, gi, gj, f, st_in, o);
+    store_pops<NC, S, BUF>(a, q.round_step, dst, rdst, ps, n, gi, gj, o.f);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-#pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        if constexpr (BUF)
-          Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
-        else
-          Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
-      }
+    for (int c = 0; c < NC; ++c)
       if (pl) ((R*)tau_out)[(size_t)c * ps + n] = o.state[c];
-    }
     store_macro<NC, R>(a, mo, n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
   }
 }
@@ -1262,15 +1297,9 @@ __device__ __forceinline__ void sc_tile(const Lat& a, const Launch2& q, bool loc
   step_math<COLL, NC, R>(a, PsiTile<R, Shape>{psiL, solL, qs}, n, gi, gj, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[pset];
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+  store_pops<NC, S, BUF>(a, q.round_step, dst, rdst, ps, n, gi, gj, o.f);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if constexpr (BUF)
-        Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
-      else
-        Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
-    }
     if (a.tau_mode == LBM_TAU_POWER_LAW) ((R*)a.tau)[(size_t)c * ps + n] = o.state[c];
     if (q.write_psi) ((R*)a.psi[pset])[(size_t)c * ps + n] = psiL[c * Shape::PN + qs];
   }
@@ -1460,15 +1489,9 @@ __device__ __forceinline__ void sc2_tile(const Lat& a, const Launch2& q, bool lo
   step_math<COLL, NC, R>(a, PsiTile<R, Shape>{&L.psi[0][0], L.sol, qs}, n, gi, gj, f, st_in, o);
   T* __restrict__ dst = (T*)a.pop[pb];
   const Rsrc rdst = make_rsrc(dst, BUF ? (uint32_t)(NC * 9) * ps * (uint32_t)sizeof(T) : 0u);
+  store_pops<NC, S, BUF>(a, q.round_step, dst, rdst, ps, n, gi, gj, o.f);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      if constexpr (BUF)
-        Pop<S>::stb(rdst, n * (uint32_t)sizeof(T), (uint32_t)(c * 9 + k) * ps * (uint32_t)sizeof(T), k, o.f[c][k]);
-      else
-        Pop<S>::st(dst + (size_t)(c * 9 + k) * ps, n, k, o.f[c][k]);
-    }
     if (q.write_psi) ((R*)a.psi[pb])[(size_t)c * ps + n] = L.psi[c][qs];
   }
   store_macro<NC, R>(a, (R*)a.mac[pb], n, o, (code & LBM_CODE_KEEP_MACRO) != 0, q.write_macro != 0);
@@ -1615,6 +1638,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_tau_back(const Lat* __restrict
   }
 }
 
+// Step counts of a batch's lattices (the keys of their stochastic rounding):
+// the last launch of every batched run and captured graph adds the steps it
+// took to each active lattice's count.
+__global__ __launch_bounds__(kBlock) void k_round_tick(const Lat* __restrict__ items, uint32_t count,
+                                                       uint32_t steps, uint32_t* __restrict__ ctr) {
+  for (uint32_t l = threadIdx.x; l < count; l += kBlock)
+    if (items[l].active) ctr[l] += steps;
+}
+__global__ void k_round_set(uint32_t* __restrict__ ctr, uint32_t value) {
+  if (threadIdx.x == 0) *ctr = value;
+}
+
 // The same for one lattice (lbm_run_blocked): a.tau2 holds the state to copy.
 __global__ __launch_bounds__(kBlock) void k_tau_back(const Lat a, uint32_t nc) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
@@ -1740,6 +1775,8 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
     return fail("population buffer exceeds 2^32 elements");
   if (lat->collision < LBM_BGK || lat->collision > LBM_MRT) return fail("bad collision kind %d", lat->collision);
   if (lat->storage < LBM_STORE_F32 || lat->storage > LBM_STORE_F64) return fail("bad storage %d", lat->storage);
+  if (lat->rounding != LBM_ROUND_NEAREST && lat->rounding != LBM_ROUND_STOCHASTIC)
+    return fail("bad rounding %d", lat->rounding);
   if (lat->shan_chen < LBM_SC_NONE || lat->shan_chen > LBM_SC_C2) return fail("bad shan_chen %d", lat->shan_chen);
   if (lat->shan_chen == LBM_SC_C2 && lat->nc != 2) return fail("ShanChenForceC2 needs 2 components");
   if (lat->n_neem < 0 || lat->n_neem > LBM_MAX_NEEM) return fail("n_neem %d out of range", lat->n_neem);
@@ -1773,6 +1810,8 @@ int validate(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule*
 Lat make_lat(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem) {
   Lat a;
   memset(&a, 0, sizeof a);
+  a.round_base = lat->round_step;
+  a.stochastic = lat->rounding == LBM_ROUND_STOCHASTIC ? 1u : 0u;
   for (int q = 0; q < 2; ++q) {
     a.pop[q] = fld->f[q];
     a.mac[q] = fld->macro[q];
@@ -1917,6 +1956,7 @@ struct lbm_batch {
   int shift_cap = 0;
   std::vector<lbm::Tile2> host_t2, host_t2lay;   // tile list; its striped layout (uploaded)
   bool captured = false;             // a graph holds the table addresses: no reallocation
+  uint32_t* round_ctr = nullptr;     // device: each lattice's step count (k_round_tick)
 };
 
 namespace {
@@ -1956,7 +1996,7 @@ namespace lbm {
 // Shan-Chen runs through k_sc (defined below); false where that kernel is
 // not built for the lattice (then the two-pass path runs).
 bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
-            hipStream_t s, int* rc);
+            hipStream_t s, int* rc, uint32_t step0 = 0);
 }  // namespace lbm
 
 extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_rule* neem,
@@ -1978,6 +2018,7 @@ extern "C" int lbm_run(const lbm_lattice* lat, const lbm_fields* fld, const lbm_
     q.step = t;
     q.local_read = (t == 0 && first_local) ? 1 : 0;
     q.write_macro = (t == nsteps - 1) ? 1 : 0;
+    q.round_step = (uint32_t)t + 1u;
     if (lat->shan_chen && (rc = launch(psik, a, q, s))) return rc;
     if ((rc = launch(step, a, q, s))) return rc;
     p ^= 1u;
@@ -2108,6 +2149,7 @@ bool run_sc2(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t pari
       q.local_read = (t == 0 && first_local) ? 1 : 0;
       q.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
       q.write_psi = (t == pairs - 1) ? 1 : 0;
+      q.round_step = 2u * (uint32_t)t + 2u;
       hipLaunchKernelGGL(fn2, dim3((unsigned)nt), dim3(threads), 0, s, a, q);
       if ((*rc = hip_check(hipGetLastError(), "Shan-Chen two-step launch"))) return true;
       p ^= 1u;
@@ -2115,7 +2157,7 @@ bool run_sc2(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t pari
   }
   const int rest = nsteps - 2 * pairs;
   if (rest > 0) {
-    if (!run_sc(lat, a, rest, p, (pairs == 0 && first_local) ? 1 : 0, s, rc)) return false;
+    if (!run_sc(lat, a, rest, p, (pairs == 0 && first_local) ? 1 : 0, s, rc, 2u * (uint32_t)pairs)) return false;
     if (*rc) return true;
     p ^= (uint32_t)(rest & 1);
   }
@@ -2124,7 +2166,7 @@ bool run_sc2(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t pari
 }
 
 bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parity, int32_t first_local,
-            hipStream_t s, int* rc) {
+            hipStream_t s, int* rc, uint32_t step0) {
   ScFn fn = pick_sc(lat->nc, lat->storage, lat->collision, fits_buffer(lat));
   if (!fn) return false;
   using Shape = ScShape<kScThreads>;
@@ -2146,6 +2188,7 @@ bool run_sc(const lbm_lattice* lat, const Lat& a, int32_t nsteps, uint32_t parit
     q.local_read = (t == 0 && first_local) ? 1 : 0;
     q.write_macro = (t == nsteps - 1) ? 1 : 0;
     q.write_psi = (t >= nsteps - 2) ? 1 : 0;
+    q.round_step = step0 + (uint32_t)t + 1u;
     hipLaunchKernelGGL(fn, dim3((unsigned)nt), dim3(kScThreads), 0, s, a, q);
     if ((*rc = hip_check(hipGetLastError(), "Shan-Chen launch"))) return true;
     p ^= 1u;
@@ -2200,6 +2243,7 @@ int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_ru
     q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
     q2.tau_in = tcur;
     q2.tau_out = tnext;
+    q2.round_step = 2u * (uint32_t)t + 2u;
     hipLaunchKernelGGL(two, dim3((unsigned)nt), dim3(block2_threads(h, w)), 0, s, a, q2);
     if ((rc = hip_check(hipGetLastError(), "two-step launch"))) return rc;
     if (pl) {
@@ -2217,6 +2261,7 @@ int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_ru
     q.step = 0;
     q.local_read = (pairs == 0 && first_local) ? 1 : 0;
     q.write_macro = 1;
+    q.round_step = 2u * (uint32_t)pairs + 1u;
     if ((rc = launch(pick(lat->nc, lat->storage, lat->collision, MODE_STEP), a, q, s))) return rc;
     p ^= 1u;
   }
@@ -2401,13 +2446,17 @@ extern "C" int lbm_batch_create(const lbm_lattice* lats, const lbm_fields* flds,
   int rc = hip_check(hipMalloc((void**)&b->items, sizeof(Lat) * count), "hipMalloc(batch items)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->tile_count, sizeof(uint32_t)), "hipMalloc(tile count)");
   if (!rc) rc = hip_check(hipMalloc((void**)&b->acc, sizeof(double) * 2 * count), "hipMalloc(residuals)");
+  if (!rc) rc = hip_check(hipMalloc((void**)&b->round_ctr, sizeof(uint32_t) * count), "hipMalloc(step counts)");
+  if (!rc) rc = hip_check(hipMemset(b->round_ctr, 0, sizeof(uint32_t) * count), "hipMemset(step counts)");
   if (rc) {
     (void)hipFree(b->items);
     (void)hipFree(b->tile_count);
     (void)hipFree(b->acc);
+    (void)hipFree(b->round_ctr);
     delete b;
     return rc;
   }
+  for (int l = 0; l < count; ++l) b->host[l].round_ctr = b->round_ctr + l;
   *out = b;
   return LBM_OK;
 }
@@ -2431,6 +2480,7 @@ extern "C" int lbm_batch_update(lbm_batch* b, int32_t index, const lbm_lattice* 
     return fail("lattice does not match the batch (components, storage, no Shan-Chen)");
   const Lat keep = b->host[index];
   b->host[index] = make_lat(lat, fld, neem);
+  b->host[index].round_ctr = keep.round_ctr;
   b->host[index].active = keep.active;
   b->host[index].start_step = keep.start_step;
   b->host[index].vel_old = keep.vel_old;
@@ -2602,7 +2652,7 @@ extern "C" int lbm_batch_reserve2(lbm_batch* b, int32_t tiles) {
 
 namespace {
 int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t write_macro, int32_t grid,
-                 hipStream_t s) {
+                 hipStream_t s, uint32_t round_step = 0) {
   using namespace lbm;
   if (!b->tile_item) return fail("commit the batch before stepping it");
   if (grid <= 0) grid = 256 * 8;
@@ -2612,9 +2662,20 @@ int batch_launch(lbm_batch* b, int mode, uint32_t parity, int32_t step, int32_t 
   q.parity = parity;
   q.step = step;
   q.write_macro = write_macro;
+  q.round_step = round_step;
   hipLaunchKernelGGL(pick_batch(b->nc, b->storage, mode), dim3(grid), dim3(kBlock), 0, s, b->items, b->tile_item,
                      b->tile_count, q);
   return hip_check(hipGetLastError(), "batch launch");
+}
+
+// After a batched run (or at the end of a captured one): the lattices' step
+// counts advance by the run's steps.
+int round_tick(lbm_batch* b, int32_t nsteps, hipStream_t s) {
+  using namespace lbm;
+  if (nsteps <= 0) return LBM_OK;
+  hipLaunchKernelGGL(k_round_tick, dim3(1), dim3(kBlock), 0, s, b->items, (uint32_t)b->count, (uint32_t)nsteps,
+                     b->round_ctr);
+  return hip_check(hipGetLastError(), "step-count launch");
 }
 }  // namespace
 
@@ -2653,6 +2714,7 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
   if (!b->tile_item) return fail("commit the batch before stepping it");
   Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2, b->tile_cols) : nullptr;
   uint32_t p = (uint32_t)parity;
+  const int total = nsteps;
   int t0 = 0;
   if (two) {
     // one block per entry of the striped table (empty entries exit): a graph
@@ -2667,6 +2729,7 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
       q2.step = t;
       q2.write_macro = (t == pairs - 1 && (nsteps & 1) == 0) ? 1 : 0;
       q2.tau_flip = (uint32_t)(t & 1);
+      q2.round_step = 2u * (uint32_t)t + 2u;
       hipLaunchKernelGGL(two, dim3(g2), dim3(block2_threads(ring_of(b->mode2), b->tile_cols)), 0, s, b->items,
                          b->tile2, (uint32_t)b->tile2_stride, q2);
       int rc = hip_check(hipGetLastError(), "batched two-step launch");
@@ -2683,10 +2746,12 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
     nsteps -= 2 * pairs;
   }
   for (int t = 0; t < nsteps; ++t) {
-    int rc = batch_launch(b, MODE_STEP, p, t0 + t, t == nsteps - 1, grid, s);
+    int rc = batch_launch(b, MODE_STEP, p, t0 + t, t == nsteps - 1, grid, s, 2u * (uint32_t)t0 + (uint32_t)t + 1u);
     if (rc) return rc;
     p ^= 1u;
   }
+  int rc = round_tick(b, total, s);
+  if (rc) return rc;
   *out = (int32_t)p;
   return LBM_OK;
 }
@@ -2769,10 +2834,11 @@ extern "C" int lbm_batch_run(lbm_batch* b, int32_t nsteps, int32_t parity, int32
   if (!b || nsteps < 0 || (parity != 0 && parity != 1)) return fail("bad lbm_batch_run arguments");
   hipStream_t s = (hipStream_t)stream;
   for (int t = 0; t < nsteps; ++t) {
-    int rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s);
+    int rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s,
+                          (uint32_t)t + 1u);
     if (rc) return rc;
   }
-  return LBM_OK;
+  return round_tick(b, nsteps, s);
 }
 
 extern "C" int lbm_batch_finish(lbm_batch* b, int32_t parity, int32_t grid_blocks, void* stream) {
@@ -2791,7 +2857,9 @@ extern "C" int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int
   if (rc) return rc;
   b->captured = true;
   for (int t = 0; t < nsteps && !rc; ++t)
-    rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s);
+    rc = batch_launch(b, MODE_STEP, (uint32_t)((parity + t) & 1), t, t == nsteps - 1, grid_blocks, s,
+                      (uint32_t)t + 1u);
+  if (!rc) rc = round_tick(b, nsteps, s);
   hipGraph_t g = nullptr;
   const int rc2 = hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
   if (rc || rc2) {
@@ -2848,8 +2916,16 @@ extern "C" int lbm_batch_destroy(lbm_batch* b) {
   (void)hipFree(b->acc);
   (void)hipFree(b->tile2);
   (void)hipFree(b->shift_masks);
+  (void)hipFree(b->round_ctr);
   delete b;
   return LBM_OK;
+}
+
+extern "C" int lbm_batch_set_round_step(lbm_batch* b, int32_t index, uint32_t step, void* stream) {
+  using namespace lbm;
+  if (!b || index < 0 || index >= b->count) return fail("bad batch index %d", index);
+  hipLaunchKernelGGL(k_round_set, dim3(1), dim3(64), 0, (hipStream_t)stream, b->round_ctr + index, step);
+  return hip_check(hipGetLastError(), "step-count set");
 }
 
 // ---- batched Shan-Chen lattices ------------------------------------------------

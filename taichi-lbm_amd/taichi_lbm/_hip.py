@@ -12,6 +12,7 @@ from ._build import LIB_PATH, STAMP_PATH, source_hash
 
 LBM_BGK, LBM_TRT, LBM_MRT = 0, 1, 2
 STORE_F32, STORE_F16_SHIFT, STORE_BF16_SHIFT, STORE_F64 = 0, 1, 2, 3
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 SC_NONE, SC_C1, SC_C2 = 0, 1, 2
 TAU_CONST, TAU_FIELD, TAU_POWER_LAW = 0, 1, 2
 NEEM_VELOCITY, NEEM_PRESSURE, RULE_VELOCITY_BB, RULE_OPEN_NEUMANN, RULE_PRESSURE_ABC, RULE_VELOCITY_EQ = \
@@ -33,7 +34,7 @@ PSI_SC, PSI_VDW, PSI_RK, PSI_PR, PSI_CS = 0, 1, 2, 3, 4
 PSI_BC_NEUMANN, PSI_BC_CONVECTIVE1, PSI_BC_CONVECTIVE2, PSI_BC_EXTRAPOLATION, PSI_BC_FIXED = 1, 2, 3, 4, 5
 MAX_NEEM = 15
 MAX_NC = 2
-ABI_VERSION = 16
+ABI_VERSION = 17
 
 
 class NeemRule(C.Structure):
@@ -59,6 +60,8 @@ class Lattice(C.Structure):
         ("g_adh", C.c_float * MAX_NC),
         ("sc_adhesion", C.c_int32), ("huang", C.c_int32),
         ("huang_k1", C.c_float * MAX_NC), ("huang_k2", C.c_float * MAX_NC), ("huang_g", C.c_float * MAX_NC),
+        ("round_step", C.c_uint32),
+        ("rounding", C.c_int32),
         ("pad1", C.c_int32),
     ]
 
@@ -133,6 +136,7 @@ def lib():
         "lbm_batch_create": ([LP, FP, NP, I32, C.POINTER(C.c_void_p)], C.c_int),
         "lbm_batch_update": ([BP, I32, LP, FP, NP], C.c_int),
         "lbm_batch_set": ([BP, I32, I32, I32, P], C.c_int),
+        "lbm_batch_set_round_step": ([BP, I32, C.c_uint32, P], C.c_int),
         "lbm_batch_commit": ([BP, P], C.c_int),
         "lbm_batch_tiles": ([BP], C.c_int),
         "lbm_batch_capacity": ([BP], C.c_int),
@@ -189,7 +193,7 @@ def exported_symbols():
             "lbm_g_velocity", "lbm_g_shear_rate", "lbm_g_relaxation_time", "lbm_g_collide", "lbm_g_boundary",
             "lbm_g_sc_update_psi", "lbm_g_sc_periodic_psi", "lbm_g_sc_force", "lbm_g_sc_psi_boundary",
             "lbm_batch_create",
-            "lbm_batch_update", "lbm_batch_set", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
+            "lbm_batch_update", "lbm_batch_set", "lbm_batch_set_round_step", "lbm_batch_commit", "lbm_batch_tiles", "lbm_batch_capacity",
             "lbm_batch_reserve", "lbm_batch_reserve2", "lbm_batch_run", "lbm_batch_finish", "lbm_batch_graph",
             "lbm_batch_mode", "lbm_batch_tile_cols", "lbm_tile_rows", "lbm_batch_ring", "lbm_batch_tiles2", "lbm_batch_release",
             "lbm_batch_run_blocked", "lbm_batch_graph_blocked", "lbm_graph_launch",

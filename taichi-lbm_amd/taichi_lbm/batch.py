@@ -165,13 +165,21 @@ class BatchRunner:
 
     ``BatchRunner(specs)`` sizes the arena for the given lattices and adds them;
     ``BatchRunner([], capacity_nodes=..., max_slots=...)`` starts empty for
-    dynamic use (``add`` / ``remove``)."""
+    dynamic use (``add`` / ``remove``).  ``rounding`` ("nearest" or
+    "stochastic") is the 16-bit stores' rounding, as ``Stepper``'s: each
+    lattice's stochastic seeds follow its own step count (kept in device
+    memory and advanced by every run and graph replay), so a lattice rounds
+    the same alone and in the batch."""
 
     def __init__(self, specs: Sequence[LatticeSpec] = (), storage: str = "f32", grid_blocks: int = 0,
                  capacity_nodes: int = 0, max_slots: int = 0, blocked: bool = True, tile_cols: int = 0,
-                 align_rows: bool = True):
+                 align_rows: bool = True, rounding: str = "nearest"):
+        from .stepper import ROUNDING
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
+        if rounding not in ROUNDING:
+            raise ValueError(f"rounding must be one of {sorted(ROUNDING)}")
+        self.rounding = ROUNDING[rounding]
         t = torch()
         dev = device()
         specs = list(specs)
@@ -272,6 +280,7 @@ class BatchRunner:
         ctypes.memmove(ctypes.byref(L), ctypes.byref(spec.lattice), ctypes.sizeof(H.Lattice))
         L.plane_stride = ps
         L.storage = STORAGE[self.storage]
+        L.rounding = self.rounding
         L.n_neem = len(spec.neem)
         self._point(slot, o, ps)
         for e, (kind, d, ux, uy, rho) in enumerate(spec.neem):
@@ -283,6 +292,8 @@ class BatchRunner:
         H.check(H.lib().lbm_batch_update(self.h, slot, ctypes.byref(L), ctypes.byref(self.flds[slot]), rules),
                 "lbm_batch_update")
         H.check(H.lib().lbm_batch_set(self.h, slot, 1, 0, self._vel_ptr(o)), "lbm_batch_set")
+        # a new lattice starts at step 0 (the seeds of 16-bit stochastic rounding)
+        H.check(H.lib().lbm_batch_set_round_step(self.h, slot, 0, stream()), "lbm_batch_set_round_step")
         self._pending_local.add(slot)
         self._dirty = True
         return slot

@@ -8,7 +8,8 @@ force and observable fields), the collision's relaxation state (tau /
 tau_sym / 1/tau of MRT, per node), Shan-Chen psi with its halo and the step
 counter ``lb_field.time`` -- and ``load`` puts them back into a lattice of
 the same shape.  A run continued from a checkpoint gives the fields of the
-uninterrupted run bit for bit.  The file holds plain arrays only (loaded
+uninterrupted run bit for bit (16-bit storage too: the file keeps the step
+count that seeds its stochastic rounding).  The file holds plain arrays only (loaded
 with ``allow_pickle=False``).
 """
 from __future__ import annotations
@@ -27,7 +28,7 @@ def save(path, lb_field, collision=None) -> None:
     lbf = lb_field
     lbf.materialize()
     arrays = {"format": np.int32(FORMAT), "shape": np.array([lbf.NX, lbf.NY, lbf.nc], np.int64),
-              "time": np.int64(lbf.time[None])}
+              "time": np.int64(lbf.time[None]), "round_step": np.int64(lbf._round_step)}
     for name in _FIELDS:
         arrays[name] = getattr(lbf, name).to_numpy()
     if collision is not None:
@@ -58,6 +59,7 @@ def load(path, lb_field, collision=None) -> None:
     for name in _FIELDS:
         getattr(lbf, name).from_numpy(data[name])
     lbf.time[None] = int(data["time"])
+    lbf._round_step = int(data["round_step"]) if "round_step" in data else 0
     if collision is not None:
         st = collision.state_tensor()
         st[:, :lbf.NX * lbf.NY].copy_(torch().as_tensor(data["relaxation_state"]))

@@ -53,6 +53,7 @@ class LBField:
         self._neighbor_from_mask = False  # neighbor_classify ran and the table is mask-derived
         self._neighbor_dev = None
         self._st = None          # device state, allocated on first device use
+        self._round_step = 0     # steps taken by the fused runs (seeds of 16-bit stochastic rounding)
         self._finisher = None    # pending boundary pass of the last fused run
         self._pop_owner = None   # a Stepper whose 16-bit populations are the newest state (None: FP32)
         self._pressure_refresh = None
@@ -123,6 +124,7 @@ class LBField:
         g = group_device(group)
         H.check(H.lib().lbm_g_init_lbm(ctypes.byref(lattice_struct(self)), ctypes.byref(gfields(self)),
                                        g[0], g[1], stream()), "init_LBM")
+        self._round_step = 0
         self._pops_written()
         print("init LBM")
 
@@ -237,6 +239,7 @@ def lattice_struct(lbf: LBField, collision=None, sc=None, storage=H.STORE_F32, t
     L = H.Lattice()
     L.nx, L.ny, L.nc = lbf.NX, lbf.NY, lbf.nc
     L.plane_stride = lbf.ps
+    L.round_step = lbf._round_step & 0xFFFFFFFF
     L.storage = storage
     L.tau_mode = tau_mode
     L.tau_min, L.tau_max = _f32(0.5000001), _f32(4.0)

@@ -80,7 +80,8 @@ class ServeStats:
 
 class SolveServer:
     def __init__(self, storage: str = "f32", hbm_budget_bytes: int = 8 << 30, max_slots: int = 256,
-                 tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True, tile_cols: int = 0):
+                 tick_steps: int = 200, grid_blocks: int = 0, keep_results: bool = True, tile_cols: int = 0,
+                 rounding: str = "stochastic"):
         if tick_steps % 4:
             raise ValueError("tick_steps must be a multiple of 4 (graph replays of two-step launches keep one "
                              "parity schedule)")
@@ -92,9 +93,12 @@ class SolveServer:
         # two steps per launch where the storage allows it; the tick's graph
         # is captured again whenever the resident set changes (the two-step
         # tiling, its table and mode follow the resident lattices)
+        # 16-bit storage rounds stochastically by default: requests run to a
+        # steady state, where rounding to nearest would push every population
+        # the same way every step
         self.runner = BatchRunner([], storage=storage, grid_blocks=grid_blocks,
                                   capacity_nodes=self.capacity_nodes, max_slots=max_slots, blocked=True,
-                                  tile_cols=tile_cols)
+                                  tile_cols=tile_cols, rounding=rounding)
         self.changed = False
         self.tick_steps = int(tick_steps)
         self.keep_results = keep_results
@@ -207,12 +211,12 @@ def _residual(d2: float, u2: float):
 
 
 def solve_alone(spec: LatticeSpec, max_steps: int, tol: float, tick_steps: int, storage: str = "f32",
-                grid_blocks: int = 0, blocked: bool = False):
+                grid_blocks: int = 0, blocked: bool = False, rounding: str = "stochastic"):
     """The same request on its own: one lattice, the same tick / residual
     schedule (and the same stop on divergence); by default the plain path
     (one step per launch), which gives the same fields bit for bit as two
     steps per launch (blocked=True)."""
-    r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks, blocked=blocked)
+    r = BatchRunner([spec], storage=storage, grid_blocks=grid_blocks, blocked=blocked, rounding=rounding)
     steps, res = 0, float("inf")
     while True:
         r.run(tick_steps)

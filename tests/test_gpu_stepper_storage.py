@@ -11,6 +11,12 @@ g = f - w_k with FP32 arithmetic.
   populations only.
 * Shan-Chen lattices (one-step path with the psi pass) against the FP64
   vectors.
+* Stochastic rounding of the stores (rounding="stochastic"): a steady
+  channel in 16-bit storage against the same channel in FP32 after 20k
+  steps; two runs of one lattice are equal bit for bit, a different step
+  count changes the rounding, and a BatchRunner (eager runs and graph
+  replays, whose step counts live in device memory) rounds the lattice as
+  its Stepper does.
 """
 import contextlib
 import io
@@ -108,3 +114,89 @@ def test_shan_chen_shifted_storage_against_fixture(storage, name):
     print(f"  max |u - u_fp64| = {du:.2e}, max |rho - rho_fp64| = {drho:.2e} (mean rho {rho_mean:.3f})", flush=True)
     assert np.isfinite(u).all()
     assert du < 4 * BOUND[storage] / min(rho_mean, 1.0) and drho < 40 * BOUND[storage]
+
+
+def _channel(u_in=0.02):
+    from taichi_lbm import cases
+    with contextlib.redirect_stdout(io.StringIO()):
+        c = cases.cylinder(NX=120, NY=40, cx=30, cy=20, r=5, tau=0.8, u_in=u_in, inlet="bb", outlet="abc",
+                           name="sr-channel")
+        cases.initialise(c)
+    return c
+
+
+def _channel_error(storage, rounding, u_in, steps=20000):
+    from taichi_lbm.stepper import Stepper
+    ref, c16 = _channel(u_in), _channel(u_in)
+    Stepper(ref["lbf"], ref["macro"], ref["coll"], ref["bce"]).run(steps)
+    Stepper(c16["lbf"], c16["macro"], c16["coll"], c16["bce"], storage=storage, rounding=rounding).run(steps)
+    u32, u16 = ref["lbf"].vel.to_numpy(), c16["lbf"].vel.to_numpy()
+    assert np.isfinite(u16).all()
+    return float(np.abs(u16 - u32).max())
+
+
+# A steady channel (Re < 2, 20k steps; the FP32 velocity moves by 2e-7 over
+# the next 20k) in 16-bit storage against FP32 storage.  Rounded to nearest,
+# a steady flow rounds every population the same way every step and the
+# bias stays in the solution; stochastic stores leave about one unit in the
+# last place of noise instead.  Measured (max |u - u_f32|):
+#   u_in = 0.02 (max |u| 0.037): FP16 6.1e-5 -> 1.7e-5, BF16 3.1e-4 -> 1.5e-4
+#   u_in = 0.005 (max |u| 0.009): FP16 1.6e-5 -> 6.2e-6, BF16 1.05e-4 -> 3.3e-5
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_steady_channel(storage):
+    for u_in, bar in ((0.02, storage == "f16"), (0.005, True)):
+        print(f"steady channel 120x40 u_in {u_in}, 20000 steps, {storage} against f32", flush=True)
+        e_near = _channel_error(storage, "nearest", u_in)
+        e_sr = _channel_error(storage, "stochastic", u_in)
+        print(f"  max |u_{storage} - u_f32|: nearest {e_near:.2e}, stochastic {e_sr:.2e}", flush=True)
+        assert e_sr < 0.7 * e_near
+        if bar:
+            assert e_sr < 1e-4      # the velocity bar
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_is_deterministic(storage):
+    from taichi_lbm.stepper import Stepper
+    print(f"rounding seeds {storage}: repeat and shifted step count", flush=True)
+    a, b, c = _channel(), _channel(), _channel()
+    for case in (a, b, c):
+        st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], storage=storage, rounding="stochastic")
+        if case is c:
+            case["lbf"]._round_step = 1000     # the same state, other seeds
+        st.run(200)
+    ua, ub, uc = (x["lbf"].vel.to_numpy() for x in (a, b, c))
+    assert np.array_equal(ua, ub)
+    d = np.abs(ua - uc).max()
+    print(f"  other seeds move u by {d:.2e}", flush=True)
+    assert 0 < d < (1e-4 if storage == "f16" else 1e-3)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_batch_matches_stepper(storage):
+    from golden.make_golden import build
+    from taichi_lbm.batch import BatchRunner, spec_from_case
+    from taichi_lbm.stepper import Stepper
+    name = "cylinder_trt_vbb_48x24"
+    print(f"stochastic {storage}: Stepper against BatchRunner (eager + graph replays), {name}", flush=True)
+    case, _ = _device_case(name)
+    st = Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=case["power_law"],
+                 storage=storage, rounding="stochastic")
+    for n in (10, 20, 20):
+        st.run(n)
+    u = case["lbf"].vel.to_numpy()
+    with contextlib.redirect_stdout(io.StringIO()):
+        bcase, _ = build(name)
+    import torch
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        runner = BatchRunner([spec_from_case(bcase)], storage=storage, rounding="stochastic")
+        runner.run(10)
+        g = runner.capture(20)
+        g.replay()
+        g.replay()
+        ub = runner.fields(0)["vel"]
+        g.close()
+        runner.close()
+    torch.cuda.synchronize()
+    print(f"  max |u_stepper - u_batch| = {np.abs(u - ub).max():.2e}", flush=True)
+    assert np.array_equal(u, ub)
