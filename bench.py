@@ -193,7 +193,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
     out = {"lattices": n_lattices, "nodes": nodes, "fluid_nodes": fluid,
            "stored_nodes": sum(stored_nodes(sp) for sp in specs), "steps_per_solve": solve_steps,
            "graph_steps": graph_steps}
-    ms_b, tile_cols, grid, mode = {}, 0, 0, None
+    ms_b, tile_cols, grid, mode, unforced = {}, 0, 0, None, None
     with torch.cuda.stream(stream):
         for storage in storages:
             runner = BatchRunner(specs, storage=storage, grid_blocks=grid)
@@ -210,6 +210,8 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
                 runner.set_tile_cols(tile_cols)
                 runner.fit_tables()
             runner.run(10)
+            if unforced is None:
+                unforced = runner.unforced
             g = runner.capture(graph_steps)
             g.replay()
             reps = max(1, steps // graph_steps)
@@ -290,7 +292,7 @@ def batched_leg(n_lattices, seed, steps, plain_steps, solve_steps, graph_steps=1
         "max_abs_u_err_vs_fp64": err, "max_abs_u_fp64": umax, "accuracy_bar": bar,
         "matched_accuracy": {"batched": best_b, "plain": best_plain,
                              "speedup": round(ms_p[best_plain] / ms_b[best_b], 2) if best_b else None},
-        "launch_mode": mode, "grid_blocks": grid, "tile_cols": tile_cols,
+        "launch_mode": mode, "unforced_kernels": unforced, "grid_blocks": grid, "tile_cols": tile_cols,
         "finite_after_solve": {st: int(sum(v)) for st, v in finite.items()}, "fp64_sample": len(sample),
     })
     return out

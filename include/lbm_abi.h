@@ -314,7 +314,9 @@ int lbm_batch_graph(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid_b
  * rules); 2 = a two-node ring (node rules); 3 = node rules with a one-node
  * ring: each lattice's tile grid is shifted (found at commit by scanning the
  * rule nodes) so that every NEEM / OpenNeumann node shares its tile with the
- * interior neighbour it reads.  lbm_batch_ring(b, 2) forces mode 2 instead of
+ * interior neighbour it reads; plus 16 when no active lattice has a body
+ * force (one-component batches in 32- or 64-column tiles then run kernels
+ * built without the force terms).  lbm_batch_ring(b, 2) forces mode 2 instead of
  * 3 (0 = automatic); lbm_batch_tiles2 is the two-step tile count.  A two-step launch flips the
  * buffers once; the parity of the buffer with the newest f2 is returned in
  * *out_parity.  grid_blocks applies to the one-step launches only; a

@@ -429,11 +429,19 @@ template <class P, class = void>
 struct has_extra_force : std::false_type {};
 template <class P>
 struct has_extra_force<P, std::void_t<decltype(&P::fx)>> : std::true_type {};
+// Psi sources whose steps always carry a force (the Shan-Chen tiles: their
+// lattices have a Shan-Chen force; the IB fix-up: the spread force); the
+// force terms are then compiled in unconditionally.
+template <class P>
+struct always_forced : has_extra_force<P> {};
 
 // S1..S6 of one step at node n = (i, j).  state_in is the per-node relaxation
 // state (BGK tau, TRT tau_sym, MRT 1/tau) for LBM_TAU_FIELD / _POWER_LAW.
 // Shan-Chen reads psi of this step through `psi` (PsiPlanes and the like).
-template <int COLL, int NC, typename R, class PS>
+// FMODE: 0 reads the force switch at run time; 1 (kernels that run only
+// lattices with a body force) compiles the force terms in, 2 (lattices
+// without one) leaves them out -- no run-time switch either way.
+template <int COLL, int NC, typename R, class PS, int FMODE = 0>
 __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t n, uint32_t i, uint32_t j,
                                           const R (&f)[NC][9], const R (&state_in)[NC], StepOut<NC, R>& o) {
   const uint32_t ps = a.ps;
@@ -514,7 +522,7 @@ __device__ __forceinline__ void step_math(const Lat& a, const PS& psi, uint32_t 
   }
 
   // S3 force_density (lbm.py:215-220)
-  const bool force = a.force || a.sc != LBM_SC_NONE || has_extra_force<PS>::value;
+  const bool force = FMODE == 1 || always_forced<PS>::value || (FMODE == 0 && (a.force || a.sc != LBM_SC_NONE));
   const R inv_rtot = rcp(o.rtot);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -842,8 +850,14 @@ constexpr int kBlock2 = 1024;
 // node rules (a rule node's interior neighbour may lie in the ring); 3 =
 // one-node ring with node rules (every such neighbour lies in its rule
 // node's tile: batches shift a lattice's tile grid to make it so).
-constexpr int ring_of(int m) { return m == 2 ? 2 : 1; }
-constexpr bool rules_of(int m) { return m != 1; }
+// Mode of a two-step tile kernel: 1 = one-node ring without node rules, 2 =
+// two-node ring with rules, 3 = one-node ring with rules (batches, shifted
+// tile grids); plus kModeForced / kModeUnforced for kernels built for
+// lattices with / without a body force (step_math's FMODE 1 / 2).
+constexpr int kModeForced = 8, kModeUnforced = 16;
+constexpr int ring_of(int m) { return (m & 7) == 2 ? 2 : 1; }
+constexpr bool rules_of(int m) { return (m & 7) != 1; }
+constexpr int fmode_of(int m) { return (m & kModeForced) ? 1 : ((m & kModeUnforced) ? 2 : 0); }
 // Threads of a two-step block (one per node of the ring-extended tile).  A
 // tile's two steps are a chain of dependent phases (code, gathers,
 // arithmetic, barrier, LDS, arithmetic, stores), so several smaller blocks
@@ -1015,7 +1029,7 @@ __device__ __forceinline__ void ring_step(const Lat& a, bool local, const typena
     }
   }
   StepOut<NC, R> o;
-  step_math<COLL, NC, R>(a, PsiNone<R>{}, n, gi, gj, in.f, in.st, o);
+  step_math<COLL, NC, R, PsiNone<R>, fmode_of(M)>(a, PsiNone<R>{}, n, gi, gj, in.f, in.st, o);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
 #pragma unroll
@@ -1106,7 +1120,7 @@ __device__ __forceinline__ void tile_second_step(const Lat& a, const Launch2& q,
     for (int c = 0; c < NC; ++c)
       st_in[c] = pl ? L.st[c][x.q] : (a.tau_mode == LBM_TAU_FIELD ? ((const R*)a.tau)[(size_t)c * ps + n] : R(0));
     StepOut<NC, R> o;
-    step_math<COLL, NC, R>(a, PsiNone<R>{}, n, gi, gj, f, st_in, o);
+    step_math<COLL, NC, R, PsiNone<R>, fmode_of(M)>(a, PsiNone<R>{}, n, gi, gj, f, st_in, o);
     store_pops<NC, S, BUF>(a, q.round_step, dst, rdst, ps, n, gi, gj, o.f);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -1206,6 +1220,8 @@ struct PsiTile {
   // x + c_k = slot q - back(opp(k)) = q + back(k)
   __device__ __forceinline__ bool solid(int k) const { return S[q + Shape::back(k)] != 0; }
 };
+template <typename R, class Shape>
+struct always_forced<PsiTile<R, Shape>> : std::true_type {};
 
 // Eight waves per SIMD where the registers allow it without spilling (one
 // component, or two with MRT: 62 VGPRs; two-component BGK / TRT would spill)
@@ -1949,6 +1965,7 @@ struct lbm_batch {
   size_t tile2_cap = 0;              // 8 * tile2_stride
   uint32_t tiles2 = 0;
   int mode2 = 0;                     // 0: one step per launch, else the ring width H
+  bool unforced = false;             // no active lattice has a body force (kernels without force terms)
   int tile_cols = lbm::kDefaultTileCols;   // two-step tile columns (16, 32 or 64)
   int ring_pref = 0;                 // 0: one-node ring with rules where the tile grids allow it; 2: two-node
   bool power_law = false;            // an active lattice has a power-law state (two-step: tau / tau2)
@@ -2065,7 +2082,26 @@ bool fits_buffer(const lbm_lattice* lat) {
 // high-precision reference path; in a 1024-thread block its registers would
 // spill) runs one step per launch.  Lattices beyond buffer addressing use
 // 64-column tiles.
-Kernel2Fn pick2(int nc, int store, int coll, int h, int w, bool buf) {
+// The one-component kernels of 32 and 64 columns with buffer addressing come
+// in builds for lattices with and without a body force (F = kModeForced /
+// kModeUnforced); fmode 1 / 2 picks them.
+template <int H, int W, int F>
+Kernel2Fn pick2_fm(int store, int coll) {
+  switch (store) {
+    case LBM_STORE_F32: return pick2_coll<1, LBM_STORE_F32, H | F, W, true>(coll);
+    case LBM_STORE_F16_SHIFT: return pick2_coll<1, LBM_STORE_F16_SHIFT, H | F, W, true>(coll);
+    case LBM_STORE_BF16_SHIFT: return pick2_coll<1, LBM_STORE_BF16_SHIFT, H | F, W, true>(coll);
+    default: return nullptr;
+  }
+}
+template <int F>
+Kernel2Fn pick2_f(int store, int coll, int h, int w) {
+  if (w == 32) return h == 1 ? pick2_fm<1, 32, F>(store, coll) : pick2_fm<2, 32, F>(store, coll);
+  return h == 1 ? pick2_fm<1, 64, F>(store, coll) : pick2_fm<2, 64, F>(store, coll);
+}
+Kernel2Fn pick2(int nc, int store, int coll, int h, int w, bool buf, int fmode = 0) {
+  if (fmode && buf && nc == 1 && (w == 32 || w == 64))
+    return fmode == 1 ? pick2_f<kModeForced>(store, coll, h, w) : pick2_f<kModeUnforced>(store, coll, h, w);
   if (!buf) return h == 1 ? pick2_h<1, 64, false>(nc, store, coll) : pick2_h<2, 64, false>(nc, store, coll);
   if (w == 16) return h == 1 ? pick2_h<1, 16, true>(nc, store, coll) : pick2_h<2, 16, true>(nc, store, coll);
   if (w == 32) return h == 1 ? pick2_h<1, 32, true>(nc, store, coll) : pick2_h<2, 32, true>(nc, store, coll);
@@ -2214,7 +2250,8 @@ int run_blocked(const lbm_lattice* lat, const lbm_fields* fld, const lbm_neem_ru
   if (!valid_tile_cols(tile_cols)) return fail("tile_cols must be 0, 16, 32 or 64");
   const bool buf = fits_buffer(lat);
   const int w = buf ? tile_cols_of(tile_cols) : 64;
-  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, w, buf);
+  // lattices in this path have no Shan-Chen force: gravity is their only one
+  Kernel2Fn two = pick2(lat->nc, lat->storage, lat->collision, h, w, buf, (lat->gx != 0.0f || lat->gy != 0.0f) ? 1 : 2);
   if (lat->shan_chen != LBM_SC_NONE) {
     const Lat a = make_lat(lat, fld, neem);
     if (run_sc2(lat, a, nsteps, (uint32_t)parity, first_local, s, &rc, out_parity)) return rc;
@@ -2596,6 +2633,9 @@ extern "C" int lbm_batch_commit(lbm_batch* b, void* stream) {
     }
   }
   b->mode2 = mode;
+  b->unforced = true;
+  for (int l = 0; l < b->count; ++l)
+    if (b->host[l].active && b->host[l].force) b->unforced = false;
   b->tiles2 = (uint32_t)b->host_t2.size();
   if ((b->tiles2 + 7) / 8 > b->tile2_stride || !b->tile2) {
     if (b->captured)
@@ -2681,10 +2721,10 @@ int round_tick(lbm_batch* b, int32_t nsteps, hipStream_t s) {
 
 namespace {
 using Batch2Fn = void (*)(const lbm::Lat*, const lbm::Tile2*, uint32_t, const lbm::Launch2);
-template <int NC, int S, int W>
+template <int NC, int S, int W, int F = 0>
 Batch2Fn pick_batch2_w(int m) {
-  return m == 1 ? lbm::k_batch2<NC, S, 1, W, true> : (m == 2 ? lbm::k_batch2<NC, S, 2, W, true>
-                                                              : lbm::k_batch2<NC, S, 3, W, true>);
+  return m == 1 ? lbm::k_batch2<NC, S, 1 | F, W, true>
+                : (m == 2 ? lbm::k_batch2<NC, S, 2 | F, W, true> : lbm::k_batch2<NC, S, 3 | F, W, true>);
 }
 template <int NC, int S>
 Batch2Fn pick_batch2_s(int m, int w) {
@@ -2694,7 +2734,21 @@ Batch2Fn pick_batch2_s(int m, int w) {
   if (w == 32) return pick_batch2_w<NC, S, 32>(m);
   return pick_batch2_w<NC, S, 64>(m);
 }
-Batch2Fn pick_batch2(int nc, int store, int h, int w) {
+// Batches of one component whose lattices have no body force, in 32- or
+// 64-column tiles: kernels built without the force terms.
+template <int S>
+Batch2Fn pick_batch2_unforced(int m, int w) {
+  return w == 32 ? pick_batch2_w<1, S, 32, lbm::kModeUnforced>(m) : pick_batch2_w<1, S, 64, lbm::kModeUnforced>(m);
+}
+Batch2Fn pick_batch2(int nc, int store, int h, int w, bool unforced = false) {
+  if (unforced && nc == 1 && (w == 32 || w == 64)) {
+    switch (store) {
+      case LBM_STORE_F32: return pick_batch2_unforced<LBM_STORE_F32>(h, w);
+      case LBM_STORE_F16_SHIFT: return pick_batch2_unforced<LBM_STORE_F16_SHIFT>(h, w);
+      case LBM_STORE_BF16_SHIFT: return pick_batch2_unforced<LBM_STORE_BF16_SHIFT>(h, w);
+      default: return nullptr;
+    }
+  }
   switch (store) {
     case LBM_STORE_F32:
       return nc == 1 ? pick_batch2_s<1, LBM_STORE_F32>(h, w) : pick_batch2_s<2, LBM_STORE_F32>(h, w);
@@ -2712,7 +2766,7 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
                    int32_t* out) {
   using namespace lbm;
   if (!b->tile_item) return fail("commit the batch before stepping it");
-  Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2, b->tile_cols) : nullptr;
+  Batch2Fn two = b->mode2 ? pick_batch2(b->nc, b->storage, b->mode2, b->tile_cols, b->unforced) : nullptr;
   uint32_t p = (uint32_t)parity;
   const int total = nsteps;
   int t0 = 0;
@@ -2757,7 +2811,11 @@ int batch_sequence(lbm_batch* b, int32_t nsteps, int32_t parity, int32_t grid, h
 }
 }  // namespace
 
-extern "C" int lbm_batch_mode(const lbm_batch* b) { return b ? b->mode2 : -1; }
+extern "C" int lbm_batch_mode(const lbm_batch* b) {
+  if (!b) return -1;
+  const bool unforced = b->mode2 && b->unforced && b->nc == 1 && (b->tile_cols == 32 || b->tile_cols == 64);
+  return b->mode2 | (unforced ? lbm::kModeUnforced : 0);
+}
 
 extern "C" int lbm_batch_tile_cols(lbm_batch* b, int32_t tile_cols) {
   using namespace lbm;

@@ -388,6 +388,15 @@ class BatchRunner:
         """0: one step per launch; 1: two-step launches, one-node ring; 2:
         two-node ring (node rules); 3: node rules on a one-node ring with
         shifted tile grids."""
+        return self._kernel_mode() & 7
+
+    @property
+    def unforced(self) -> bool:
+        """The two-step kernels are built without the force terms (no
+        active lattice has a body force)."""
+        return bool(self._kernel_mode() & 16)
+
+    def _kernel_mode(self) -> int:
         self.commit()
         return H.lib().lbm_batch_mode(self.h) if self.blocked else 0
 
@@ -427,7 +436,7 @@ class BatchRunner:
             H.check(H.lib().lbm_batch_graph(self.h, int(nsteps), self.parity, self.grid, s, ctypes.byref(g)),
                     "lbm_batch_graph")
         self._open_graphs += 1
-        return BatchGraph(self, g, int(nsteps), mode)
+        return BatchGraph(self, g, int(nsteps), self._kernel_mode())
 
     def _graph_closed(self):
         self._open_graphs -= 1
@@ -502,9 +511,9 @@ class BatchGraph:
         r = self.runner
         if r.parity != self.parity:
             raise RuntimeError("the batch parity moved since capture (an odd eager run in between)")
-        if r.mode != self.mode:
-            raise RuntimeError("the batch's launch mode changed since capture (power law or node rules "
-                               "admitted); capture a new graph")
+        if r._kernel_mode() != self.mode:
+            raise RuntimeError("the batch's launch mode changed since capture (power law, node rules or a "
+                               "body force admitted); capture a new graph")
         r.commit()
         H.check(H.lib().lbm_graph_launch(self.g, stream()), "lbm_graph_launch")
         r.finished = False
