@@ -76,8 +76,9 @@ def time_gpu(case, steps, warmup, storage="f32", rounding="nearest"):
 
 def kernel_symbol(tile_cols):
     """The MRT single-component FP32 two-step kernel a run with these tile
-    columns launches (as the profiler names it)."""
-    return f"k_fused2<2, 1, 0, 1, {tile_cols}, true>"
+    columns launches (as the profiler names it): mode 9 is the one-node ring
+    without node rules (1) built for lattices with a body force (8)."""
+    return f"k_fused2<2, 1, 0, 9, {tile_cols}, true>"
 
 
 def kernel_label(tile_cols):
