@@ -293,3 +293,32 @@ def test_power_law_batch_two_steps():
         torch.cuda.synchronize()
         same()
         g.close()
+
+
+def test_force_variants_equal_single_runs():
+    """A batch without body forces runs kernels built without the force
+    terms (BatchRunner.unforced); one lattice with gravity switches the batch
+    to the kernels that read the force switch at run time.  Either way every
+    lattice equals its own Stepper run (which takes the forced or unforced
+    two-step build) bit for bit."""
+    import contextlib
+    import io
+    cs = _mix()[:3]
+    with contextlib.redirect_stdout(io.StringIO()):
+        porous = cases.porous(NX=64, NY=48, seed=2, collision="mrt", rmin=3.0, rmax=6.0, g=(2e-5, 1e-6))
+    plain = BatchRunner([spec_from_case(c) for c in cs], storage="f32")
+    mixed = BatchRunner([spec_from_case(c) for c in cs + [porous]], storage="f32")
+    assert plain.unforced and not mixed.unforced
+    n = 24
+    plain.run(n)
+    mixed.run(n)
+    for q, c in enumerate(cs + [porous]):
+        print(f"lattice {q}", flush=True)
+        with contextlib.redirect_stdout(io.StringIO()):
+            cases.initialise(c)
+        Stepper(c["lbf"], c["macro"], c["coll"], c["bce"], power_law=c["power_law"]).run(n)
+        ref = c["lbf"].vel.to_numpy()
+        for r, slot in ((mixed, q),) + (((plain, q),) if q < len(cs) else ()):
+            assert np.array_equal(r.fields(slot)["vel"], ref), q
+    plain.close()
+    mixed.close()
