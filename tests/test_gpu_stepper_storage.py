@@ -31,10 +31,10 @@ pytestmark = pytest.mark.gpu
 BOUND = {"f16": 7e-5, "bf16": 5e-4}
 
 
-def _stepper(case, storage):
+def _stepper(case, storage, rounding="nearest"):
     from taichi_lbm.stepper import Stepper
     return Stepper(case["lbf"], case["macro"], case["coll"], case["bce"], power_law=case["power_law"],
-                   boundary_engine_sc=case.get("bsc"), storage=storage)
+                   boundary_engine_sc=case.get("bsc"), storage=storage, rounding=rounding)
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
@@ -200,3 +200,25 @@ def test_stochastic_rounding_batch_matches_stepper(storage):
     torch.cuda.synchronize()
     print(f"  max |u_stepper - u_batch| = {np.abs(u - ub).max():.2e}", flush=True)
     assert np.array_equal(u, ub)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_stochastic_rounding_shan_chen(storage):
+    """Shan-Chen lattices in 16-bit storage (psi pass + one-step kernel)
+    round stochastically too: two runs equal bit for bit, and the fields stay
+    within the FP64 fixture bounds of rounding to nearest."""
+    name = "shan_chen_c2_32x24"
+    print(f"stochastic {storage} Stepper {name}", flush=True)
+    fx = _load(name)
+    outs = []
+    for _ in range(2):
+        case, steps = _device_case(name)
+        _stepper(case, storage, rounding="stochastic").run(steps)
+        outs.append((case["lbf"].vel.to_numpy(), case["lbf"].rho.to_numpy()))
+    (u, rho), (u2, rho2) = outs
+    assert np.array_equal(u, u2) and np.array_equal(rho, rho2)
+    du = np.abs(u[_s(fx, "out64")] - fx["out64_vel"]).max()
+    drho = np.abs(rho[_s(fx, "out64")] - fx["out64_rho"]).max()
+    print(f"  max |u - u_fp64| = {du:.2e}, max |rho - rho_fp64| = {drho:.2e}", flush=True)
+    assert np.isfinite(u).all()
+    assert du < 4 * BOUND[storage] and drho < 40 * BOUND[storage]
