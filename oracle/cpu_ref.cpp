@@ -8,7 +8,10 @@
 // CPU-baseline leg load it.
 //
 // Per step (cases/Power Law Fluid.py:160-173): density (lbm.py:153-164) ->
-// pressure0 (:166-173) -> force_density (:215-220) -> velocity (:223-237) ->
+// pressure0 (:166-173) -> [Shan-Chen: update_psi -> PeriodicPsi_LR/BT ->
+// cau_SCforce (cases/Bubble or two-phase separation.py:197-212,
+// ExternalForce.py:42-117, SCBoundaryCondition.py:126-144)] ->
+// force_density (:215-220) -> velocity (:223-237) ->
 // [shear_rate -> relaxation_time] (CollisionEngine.py:72-92, 157-178,
 // 322-377) -> collide (:96-110, 223-245, 424-451) -> boundary list in order
 // (BoundaryCondition.py:154-166, 215-226, 244-259, 424-449).
@@ -32,6 +35,11 @@ typedef struct cref_params {
   float magic;        // TRT
   float s_bulk;       // MRT diag[1]
   float visc_m, visc_nm1, tau_min, tau_max, gx, gy;
+  // Shan-Chen with SC_psi on a periodic lattice whose nodes are all fluid
+  // for the force (no adhesion, no psi boundaries other than periodic)
+  int32_t sc;          // 0 none, 1 ShanChenForceC1 (component 0), 2 ShanChenForceC2
+  float g_coh[4];      // C1: g_coh[0]; C2: the (nc, nc) matrix, row-major
+  float psi_rho0[2];   // SC_psi rho0 (C1: [0]; C2: per component)
 } cref_params;
 
 typedef struct cref_bc {
@@ -63,6 +71,7 @@ const float MD[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
 struct Ctx {
   cref_params p;
   float *f, *f2, *rho, *trho, *vel, *bf, *shear, *tau, *pressure, *tpressure;
+  float *sc, *psi;     // SCforce (NX, NY, nc, 2); psi with its halo ((NX+2), (NY+2), nc), offset -1
   const int32_t* nb;
   const int32_t* grp;
   int32_t gcount;
@@ -129,14 +138,63 @@ void step(Ctx& C, const cref_bc* bcs, int nbc) {
     C.pressure[n * nc] = C.pressure[n * nc] + C.rho[n * nc] / 3.0f;
     C.tpressure[n] = C.pressure[n * nc];
   }
-  // force_density (lbm.py:215-220; no Shan-Chen force here)
+  if (p.sc) {
+    const int py = ny + 2;
+    auto P = [&](int i, int j, int c) -> float& { return C.psi[((size_t)(i + 1) * py + (j + 1)) * nc + c]; };
+    // update_psi (ExternalForce.py:42-47, 86-92): SC_psi of each component's
+    // density on the group (C1: one psi object for every component)
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < C.gcount; ++m) {
+      const int i = C.grp[2 * m], j = C.grp[2 * m + 1];
+      const size_t n = C.node(i, j);
+      for (int c = 0; c < nc; ++c) {
+        const float r0 = p.sc == 1 ? p.psi_rho0[0] : p.psi_rho0[c];
+        P(i, j, c) = r0 * (1.0f - expf(-C.rho[n * nc + c] / r0));
+      }
+    }
+    // PeriodicPsi_LR, then PeriodicPsi_BT (SCBoundaryCondition.py:126-144)
+    for (int j = -1; j <= ny; ++j)
+      for (int c = 0; c < nc; ++c) P(-1, j, c) = P(nx - 1, j, c), P(nx, j, c) = P(0, j, c);
+    for (int i = -1; i <= nx; ++i)
+      for (int c = 0; c < nc; ++c) P(i, -1, c) = P(i, ny - 1, c), P(i, ny, c) = P(i, 0, c);
+    // cau_SCforce: SCforce := 0, then C1 (:78-91) F_0 = -G psi_0(x) sum_k w_k c_k psi_0(x + c_k);
+    // C2 (:96-117, every neighbour fluid) F_c1 = -psi_c1(x) sum_k sum_{c2 != c1} G[c1][c2] w_k c_k psi_c2(x + c_k)
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < (long)N * nc * 2; ++n) C.sc[n] = 0.0f;
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < C.gcount; ++m) {
+      const int i = C.grp[2 * m], j = C.grp[2 * m + 1];
+      const size_t n = C.node(i, j);
+      for (int c1 = 0; c1 < (p.sc == 1 ? 1 : nc); ++c1) {
+        float Fx = 0.0f, Fy = 0.0f;
+        for (int k = 1; k < 9; ++k) {
+          const float wx = W[k] * (float)CX[k], wy = W[k] * (float)CY[k];
+          if (p.sc == 1) {
+            const float v = P(i + CX[k], j + CY[k], 0);
+            Fx += wx * v, Fy += wy * v;
+          } else {
+            for (int c2 = 0; c2 < nc; ++c2) {
+              if (c2 == c1) continue;
+              const float v = P(i + CX[k], j + CY[k], c2) * p.g_coh[c1 * 2 + c2];
+              Fx += wx * v, Fy += wy * v;
+            }
+          }
+        }
+        const float s = p.sc == 1 ? -p.g_coh[0] * P(i, j, 0) : -P(i, j, c1);
+        C.sc[(n * nc + c1) * 2] = Fx * s;
+        C.sc[(n * nc + c1) * 2 + 1] = Fy * s;
+      }
+    }
+  }
+  // force_density (lbm.py:215-220)
 #pragma omp parallel for schedule(static)
   for (long m = 0; m < C.gcount; ++m) {
     const size_t n = C.node(C.grp[2 * m], C.grp[2 * m + 1]);
     for (int c = 0; c < nc; ++c) {
       const float frac = C.rho[n * nc + c] / C.trho[n];
-      C.bf[(n * nc + c) * 2] = 0.0f + frac * p.gx;
-      C.bf[(n * nc + c) * 2 + 1] = 0.0f + frac * p.gy;
+      const float sx = p.sc ? C.sc[(n * nc + c) * 2] : 0.0f, sy = p.sc ? C.sc[(n * nc + c) * 2 + 1] : 0.0f;
+      C.bf[(n * nc + c) * 2] = sx + frac * p.gx;
+      C.bf[(n * nc + c) * 2 + 1] = sy + frac * p.gy;
     }
   }
   // velocity (lbm.py:223-237)
@@ -365,10 +423,13 @@ void step(Ctx& C, const cref_bc* bcs, int nbc) {
 
 }  // namespace
 
+// sc_force (NX, NY, nc, 2) and psi ((NX+2), (NY+2), nc) are read and written
+// when p->sc is set (may be NULL otherwise).
 extern "C" int cref_run(const cref_params* p, float* f, float* f2, float* rho, float* total_rho, float* vel,
                         float* body_force, float* shear_rate, float* tau_state, float* pressure,
                         float* total_pressure, const int32_t* neighbor, const int32_t* group, int32_t count,
-                        const cref_bc* bcs, int32_t nbc, int32_t nsteps) {
+                        const cref_bc* bcs, int32_t nbc, int32_t nsteps, float* sc_force, float* psi) {
+  if (p->sc && (!sc_force || !psi || p->nc > 2 || (p->sc == 2 && p->nc != 2))) return 1;
 #ifdef _OPENMP
   if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
@@ -377,6 +438,7 @@ extern "C" int cref_run(const cref_params* p, float* f, float* f2, float* rho, f
   C.f = f, C.f2 = f2, C.rho = rho, C.trho = total_rho, C.vel = vel, C.bf = body_force;
   C.shear = shear_rate, C.tau = tau_state, C.pressure = pressure, C.tpressure = total_pressure;
   C.nb = neighbor, C.grp = group, C.gcount = count;
+  C.sc = sc_force, C.psi = psi;
   for (int k = 0; k < 9; ++k)
     for (int r = 0; r < 9; ++r) C.minv[k][r] = MM[r][k] * (1.0f / MD[r]);
   for (int s = 0; s < nsteps; ++s) step(C, bcs, nbc);

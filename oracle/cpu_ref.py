@@ -18,7 +18,8 @@ class Params(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nc", C.c_int32), ("coll", C.c_int32),
                 ("power_law", C.c_int32), ("nthreads", C.c_int32), ("magic", C.c_float), ("s_bulk", C.c_float),
                 ("visc_m", C.c_float), ("visc_nm1", C.c_float), ("tau_min", C.c_float), ("tau_max", C.c_float),
-                ("gx", C.c_float), ("gy", C.c_float)]
+                ("gx", C.c_float), ("gy", C.c_float), ("sc", C.c_int32), ("g_coh", C.c_float * 4),
+                ("psi_rho0", C.c_float * 2)]
 
 
 class BC(C.Structure):
@@ -54,10 +55,17 @@ class CpuLattice:
     """State of one lattice in the reference's AoS layout."""
 
     def __init__(self, NX, NY, nc, coll, group, bcs, neighbor, *, power_law=False, magic=0.25, s_bulk=1.0,
-                 visc=(0.0, 0.0), tau_clamp=(0.5000001, 4.0), g=(0.0, 0.0), nthreads=0):
+                 visc=(0.0, 0.0), tau_clamp=(0.5000001, 4.0), g=(0.0, 0.0), nthreads=0, sc=0, g_coh=(0.0,) * 4,
+                 psi_rho0=(1.0, 1.0)):
+        """sc: 0, 1 (ShanChenForceC1) or 2 (ShanChenForceC2) with SC_psi(psi_rho0), g_coh
+        the C1 constant ([0]) or the C2 matrix (row-major), on a periodic
+        lattice that is all fluid for the force."""
         f32 = np.float32
         self.p = Params(NX, NY, nc, coll, int(power_law), nthreads, f32(magic), f32(s_bulk), f32(visc[0]),
-                        f32(visc[1]), f32(tau_clamp[0]), f32(tau_clamp[1]), f32(g[0]), f32(g[1]))
+                        f32(visc[1]), f32(tau_clamp[0]), f32(tau_clamp[1]), f32(g[0]), f32(g[1]), int(sc),
+                        (C.c_float * 4)(*[f32(v) for v in g_coh]), (C.c_float * 2)(*[f32(v) for v in psi_rho0]))
+        self.SCforce = np.zeros((NX, NY, nc, 2), f32)
+        self.psi = np.zeros((NX + 2, NY + 2, nc), f32)
         self.f = np.zeros((NX, NY, nc, 9), f32)
         self.f2 = np.zeros_like(self.f)
         self.rho = np.zeros((NX, NY, nc), f32)
@@ -87,7 +95,7 @@ class CpuLattice:
         rc = lib().cref_run(C.byref(self.p), P(self.f), P(self.f2), P(self.rho), P(self.total_rho), P(self.vel),
                             P(self.body_force), P(self.shear_rate), P(self.tau), P(self.pressure),
                             P(self.total_pressure), P(self.neighbor), P(self.group), C.c_int32(len(self.group)),
-                            self._bcs, C.c_int32(self._nbc), C.c_int32(nsteps))
+                            self._bcs, C.c_int32(self._nbc), C.c_int32(nsteps), P(self.SCforce), P(self.psi))
         if rc != 0:
             raise RuntimeError(f"cref_run failed ({rc})")
 
@@ -121,9 +129,22 @@ def from_case(case, nthreads=0):
         tb = min(max(tb, np.float32(coll.tau_min)), np.float32(coll.tau_max))
         s_bulk = np.float32(1.0) / np.float32(tb)
     group = case["fluid"].group.nodes
+    sc = {}
+    spec = case.get("sc_spec")
+    if spec is not None:
+        obj = lbf.sc_field
+        psis = [obj.psi] if spec["kind"] == "c1" else [obj.psi1, obj.psi2]
+        if (any(type(q).__name__ != "SC_psi" for q in psis) or getattr(obj, "fluid_map", None) is not None
+                or any(type(b).__name__ not in ("PeriodicPsi_LR", "PeriodicPsi_BT")
+                       for b in case["bsc"].boundary_conditions.values())):
+            raise NotImplementedError("the C++ restatement takes SC_psi on periodic all-fluid lattices only")
+        g_coh = ([float(spec["g_coh"])] + [0.0] * 3 if spec["kind"] == "c1"
+                 else [float(v) for v in np.asarray(spec["g_coh"], np.float64).reshape(-1)])
+        sc = dict(sc=1 if spec["kind"] == "c1" else 2, g_coh=g_coh,
+                  psi_rho0=[float(v) for v in spec["rho0"]] + [1.0] * (2 - len(spec["rho0"])))
     cl = CpuLattice(lbf.NX, lbf.NY, lbf.nc, kind, group, bcs, lbf.neighbor.arr, power_law=case["power_law"],
                     magic=float(getattr(coll, "magic", [0.25])[0]), s_bulk=float(s_bulk), visc=visc,
-                    tau_clamp=(coll.tau_min, coll.tau_max), g=lbf.gravity_force, nthreads=nthreads)
+                    tau_clamp=(coll.tau_min, coll.tau_max), g=lbf.gravity_force, nthreads=nthreads, **sc)
     # initTau: the collision's per-node state on the group
     st = coll.init_state_values()
     I, J = group[:, 0], group[:, 1]
@@ -131,7 +152,8 @@ def from_case(case, nthreads=0):
         cl.tau[I, J, c] = st[c]
     cl.rho[...] = case["rho"]
     cl.vel[...] = case["vel"]
-    fe = O.f_eq(cl.vel[I, J].astype(np.float32), cl.rho[I, J, 0].astype(np.float32), np.float32)
-    cl.f[I, J, 0] = fe
-    cl.f2[I, J, 0] = fe
+    for c in range(lbf.nc):
+        fe = O.f_eq(cl.vel[I, J].astype(np.float32), cl.rho[I, J, c].astype(np.float32), np.float32)
+        cl.f[I, J, c] = fe
+        cl.f2[I, J, c] = fe
     return cl

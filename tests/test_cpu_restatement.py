@@ -43,6 +43,34 @@ def test_cylinder_velocity_bb_inlet():
     _compare(case, 30)
 
 
+@pytest.mark.parametrize("nc,coll", [(1, "mrt"), (2, "mrt"), (2, "bgk")])
+def test_shan_chen(nc, coll):
+    """ShanChenForceC1 / C2 with SC_psi on a periodic lattice (the
+    two-phase separation case's step order): the force, psi and the fields
+    agree with the NumPy oracle over 40 steps."""
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.shan_chen(NX=40, NY=32, nc=nc, gcross=0.9, seed=5, collision=coll)
+    cl = cpu_ref.from_case(case, nthreads=2)
+    cl.run(40)
+    sol = oracle_solver(case, np.float32)
+    sol.run(40)
+    for name in ("f", "rho", "vel", "total_rho", "SCforce", "body_force"):
+        check(name, getattr(cl, name), getattr(sol.lat, name))
+    check("psi", cl.psi, sol.lat.psi)
+
+
+def test_shan_chen_refuses_what_it_does_not_restate():
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):
+        case = cases.shan_chen(NX=40, NY=32, nc=2, gcross=0.9, seed=5, collision="mrt", disk=(20, 16, 5),
+                               gadh=(0.3, -0.3))
+    with pytest.raises(NotImplementedError):
+        cpu_ref.from_case(case, nthreads=2)
+
+
 _ASAN_RUN = r"""
 import contextlib, io, sys
 sys.path.insert(0, "taichi-lbm_amd"); sys.path.insert(0, ".")

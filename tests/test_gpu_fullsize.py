@@ -16,8 +16,9 @@ Fluid.py`'s step order):
 
 The 2048 x 2048 two-component Shan-Chen separation (MRT, G_cross = 0.9,
 densities 1 +- 1 % seeded per component; `cases/Bubble or two-phase
-separation.py`'s step order): 2 steps of the LDS-tiled Shan-Chen kernel
-against 2 steps of the FP32 NumPy oracle (about 12 s a step on the CPU),
+separation.py`'s step order): 40 steps of the LDS-tiled Shan-Chen kernels
+(two steps per launch) against 40 steps of the C++ restatement (its
+Shan-Chen step agrees with the NumPy oracle, tests/test_cpu_restatement.py),
 every field within the FP32 parity bounds.
 
 The 512 batched cylinder channels: a sample of lattices against the same
@@ -85,22 +86,24 @@ def test_headline_lattice_matches_cpu_restatement(collision):
 
 
 def test_shan_chen_2048_two_components_matches_oracle():
-    from oracle_bridge import oracle_solver
+    import os
+    from oracle import cpu_ref
     from taichi_lbm import cases
-    n = 2
-    print(f"case shan-chen 2048x2048x2 MRT, {n} steps", flush=True)
+    n = 40
+    threads = min(16, os.cpu_count() or 1)
+    print(f"case shan-chen 2048x2048x2 MRT, {n} steps (C++ restatement on {threads} threads)", flush=True)
     with contextlib.redirect_stdout(io.StringIO()):
         case = cases.shan_chen(NX=2048, NY=2048, nc=2, gcross=0.9, seed=3)
-    sol = oracle_solver(case, np.float32)
-    sol.run(n)
+    cl = cpu_ref.from_case(case, nthreads=threads)
+    cl.run(n)
     with contextlib.redirect_stdout(io.StringIO()):
         cases.initialise(case)
     st = cases.stepper(case)
     st.run(n)
     lbf = case["lbf"]
     for name in ("f", "rho", "vel", "total_rho", "SCforce", "body_force"):
-        check(name, getattr(lbf, name).to_numpy(), getattr(sol.lat, name))
-    check("psi", lbf.sc_field.psi_field.to_numpy(), sol.lat.psi)
+        check(name, getattr(lbf, name).to_numpy(), getattr(cl, name))
+    check("psi", lbf.sc_field.psi_field.to_numpy(), cl.psi)
     assert st.reduce()["nonfinite"] == 0
 
 
