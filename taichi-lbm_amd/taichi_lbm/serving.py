@@ -27,6 +27,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
+from . import _hip as H
 from .batch import BatchRunner, LatticeSpec, footprint_bytes, plane_stride, stored_nodes
 from ._device import torch
 
@@ -153,15 +154,19 @@ class SolveServer:
                     continue
                 stats.peak_resident = max(stats.peak_resident, len(self.resident))
                 stats.peak_bytes = max(stats.peak_bytes, self.used_bytes)
-                if self.changed and self.graph is not None:
+                if self.changed and self.graph is not None and not self._graph_holds():
                     self.graph.close()      # releases the batch's tables for the new tiling
                     self.graph = None
                 self.changed = False
                 if self.graph is None:
-                    # first tick after a change runs eagerly (fresh lattices read
-                    # in place), then the tick is captured at the tiling's size
+                    # the tick runs eagerly, then is captured with room for
+                    # about a quarter more tiles: later changes to the resident
+                    # set are committed into the captured tables (a fresh
+                    # lattice reads in place at its first launch) until they
+                    # outgrow them or change the kernel variant
                     self.runner.run(self.tick_steps)
-                    self.runner.fit_tables()
+                    tiles = H.lib().lbm_batch_tiles2(self.runner.h)
+                    self.runner.fit_tables(margin=max(64, tiles // 4))
                     self.graph = self.runner.capture(self.tick_steps)
                 else:
                     self.graph.replay()
@@ -180,6 +185,14 @@ class SolveServer:
                     on_tick(self, stats)
         stats.wall = clock() - t0
         return stats
+
+    def _graph_holds(self) -> bool:
+        """Commits the changed resident set; True if the captured tick still
+        runs it (same kernel variant, tiles within the captured tables)."""
+        try:
+            return self.runner._kernel_mode() == self.graph.mode
+        except H.LBMError:      # the two-step table is full after the capture
+            return False
 
     def close(self):
         if self.graph is not None:
