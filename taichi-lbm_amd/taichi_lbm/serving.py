@@ -59,6 +59,7 @@ class ServeStats:
     completed: List[Request] = field(default_factory=list)
     wall: float = 0.0
     ticks: int = 0
+    captures: int = 0        # graphs captured (a change that fits the captured tick needs none)
     peak_resident: int = 0
     peak_bytes: int = 0
 
@@ -74,6 +75,7 @@ class ServeStats:
             "failed": int(sum(r.failed for r in self.completed)),
             "rejected": int(sum(r.rejected is not None for r in self.completed)),
             "ticks": self.ticks,
+            "captures": self.captures,
             "peak_resident": self.peak_resident,
             "peak_resident_bytes": self.peak_bytes,
         }
@@ -168,6 +170,7 @@ class SolveServer:
                     tiles = H.lib().lbm_batch_tiles2(self.runner.h)
                     self.runner.fit_tables(margin=max(64, tiles // 4))
                     self.graph = self.runner.capture(self.tick_steps)
+                    stats.captures += 1
                 else:
                     self.graph.replay()
                 stats.ticks += 1
